@@ -1,0 +1,108 @@
+/*
+ * lbmib_oracle.h -- CPU restatement of the reference solver's per-timestep
+ * hot path (D2Q9 MRT lattice Boltzmann + implicit velocity-correction
+ * immersed boundary + characteristic/NEE outer boundaries + prescribed
+ * marker kinematics).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (liblbmib.so) never links, loads or calls anything in oracle/.
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic is
+ * plain IEEE double, evaluated left to right exactly as the Fortran source is
+ * written, compiled with -ffp-contract=off (no FMA contraction) so that the
+ * GPU path, built under the same rule, can be compared bit for bit.
+ */
+#ifndef LBMIB_ORACLE_H
+#define LBMIB_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    int nx, ny;            /* lattice nodes, mesh%Nx, mesh%Ny (ini_mesh.f90:22-23) */
+    double ox, oy;         /* physical origin of node (1,1) (ini_mesh.f90:36-37) */
+    double scale;          /* lattice spacing in chord units, mesh%scale (ini_mesh.f90:21) */
+    double U;              /* inflow lattice velocity UU (main.f90:11-12) */
+    double xc, yc;         /* body reference point (main.f90:85) */
+    double xp, yp;         /* rotation axis offset applied to the polygon (main.f90:86, airfoil.f90:240-241) */
+    double s[9];           /* MRT relaxation rates mesh%s (ini_mesh.f90:59-67) */
+    int smooth;            /* 1: smoothed 3-point kernel, 0: Peskin 4-point (boundary_condition.f90:1530,1563-1600) */
+    int n_wings;           /* wing_count (main.f90:18) */
+    int pts_per_wing;      /* boundary_points_per_wing, closed polygon (main.f90:21) */
+    int solver;            /* 0: SPD Cholesky of D D^T in zig-zag order; 1: dense LU with partial pivoting of D D^T S */
+    int dense_ib;          /* 1: literal dense Nseg x Nadj matrices; 0: box-sparse evaluation (bit-identical) */
+    int n_threads;         /* OpenMP threads for the lattice passes (results do not depend on it) */
+} orc_params;
+
+typedef struct orc_case orc_case;
+
+/* init (solution.f90:1-72) + read_boundary's post-processing (airfoil.f90:235-241)
+ * + build_motion_step_table (mode.f90:245-269).
+ * xb0/yb0: n_wings*pts_per_wing polygon points in chord units, body frame, as
+ * read with '(2f14.8)' (airfoil.f90:229).  motion: rows x ncols row-major table
+ * in the motion_params.dat column order (mode.f90:161-172), ncols = 1+9*n_wings.
+ * Returns NULL and fills err (>=256 bytes) on invalid input. */
+orc_case* orc_create(const orc_params* p, const double* xb0, const double* yb0,
+                     const double* motion, int rows, int ncols, char* err);
+void orc_destroy(orc_case* c);
+
+/* Run nsteps timesteps of the main loop body (main.f90:352-359): erranalysis
+ * when n >= 2, then evolution(n).  Returns 0, or <0 on a solver failure. */
+int orc_step(orc_case* c, int nsteps);
+
+/* Steps taken so far, motion_end_time, and the reference's loop bound
+ * (the last n with (n-1)*scale*U <= end_time + 1e-12, main.f90:352-353). */
+int orc_steps_done(const orc_case* c);
+int orc_max_steps(const orc_case* c);
+
+/* Fields (Fortran column-major, x fastest): rho,u,v: nx*ny; f,f0: 9*nx*ny
+ * population-planar.  Any pointer may be NULL. */
+void orc_get_fields(const orc_case* c, double* rho, double* u, double* v,
+                    double* f, double* f0);
+/* Overwrite the populations f (9*nx*ny) -- test hook (restart upload). */
+void orc_set_f(orc_case* c, const double* f);
+
+/* Force-row history (solution.f90:349-376): row n holds time, per wing
+ * (Cx,Cy,CM), totals (Cx,Cy,CM), per wing 9 kinematics = 4+12*nw columns. */
+int orc_history_cols(const orc_case* c);
+int orc_get_history(const orc_case* c, double* rows, int max_rows);
+
+/* erranalysis value err (solution.f90:74-127) as of the last step. */
+double orc_err(const orc_case* c);
+
+/* IB diagnostics of the last step: node count, node list (0-based i,j,wing),
+ * glx/gly (velocity corrections), rho at nodes, marker midpoints. */
+int orc_ib_count(const orc_case* c);
+void orc_ib_nodes(const orc_case* c, int* pi, int* pj, int* wing,
+                  double* glx, double* gly, double* rho_adj);
+int orc_n_segments(const orc_case* c);
+void orc_markers(const orc_case* c, double* xb2, double* yb2, double* ub, double* vb,
+                 double* arc_l);
+/* Rebuild the IB node set from given midpoints (test hook on move_boundary). */
+int orc_build_ib_set(const orc_case* c, const double* xb2, const double* yb2,
+                     int* pi, int* pj, int* wing, int max_nodes);
+
+/* ---- component-level functions (unit tests) ---- */
+void orc_weights(double w[9]);                    /* D2Q9model.f90:24-32 (REAL(4) weights) */
+void orc_minv(double minv[81]);                   /* nizhen on matm (solution.f90:17-27,1462-1517), row-major */
+double orc_asin(double x);                        /* asin used by the delta kernel */
+double orc_delta(double r, int smooth);           /* one-axis kernel (boundary_condition.f90:1563-1600) */
+double orc_cs(void);                              /* sqrt(3.0)/3.0d0 with REAL(4) sqrt (boundary_condition.f90:304) */
+double orc_feq(double rh, double u1, double v1, int k);  /* D2Q9feq (D2Q9model.f90:35-46), k 0-based */
+/* Motion step table (mode.f90:245-269) on a dt grid: out has nsteps x ncols. */
+int orc_motion_step_table(const double* motion, int rows, int ncols, double dt,
+                          double* out, int max_steps);
+/* Solve the IB system for a given dense symmetric G (n x n row-major), rhs b
+ * (n x 2, column k at b+k*n), scaling S (arc lengths), with solver 0 (SPD
+ * Cholesky in the given order perm) or 1 (LU of G*S with partial pivoting).
+ * Result y satisfies G y = b (solver 0) or the c = S^-1 y equivalent is
+ * returned already multiplied by S (solver 1), i.e. both return y = G^-1 b. */
+int orc_solve(const double* G, const double* S, const double* b, int n,
+              const int* perm, int solver, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
