@@ -1,0 +1,111 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep hot path on one MI355X.
+
+One step = one timestep of the FP64 drop-in path on the 4096x2048 workload
+(NACA0012, chord 256 lattice units, 1024 markers, Re 1000, plunge 0.5c with
+30 deg pitch leading by 90 deg, Strouhal 0.3, U = 0.05): marker kinematics,
+immersed-boundary interpolate/solve/spread/force, fused stream + MRT collide,
+outer boundaries.  The lattice starts resident in HBM; the timed region is
+exactly K steps between two device synchronisations.
+
+value = lattice updates per second of whole steps (MLUPS, nx*ny per step).
+roofline = the fused stream+collide kernel (the dominant one): 144 algorithmic
+bytes per interior node (9 FP64 populations read + 9 written) over its mean
+launch time, timed with HIP events on the stream it runs on.
+cpu_baseline = the CPU oracle (C restatement of the reference step, OpenMP
+over the host cores) on the first steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE_F64 = 144
+
+
+def cpu_baseline(case, seconds_budget=20.0, threads=None):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = threads or min(16, os.cpu_count() or 1)
+    o = pyoracle.Oracle(case, n_threads=threads)
+    cells = case.params.nx * case.params.ny
+    t0 = time.time()
+    o.step(1)
+    t1 = time.time()
+    per = t1 - t0
+    n = max(1, min(int(seconds_budget / max(per, 1e-3)), 200))
+    t2 = time.time()
+    o.step(n)
+    dt = time.time() - t2
+    o.close()
+    return {"value": round(cells * n / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "%d timesteps (after 1 untimed) of the same %dx%d workload from rest, "
+                      "oracle/liblbmib_oracle.so, %d OpenMP threads" % (n, case.params.nx, case.params.ny, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one case on one GPU; --gpus %d is not supported "
+              "(the path shards by case across GPUs, see the design notes)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    K, W = args.steps, args.warmup
+    case = cases.config(args.config, n_steps=K + W + 8)
+    p = case.params
+    g = lbmib.LBMIB(case, device=0)
+    if W:
+        g.evolution(W)
+    g.set_profiling(True)
+    t0 = time.perf_counter()
+    g.evolution(K)  # synchronises the device before returning
+    dt = time.perf_counter() - t0
+    tm = g.timings()
+    cells = p.nx * p.ny
+    interior = (p.nx - 2) * (p.ny - 2)
+    lat_s = tm["lattice_ms"] / tm["steps"] * 1e-3
+    achieved = interior * BYTES_PER_NODE_F64 / lat_s / 1e9
+    value = cells * K / dt / 1e6
+    out = {
+        "metric": "MLUPS (whole FP64 timestep incl. immersed boundary)",
+        "value": round(value, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(dt / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA0012 markers and sinusoidal kinematics)",
+        "config": {"workload": "%s: %dx%d lattice, NACA0012 chord %g lu, %d markers, Re 1000, "
+                               "plunge 0.5c + pitch 30deg (90deg lead), St 0.3, U 0.05, FP64 drop-in"
+                               % (case.name, p.nx, p.ny, 1.0 / p.scale, p.pts_per_wing - 1),
+                   "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_collide_f64", "bytes_per_launch": interior * BYTES_PER_NODE_F64,
+                     "mean_launch_ms": round(lat_s * 1e3, 5)},
+        "whole_step_hbm_frac": round(value * 1e6 * BYTES_PER_NODE_F64 / (HBM_PEAK_GBS * 1e9), 4),
+        "phase_ms_per_step": {k: round(v / tm["steps"], 5) for k, v in tm.items() if k != "steps"},
+    }
+    g.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

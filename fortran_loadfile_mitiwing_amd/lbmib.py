@@ -1,0 +1,227 @@
+"""Python binding of liblbmib (ctypes over the C ABI in include/lbmib.h).
+
+The class :class:`LBMIB` mirrors the reference solver's per-timestep interface
+(module state + subroutines, reference ``main.f90`` / ``solution.f90``):
+
+=====================  ==========================================================
+reference              this binding
+=====================  ==========================================================
+ini_mesh + init +      ``LBMIB(case)`` -> ``lbmib_create`` (+ the polygon of
+read_boundary          ``boundary.dat``)
+load_motion_table +    ``LBMIB(case)`` -> ``lbmib_set_motion_table``
+build_motion_step_table
+evolution(mesh, n)     ``evolution(n_steps=1)`` -> ``lbmib_step``
+erranalysis            ``erranalysis()`` -> ``lbmib_residual``
+force.dat rows         ``history()`` -> ``lbmib_get_history``
+output / saveflowfield ``fields()`` -> ``lbmib_download``
+readflowfield          ``upload(f)`` -> ``lbmib_upload_state``
+=====================  ==========================================================
+
+Errors raise :class:`LBMIBError` carrying the library's code and message
+(the reference stops the program instead).  The library is loaded from this
+package directory; if it is missing the import of this module raises, there is
+no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbmib.so")
+
+PREC_FP64 = 0
+PREC_FP32DEV = 1
+
+
+class LBMIBError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("lbmib error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32),
+                ("ox", C.c_double), ("oy", C.c_double), ("scale", C.c_double), ("U", C.c_double),
+                ("xc", C.c_double), ("yc", C.c_double), ("xp", C.c_double), ("yp", C.c_double),
+                ("s", C.c_double * 9), ("smooth", C.c_int32), ("precision", C.c_int32),
+                ("n_wings", C.c_int32), ("pts_per_wing", C.c_int32), ("device", C.c_int32),
+                ("nan_guard", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lib = None
+
+EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
+           "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
+           "lbmib_upload_state", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version"]
+
+
+def load(path: str = LIB_PATH):
+    """Load liblbmib.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("liblbmib.so not built at %s (run __graft_entry__.build())" % path)
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.lbmib_create.argtypes = [C.POINTER(Params), _dp, _dp, C.POINTER(vp)]
+    L.lbmib_set_motion_table.argtypes = [vp, _dp, C.c_int32, C.c_int32]
+    L.lbmib_step.argtypes = [vp, C.c_int32]
+    L.lbmib_steps_done.argtypes = [vp, _ip]
+    L.lbmib_max_steps.argtypes = [vp, _ip]
+    L.lbmib_get_forces.argtypes = [vp, _dp, _dp]
+    L.lbmib_get_history.argtypes = [vp, _dp, C.c_int32, _ip]
+    L.lbmib_download.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.lbmib_upload_state.argtypes = [vp, _dp]
+    L.lbmib_residual.argtypes = [vp, _dp]
+    L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
+    L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
+    L.lbmib_get_timings.argtypes = [vp, _dp]
+    L.lbmib_destroy.argtypes = [vp]
+    L.lbmib_last_error.restype = C.c_char_p
+    L.lbmib_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise LBMIBError(rc, load().lbmib_last_error().decode())
+
+
+def make_params(p, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0) -> Params:
+    prm = Params()
+    prm.nx, prm.ny = p.nx, p.ny
+    prm.ox, prm.oy, prm.scale, prm.U = p.ox, p.oy, p.scale, p.U
+    prm.xc, prm.yc, prm.xp, prm.yp = p.xc, p.yc, p.xp, p.yp
+    for k in range(9):
+        prm.s[k] = p.s[k]
+    prm.smooth, prm.precision = p.smooth, precision
+    prm.n_wings, prm.pts_per_wing = p.n_wings, p.pts_per_wing
+    prm.device, prm.nan_guard = device, nan_guard
+    return prm
+
+
+class LBMIB:
+    """One solver case resident on one GPU."""
+
+    def __init__(self, case, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0):
+        L = load()
+        self.case = case
+        self.p = case.params
+        prm = make_params(self.p, precision, device, nan_guard)
+        xb = np.ascontiguousarray(case.xb0, dtype=np.float64)
+        yb = np.ascontiguousarray(case.yb0, dtype=np.float64)
+        h = C.c_void_p()
+        _check(L.lbmib_create(C.byref(prm), _d(xb), _d(yb), C.byref(h)))
+        self.h = h
+        mt = np.ascontiguousarray(case.motion, dtype=np.float64)
+        _check(L.lbmib_set_motion_table(self.h, _d(mt), mt.shape[0], mt.shape[1]))
+        self.N = self.p.nx * self.p.ny
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lbmib_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference: evolution (solution.f90:269-410), n_steps times
+    def evolution(self, n_steps: int = 1):
+        _check(load().lbmib_step(self.h, int(n_steps)))
+
+    step = evolution
+
+    @property
+    def steps_done(self) -> int:
+        n = C.c_int32()
+        _check(load().lbmib_steps_done(self.h, C.byref(n)))
+        return n.value
+
+    def max_steps(self) -> int:
+        n = C.c_int32()
+        _check(load().lbmib_max_steps(self.h, C.byref(n)))
+        return n.value
+
+    # reference: erranalysis (solution.f90:74-127)
+    def erranalysis(self) -> float:
+        e = C.c_double()
+        _check(load().lbmib_residual(self.h, C.byref(e)))
+        return e.value
+
+    def forces(self):
+        nw = self.p.n_wings
+        wf = np.zeros(2 * nw)
+        wm = np.zeros(nw)
+        _check(load().lbmib_get_forces(self.h, _d(wf), _d(wm)))
+        return wf.reshape(nw, 2), wm
+
+    def history(self) -> np.ndarray:
+        n = C.c_int32()
+        _check(load().lbmib_get_history(self.h, None, 0, C.byref(n)))
+        cols = 4 + 12 * self.p.n_wings
+        out = np.zeros((n.value, cols))
+        if n.value:
+            _check(load().lbmib_get_history(self.h, _d(out), n.value, C.byref(n)))
+        return out
+
+    def fields(self, with_f: bool = True):
+        nx, ny = self.p.nx, self.p.ny
+        rho = np.zeros(self.N); u = np.zeros(self.N); v = np.zeros(self.N)
+        f = np.zeros(9 * self.N) if with_f else None
+        _check(load().lbmib_download(self.h, _d(rho), _d(u), _d(v), _d(f) if with_f else None))
+        out = dict(rho=rho.reshape(ny, nx), u=u.reshape(ny, nx), v=v.reshape(ny, nx))
+        if with_f:
+            out["f"] = f.reshape(9, ny, nx)
+        return out
+
+    def upload(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if f.size != 9 * self.N:
+            raise ValueError("f must have 9*nx*ny entries")
+        _check(load().lbmib_upload_state(self.h, _d(f)))
+
+    def ib_nodes(self):
+        n = C.c_int32()
+        L = load()
+        _check(L.lbmib_ib_nodes(self.h, C.byref(n), None, None, None, None, None))
+        m = n.value
+        pi = np.zeros(m, np.int32); pj = np.zeros(m, np.int32); w = np.zeros(m, np.int32)
+        gx = np.zeros(m); gy = np.zeros(m)
+        _check(L.lbmib_ib_nodes(self.h, C.byref(n), pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip),
+                                w.ctypes.data_as(_ip), _d(gx), _d(gy)))
+        return dict(i=pi, j=pj, wing=w, glx=gx, gly=gy)
+
+    def set_profiling(self, on: bool = True):
+        _check(load().lbmib_set_profiling(self.h, 1 if on else 0))
+
+    def timings(self) -> dict:
+        t = np.zeros(6)
+        _check(load().lbmib_get_timings(self.h, _d(t)))
+        return dict(step_ms=t[0], lattice_ms=t[1], ib_ms=t[2], ib_collide_ms=t[3],
+                    edges_ms=t[4], steps=int(t[5]))
+
+
+def version() -> str:
+    return load().lbmib_version().decode()
