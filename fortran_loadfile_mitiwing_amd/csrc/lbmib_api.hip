@@ -23,18 +23,21 @@ namespace lbmib {
 // kernels (lbmib_kernels.hip)
 __global__ void k_collide_f64(const double*, double*, int, int, long long, Rates, double*);
 __global__ void k_edges_f64(DevCase, int, double, double*);
-__global__ void k_markers(DevCase, int);
-__global__ void k_clear(DevCase);
-__global__ void k_claim(DevCase);
-__global__ void k_count(DevCase);
-__global__ void k_fill(DevCase);
-__global__ void k_nodes(DevCase, int);
-__global__ void k_dvals(DevCase);
-__global__ void k_assemble(DevCase);
-__global__ void k_chol(DevCase);
-__global__ void k_spread(DevCase);
-__global__ void k_band_f64(DevCase, int, double*);
-__global__ void k_forces(DevCase, int);
+__global__ void k_geo_clear(DevCase, int);
+__global__ void k_geo_markers(DevCase, int, int);
+__global__ void k_geo_fill(DevCase, int);
+__global__ void k_geo_phi(DevCase, int);
+__global__ void k_geo_pairs(DevCase, int);
+__global__ void k_geo_dsort(DevCase, int);
+__global__ void k_geo_nsort(DevCase, int);
+__global__ void k_geo_assemble(DevCase, int);
+__global__ void k_geo_factor(DevCase, int);
+__global__ void k_crit_rhs(DevCase, int, int);
+template <int WS> __global__ void k_crit_solve(DevCase, int);
+__global__ void k_band_f64(DevCase, int, int, double*);
+__global__ void k_forces(DevCase, int, int);
+size_t factor_smem_bytes();
+size_t solve_smem_bytes(int nseg);
 __global__ void k_residual(const double*, const double*, int, int, long long, double*);
 __global__ void k_nan_check(const double*, long long, int*);
 }  // namespace lbmib
@@ -172,8 +175,17 @@ void zigzag(int nw, int spw, std::vector<int>& perm)
 struct lbmib_handle {
     lbmib_params p;
     DevCase D;
-    hipStream_t s0 = nullptr, s1 = nullptr;
-    hipEvent_t ev_ib = nullptr, ev_step = nullptr;
+    // s0 lattice; s1 per-step IB solve; s2 IB geometry of later batches;
+    // s3 forces.  With LBMIB_IB_CUS > 0, s1-s3 run on that many reserved CUs.
+    static constexpr int NSTREAM = 4;
+    hipStream_t s[NSTREAM] = {};
+    static constexpr int RING = 64;  // per-step events of step n live in slot n % RING
+    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {};
+    static constexpr int BRING = 8;  // per-batch events of batch j live in slot j % BRING
+    hipEvent_t ev_built[BRING] = {}, ev_forces[BRING] = {};
+    int batch = 8;                   // steps per geometry batch (slots = 2 * batch)
+    int built_batches = 0;           // batches whose geometry has been enqueued
+    int forces_upto = 0;             // steps whose forces have been enqueued
     std::vector<double> xb0, yb0, arc_l;  // polygon after the (xc-xp, yc-yp) shift
     std::vector<int> perm, iperm;
     double cs = 0.0;
@@ -189,7 +201,7 @@ struct lbmib_handle {
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> evpool;
-    double tms[6] = {0, 0, 0, 0, 0, 0};
+    double tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int dev = 0;
 };
 
@@ -251,14 +263,18 @@ int lbmib_destroy(lbmib_handle* h)
 {
     if (!h) return 0;
     (void)hipSetDevice(h->dev);
-    if (h->s0) (void)hipStreamSynchronize(h->s0);
-    if (h->s1) (void)hipStreamSynchronize(h->s1);
+    for (hipStream_t st : h->s)
+        if (st) (void)hipStreamSynchronize(st);
     for (void* p : h->allocs) (void)hipFree(p);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-    if (h->ev_ib) (void)hipEventDestroy(h->ev_ib);
-    if (h->ev_step) (void)hipEventDestroy(h->ev_step);
-    if (h->s0) (void)hipStreamDestroy(h->s0);
-    if (h->s1) (void)hipStreamDestroy(h->s1);
+    for (int i = 0; i < lbmib_handle::RING; ++i)
+        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i]})
+            if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < lbmib_handle::BRING; ++i)
+        for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]})
+            if (e) (void)hipEventDestroy(e);
+    for (hipStream_t st : h->s)
+        if (st) (void)hipStreamDestroy(st);
     delete h;
     return 0;
 }
@@ -273,6 +289,8 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     if (!(p->scale > 0.0)) return fail(LBMIB_E_ARG, "scale must be positive");
     if (p->precision != LBMIB_PREC_FP64) return fail(LBMIB_E_ARG, "precision %d not available in this build", p->precision);
     if (p->smooth != 0 && p->smooth != 1) return fail(LBMIB_E_ARG, "smooth must be 0 or 1");
+    if ((long long)p->n_wings * (p->pts_per_wing - 1) > MAXSEG)
+        return fail(LBMIB_E_ARG, "at most %d marker segments supported", MAXSEG);
 
     // the kernels carry nizhen's inverse as constants; check them bit for bit
     double minv_host[81], minv_dev[81];
@@ -290,24 +308,52 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     h->p = *p;
     h->dev = p->device;
     int rc = 0;
-#define TRY(x)                         \
-    do {                               \
-        rc = (x);                      \
+#define TRY(x)                                   \
+    do {                                         \
+        rc = (x);                                \
         if (rc) { lbmib_destroy(h); return rc; } \
     } while (0)
-#define HTRY(x)                                                                                 \
-    do {                                                                                        \
-        hipError_t e_ = (x);                                                                    \
-        if (e_ != hipSuccess) {                                                                 \
-            lbmib_destroy(h);                                                                   \
-            return fail(LBMIB_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_));               \
-        }                                                                                       \
+#define HTRY(x)                                                                   \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            lbmib_destroy(h);                                                     \
+            return fail(LBMIB_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+        }                                                                         \
     } while (0)
     HTRY(hipSetDevice(h->dev));
-    HTRY(hipStreamCreateWithFlags(&h->s0, hipStreamNonBlocking));
-    HTRY(hipStreamCreateWithFlags(&h->s1, hipStreamNonBlocking));
-    HTRY(hipEventCreateWithFlags(&h->ev_ib, hipEventDisableTiming));
-    HTRY(hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming));
+    {
+        // Optional CU split: the IB streams get LBMIB_IB_CUS CUs, the lattice
+        // stream the rest.  The mask bits are spread over the accelerator's 8
+        // XCDs and a queue whose mask leaves an XCD without CUs is not usable,
+        // so only whole multiples of 8 are taken.
+        hipDeviceProp_t prop;
+        HTRY(hipGetDeviceProperties(&prop, h->dev));
+        const int ncu = prop.multiProcessorCount;
+        int ib_cus = 0;
+        if (const char* e = getenv("LBMIB_IB_CUS")) ib_cus = atoi(e);
+        ib_cus = (ib_cus + 7) / 8 * 8;
+        if (ib_cus > 0 && ib_cus < ncu / 2) {
+            const int nwords = (ncu + 31) / 32;
+            std::vector<uint32_t> lat(nwords, 0u), ib(nwords, 0u);
+            for (int c = 0; c < ncu; ++c) {
+                if (c >= ncu - ib_cus) ib[c / 32] |= 1u << (c % 32);
+                else lat[c / 32] |= 1u << (c % 32);
+            }
+            HTRY(hipExtStreamCreateWithCUMask(&h->s[0], nwords, lat.data()));
+            for (int i = 1; i < lbmib_handle::NSTREAM; ++i)
+                HTRY(hipExtStreamCreateWithCUMask(&h->s[i], nwords, ib.data()));
+        } else {
+            for (int i = 0; i < lbmib_handle::NSTREAM; ++i)
+                HTRY(hipStreamCreateWithFlags(&h->s[i], hipStreamNonBlocking));
+        }
+    }
+    for (int i = 0; i < lbmib_handle::RING; ++i)
+        for (hipEvent_t* e : {&h->ev_ib[i], &h->ev_step[i]}) HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (int i = 0; i < lbmib_handle::BRING; ++i)
+        for (hipEvent_t* e : {&h->ev_built[i], &h->ev_forces[i]})
+            HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    if (const char* e = getenv("LBMIB_GEO_BATCH")) h->batch = std::max(1, std::min(64, atoi(e)));
 
     const int nx = p->nx, ny = p->ny, nw = p->n_wings, ppw = p->pts_per_wing;
     const long long N = (long long)nx * ny;
@@ -319,6 +365,7 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     D.smooth = p->smooth;
     D.nw = nw; D.ppw = ppw; D.spw = ppw - 1; D.nseg = nw * (ppw - 1);
     D.cap = D.nseg * BOXMAX;
+    D.nslot = 2 * h->batch;
     h->cs = (double)std::sqrt(3.0f) / 3.0;  // REAL(4) sqrt (boundary_condition.f90:304)
 
     // polygon after read_boundary's shift (airfoil.f90:235-241)
@@ -367,29 +414,14 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
         std::fill(plane.begin(), plane.end(), 0.0);
         HTRY(hipMemcpy(D.macro[0] + 2 * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     }
-    // immersed boundary
+    // markers
     double *dxb0, *dyb0;
     TRY(dalloc(h, &dxb0, npts));
     TRY(dalloc(h, &dyb0, npts));
     HTRY(hipMemcpy(dxb0, h->xb0.data(), sizeof(double) * npts, hipMemcpyHostToDevice));
     HTRY(hipMemcpy(dyb0, h->yb0.data(), sizeof(double) * npts, hipMemcpyHostToDevice));
     D.xb0 = dxb0; D.yb0 = dyb0;
-    const int ns = D.nseg, cap = D.cap;
-    TRY(dalloc(h, &D.xb2, ns)); TRY(dalloc(h, &D.yb2, ns));
-    TRY(dalloc(h, &D.ub, ns)); TRY(dalloc(h, &D.vb, ns));
-    TRY(dalloc(h, &D.box, 4 * (size_t)ns));
-    TRY(dalloc(h, &D.claim, (size_t)N));
-    TRY(dalloc(h, &D.nodeidx, (size_t)N));
-    TRY(dalloc(h, &D.segcnt, ns));
-    TRY(dalloc(h, &D.adj, 3 * (size_t)cap));
-    TRY(dalloc(h, &D.nadj, 1));
-    TRY(dalloc(h, &D.ustar, cap)); TRY(dalloc(h, &D.vstar, cap)); TRY(dalloc(h, &D.rhoadj, cap));
-    TRY(dalloc(h, &D.dcnt, ns));
-    TRY(dalloc(h, &D.didx, (size_t)ns * BOXMAX));
-    TRY(dalloc(h, &D.dval, (size_t)ns * BOXMAX));
-    TRY(dalloc(h, &D.nmkcnt, cap));
-    TRY(dalloc(h, &D.nmk, (size_t)cap * NCAP));
-    TRY(dalloc(h, &D.nmkv, (size_t)cap * NCAP));
+    const int ns = D.nseg;
     int *dperm, *diperm;
     TRY(dalloc(h, &dperm, ns)); TRY(dalloc(h, &diperm, ns));
     HTRY(hipMemcpy(dperm, h->perm.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
@@ -397,17 +429,16 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     D.perm = dperm; D.iperm = diperm;
     TRY(dalloc(h, &D.rhs, 2 * (size_t)ns));
     TRY(dalloc(h, &D.y, 2 * (size_t)ns));
-    TRY(dalloc(h, &D.glx, cap)); TRY(dalloc(h, &D.gly, cap));
     TRY(dalloc(h, &D.err, 1));
     TRY(dalloc(h, &D.red, 2 * 4096));
-    TRY(fill_i32(D.claim, INT_MAX, (size_t)N, h->s0));
-    TRY(fill_i32(D.nodeidx, -1, (size_t)N, h->s0));
-    TRY(fill_i32(D.nadj, 0, 1, h->s0));
-    TRY(fill_i32(D.err, 0, 1, h->s0));
+    TRY(fill_i32(D.err, 0, 1, h->s[0]));
     D.bw = geometric_bandwidth(h);
     if (D.bw >= ns) D.bw = ns - 1;
-    HTRY(hipEventRecord(h->ev_step, h->s0));
-    HTRY(hipStreamSynchronize(h->s0));
+    for (int i = 0; i < lbmib_handle::RING; ++i)
+        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i]}) HTRY(hipEventRecord(e, h->s[0]));
+    for (int i = 0; i < lbmib_handle::BRING; ++i)
+        for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]}) HTRY(hipEventRecord(e, h->s[0]));
+    HTRY(hipStreamSynchronize(h->s[0]));
     *out = h;
     return 0;
 #undef TRY
@@ -423,7 +454,7 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
     for (int r = 1; r < nrows; ++r)
         if (rows[(size_t)r * ncols] < rows[(size_t)(r - 1) * ncols])
             return fail(LBMIB_E_ARG, "motion table time column must be non-decreasing (row %d)", r + 1);
-    if (h->steps_done > 0) return fail(LBMIB_E_STATE, "motion table must be set before the first step");
+    if (h->have_motion) return fail(LBMIB_E_STATE, "the motion table can be set once per handle");
     HIP_TRY(hipSetDevice(h->dev));
     MotionTable m;
     m.t.assign(rows, rows + (size_t)nrows * ncols);
@@ -459,21 +490,36 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
             k[6] = c[7];  // sbv
         }
     }
-    // wings that may approach each other couple across their blocks
-    if (nw > 1) {
-        double margin = 6.0 * h->p.scale;
+    DevCase& D = h->D;
+    // lattice window holding every IB node of every step, and whether wings
+    // ever come within coupling distance of each other
+    {
+        const int ppw = h->p.pts_per_wing;
+        double bx0[8], bx1[8], by0[8], by1[8];
+        for (int w = 0; w < nw; ++w) {
+            bx0[w] = by0[w] = 1e300; bx1[w] = by1[w] = -1e300;
+            for (int i = 0; i < ppw; ++i) {
+                bx0[w] = std::min(bx0[w], h->xb0[w * ppw + i]); bx1[w] = std::max(bx1[w], h->xb0[w * ppw + i]);
+                by0[w] = std::min(by0[w], h->yb0[w * ppw + i]); by1[w] = std::max(by1[w], h->yb0[w * ppw + i]);
+            }
+        }
+        const double margin = 6.0 * h->p.scale;
+        double X0 = 1e300, X1 = -1e300, Y0 = 1e300, Y1 = -1e300;
         bool close = false;
-        for (int n = 0; n < maxn && !close; ++n) {
+        for (int n = 0; n < maxn; ++n) {
             double lo[8][2], hi[8][2];
             for (int w = 0; w < nw; ++w) {
                 const double* k = &kin[((size_t)n * nw + w) * KIN];
                 lo[w][0] = lo[w][1] = 1e300; hi[w][0] = hi[w][1] = -1e300;
-                for (int i = 0; i < h->p.pts_per_wing; ++i) {
-                    double x = h->xb0[w * h->p.pts_per_wing + i], y = h->yb0[w * h->p.pts_per_wing + i];
-                    double X = k[1] * x + k[0] * y + k[3], Y = -k[0] * x + k[1] * y + k[5];
+                const double cx[4] = {bx0[w], bx1[w], bx0[w], bx1[w]}, cy[4] = {by0[w], by0[w], by1[w], by1[w]};
+                for (int q = 0; q < 4; ++q) {
+                    double X = k[1] * cx[q] + k[0] * cy[q] + k[3] + h->p.xc;
+                    double Y = -k[0] * cx[q] + k[1] * cy[q] + k[5] + h->p.yc;
                     lo[w][0] = std::min(lo[w][0], X); hi[w][0] = std::max(hi[w][0], X);
                     lo[w][1] = std::min(lo[w][1], Y); hi[w][1] = std::max(hi[w][1], Y);
                 }
+                X0 = std::min(X0, lo[w][0]); X1 = std::max(X1, hi[w][0]);
+                Y0 = std::min(Y0, lo[w][1]); Y1 = std::max(Y1, hi[w][1]);
             }
             for (int a = 0; a < nw && !close; ++a)
                 for (int b = 0; b < a && !close; ++b)
@@ -481,61 +527,181 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
                         lo[a][1] - margin <= hi[b][1] && lo[b][1] - margin <= hi[a][1])
                         close = true;
         }
-        if (close) h->D.bw = h->D.nseg - 1;
+        if (close) D.bw = D.nseg - 1;
+        const double sc = h->p.scale;
+        int i0 = (int)std::floor((X0 - h->p.ox) / sc) - 6, i1 = (int)std::ceil((X1 - h->p.ox) / sc) + 6;
+        int j0 = (int)std::floor((Y0 - h->p.oy) / sc) - 6, j1 = (int)std::ceil((Y1 - h->p.oy) / sc) + 6;
+        i0 = std::max(0, i0); j0 = std::max(0, j0);
+        i1 = std::min(D.nx - 1, i1); j1 = std::min(D.ny - 1, j1);
+        if (i1 < i0 || j1 < j0) return fail(LBMIB_E_GEOMETRY, "the body never overlaps the lattice");
+        D.gi0 = i0; D.gj0 = j0; D.gw = i1 - i0 + 1; D.gh = j1 - j0 + 1;
     }
-    DevCase& D = h->D;
+    if (D.bw > MAXBW)
+        return fail(LBMIB_E_ARG, "immersed-boundary bandwidth %d exceeds the banded solver's limit of %d "
+                    "(wings too close together, or markers much denser than the lattice)", D.bw, MAXBW);
+    HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)factor_smem_bytes()));
+    HIP_TRY(hipFuncSetAttribute((const void*)k_crit_solve<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)solve_smem_bytes(D.nseg)));
+    HIP_TRY(hipFuncSetAttribute((const void*)k_crit_solve<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)solve_smem_bytes(D.nseg)));
     double* dkin;
     if (dalloc(h, &dkin, kin.size())) return LBMIB_E_HIP;
     HIP_TRY(hipMemcpy(dkin, kin.data(), sizeof(double) * kin.size(), hipMemcpyHostToDevice));
     D.kin = dkin;
     D.steps_cap = maxn;
     if (dalloc(h, &D.hist, (size_t)maxn * 3 * nw)) return LBMIB_E_HIP;
-    if (dalloc(h, &D.G, (size_t)D.nseg * (D.bw + 1))) return LBMIB_E_HIP;
+    // slot pools
+    {
+        const long long ns = D.nseg, cap = D.cap, W = D.bw + 1, grid = (long long)D.gw * D.gh;
+        auto al = [](long long x) { return (x + 63) / 64 * 64; };
+        SlotInts& io = D.io;
+        long long o = 0;
+        io.claim = o; o += al(grid);
+        io.nodeidx = o; o += al(grid);
+        io.box = o; o += al(4 * ns);
+        io.draw_idx = o; o += al(ns * BOXMAX);
+        io.nraw_s = o; o += al(cap * NCAP);
+        io.adj = o; o += al(3 * cap);
+        io.nadj = o; o += al(1);
+        io.dcnt = o; o += al(ns);
+        io.didx = o; o += al(ns * BOXMAX);
+        io.nmkcnt = o; o += al(cap);
+        io.nmk = o; o += al(cap * NCAP);
+        io.total = o;
+        SlotDoubles& dd = D.dof;
+        o = 0;
+        dd.xb2 = o; o += al(ns);
+        dd.yb2 = o; o += al(ns);
+        dd.ub = o; o += al(ns);
+        dd.vb = o; o += al(ns);
+        dd.phi = o; o += al(12 * ns);
+        dd.draw_val = o; o += al(ns * BOXMAX);
+        dd.nraw_w = o; o += al(cap * NCAP);
+        dd.dval = o; o += al(ns * BOXMAX);
+        dd.nmkv = o; o += al(cap * NCAP);
+        dd.G = o; o += al(ns * W);
+        dd.L = o; o += al(ns * W);
+        dd.LT = o; o += al(ns * W);
+        dd.rhoadj = o; o += al(cap);
+        dd.glx = o; o += al(cap);
+        dd.gly = o; o += al(cap);
+        dd.total = o;
+        if (dalloc(h, &D.ipool, (size_t)D.nslot * io.total)) return LBMIB_E_HIP;
+        if (dalloc(h, &D.dpool, (size_t)D.nslot * dd.total)) return LBMIB_E_HIP;
+        for (int g = 0; g < D.nslot; ++g) {
+            Slot S = slot_of(D, g);
+            if (fill_i32(S.claim, INT_MAX, (size_t)grid, h->s[0])) return LBMIB_E_HIP;
+            if (fill_i32(S.nodeidx, -1, (size_t)grid, h->s[0])) return LBMIB_E_HIP;
+            if (fill_i32(S.nadj, 0, 1, h->s[0])) return LBMIB_E_HIP;
+        }
+    }
     h->have_motion = true;
+    return 0;
+}
+
+// Geometry of batch j (steps j*B+1 .. j*B+B, slots (j*B .. j*B+B-1) % nslot)
+// on s2: node sets, kernel weights, bands of D D^T and their factors, one
+// slot per step in parallel.  The slots were last used by batch j-2, which
+// must have finished its IB collides and forces.
+static int enqueue_batch(lbmib_handle* h, int j, hipEvent_t* ev)
+{
+    DevCase& D = h->D;
+    const int B = h->batch;
+    const int n0 = j * B + 1;
+    const int nb = std::min(B, h->max_steps - n0 + 1);
+    if (nb <= 0) return 0;
+    const int g0 = (n0 - 1) % D.nslot;
+    hipStream_t st = h->s[2];
+    if (j >= 2) {
+        const int last = (j - 1) * B;  // last step of batch j-2
+        HIP_TRY(hipStreamWaitEvent(st, h->ev_step[last % lbmib_handle::RING], 0));
+        HIP_TRY(hipStreamWaitEvent(st, h->ev_forces[(j - 2) % lbmib_handle::BRING], 0));
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[7], st));
+    const int ns = D.nseg, cap = D.cap;
+    hipLaunchKernelGGL(k_geo_clear, dim3((cap + 255) / 256, nb), dim3(256), 0, st, D, g0);
+    hipLaunchKernelGGL(k_geo_markers, dim3((ns + 255) / 256, nb), dim3(256), 0, st, D, g0, n0 - 1);
+    hipLaunchKernelGGL(k_geo_fill, dim3(nb), dim3(1024), 0, st, D, g0);
+    hipLaunchKernelGGL(k_geo_phi, dim3((ns * 12 + 255) / 256, nb), dim3(256), 0, st, D, g0);
+    hipLaunchKernelGGL(k_geo_pairs, dim3((ns * BOXMAX + 255) / 256, nb), dim3(256), 0, st, D, g0);
+    hipLaunchKernelGGL(k_geo_dsort, dim3((ns * BOXMAX + 255) / 256, nb), dim3(256), 0, st, D, g0);
+    hipLaunchKernelGGL(k_geo_nsort, dim3(64, nb), dim3(256), 0, st, D, g0);
+    long long nbw = (long long)ns * (D.bw + 1);
+    hipLaunchKernelGGL(k_geo_assemble, dim3((unsigned)((nbw + 255) / 256), nb), dim3(256), 0, st, D, g0);
+    if (ev) HIP_TRY(hipEventRecord(ev[8], st));
+    if (ev) HIP_TRY(hipEventRecord(ev[9], st));
+    hipLaunchKernelGGL(k_geo_factor, dim3(nb), dim3(256), factor_smem_bytes(), st, D, g0);
+    if (ev) HIP_TRY(hipEventRecord(ev[10], st));
+    HIP_TRY(hipEventRecord(h->ev_built[j % lbmib_handle::BRING], st));
+    h->built_batches = j + 1;
+    return 0;
+}
+
+// Forces of steps first..last (one workgroup each) on s3.
+static int enqueue_forces(lbmib_handle* h, int first, int last)
+{
+    if (last < first) return 0;
+    DevCase& D = h->D;
+    hipStream_t st = h->s[3];
+    HIP_TRY(hipStreamWaitEvent(st, h->ev_step[last % lbmib_handle::RING], 0));
+    hipLaunchKernelGGL(k_forces, dim3(last - first + 1), dim3(256), 0, st, D, (first - 1) % D.nslot, first - 1);
+    const int j = (last - 1) / h->batch;
+    HIP_TRY(hipEventRecord(h->ev_forces[j % lbmib_handle::BRING], st));
+    h->forces_upto = last;
     return 0;
 }
 
 static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
 {
     DevCase& D = h->D;
-    const int cur = n & 1;
+    const int B = h->batch, R = lbmib_handle::RING;
+    const int cur = n & 1, g = (n - 1) % D.nslot, j = (n - 1) / B;
     const int nx = D.nx, ny = D.ny;
-    const int step = n - 1;
     double* macro = flag ? D.macro[cur] : nullptr;
     const int ns = D.nseg, cap = D.cap;
-    // IB chain on s1, after the previous step completed on s0
-    HIP_TRY(hipStreamWaitEvent(h->s1, h->ev_step, 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], h->s1));
-    hipLaunchKernelGGL(k_markers, dim3((ns + 255) / 256), dim3(256), 0, h->s1, D, step);
-    hipLaunchKernelGGL(k_clear, dim3((cap + 255) / 256), dim3(256), 0, h->s1, D);
-    hipLaunchKernelGGL(k_claim, dim3((ns * BOXMAX + 255) / 256), dim3(256), 0, h->s1, D);
-    hipLaunchKernelGGL(k_count, dim3((ns + 255) / 256), dim3(256), 0, h->s1, D);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(1024), 0, h->s1, D);
-    hipLaunchKernelGGL(k_nodes, dim3((cap + 255) / 256), dim3(256), 0, h->s1, D, cur);
-    hipLaunchKernelGGL(k_dvals, dim3((ns + 63) / 64), dim3(64), 0, h->s1, D);
-    long long nb = (long long)ns * (D.bw + 1);
-    hipLaunchKernelGGL(k_assemble, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, h->s1, D);
-    hipLaunchKernelGGL(k_chol, dim3(1), dim3(1024), 0, h->s1, D);
-    hipLaunchKernelGGL(k_spread, dim3((cap + 255) / 256), dim3(256), 0, h->s1, D);
-    if (ev) HIP_TRY(hipEventRecord(ev[3], h->s1));
-    HIP_TRY(hipEventRecord(h->ev_ib, h->s1));
-    hipLaunchKernelGGL(k_forces, dim3(1), dim3(64), 0, h->s1, D, step);
-    // lattice on s0
-    if (ev) HIP_TRY(hipEventRecord(ev[0], h->s0));
-    hipLaunchKernelGGL(k_collide_f64, dim3((nx - 2 + 255) / 256, ny - 2), dim3(256), 0, h->s0,
-                       (const double*)D.f[cur ^ 1], D.f[cur], nx, ny, D.N, D.R, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[1], h->s0));
-    hipLaunchKernelGGL(k_edges_f64, dim3((ny - 2 + nx + 255) / 256), dim3(256), 0, h->s0, D, cur, h->cs, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[6], h->s0));
-    HIP_TRY(hipStreamWaitEvent(h->s0, h->ev_ib, 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[4], h->s0));
-    hipLaunchKernelGGL(k_band_f64, dim3((cap + 255) / 256), dim3(256), 0, h->s0, D, cur, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[5], h->s0));
-    HIP_TRY(hipEventRecord(h->ev_step, h->s0));
-    HIP_TRY(hipGetLastError());
-    if (flag) {
-        h->last_flag_step[cur] = n;
+    bool timed = false;
+    if ((n - 1) % B == 0) {
+        // a new batch starts: forces of the previous one, geometry of the next
+        if (h->forces_upto < n - 1) {
+            int rc = enqueue_forces(h, h->forces_upto + 1, n - 1);
+            if (rc) return rc;
+        }
+        for (int jj = h->built_batches; jj <= j + 1; ++jj) {
+            bool t = ev && jj == j + 1;
+            int rc = enqueue_batch(h, jj, t ? ev : nullptr);
+            if (rc) return rc;
+            timed |= t;
+        }
     }
+    if (ev && !timed)
+        for (int i = 7; i <= 10; ++i) HIP_TRY(hipEventRecord(ev[i], h->s[2]));
+    // per-step IB solve on s1: needs f of step n-1 and the geometry of batch j
+    hipStream_t s1 = h->s[1], s0 = h->s[0];
+    HIP_TRY(hipStreamWaitEvent(s1, h->ev_step[(n - 1) % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s1, h->ev_built[j % lbmib_handle::BRING], 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s1));
+    hipLaunchKernelGGL(k_crit_rhs, dim3((ns + 6) / 7), dim3(256), 0, s1, D, g, cur);
+    if (D.bw <= 30)
+        hipLaunchKernelGGL(k_crit_solve<32>, dim3(1), dim3(64), solve_smem_bytes(ns), s1, D, g);
+    else
+        hipLaunchKernelGGL(k_crit_solve<64>, dim3(1), dim3(64), solve_smem_bytes(ns), s1, D, g);
+    if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
+    HIP_TRY(hipEventRecord(h->ev_ib[n % R], s1));
+    // lattice on s0
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
+    hipLaunchKernelGGL(k_collide_f64, dim3((nx - 1 + 255) / 256, ny - 2), dim3(256), 0, s0,
+                       (const double*)D.f[cur ^ 1], D.f[cur], nx, ny, D.N, D.R, macro);
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
+    hipLaunchKernelGGL(k_edges_f64, dim3((ny - 2 + nx + 255) / 256), dim3(256), 0, s0, D, cur, h->cs, macro);
+    if (ev) HIP_TRY(hipEventRecord(ev[6], s0));
+    HIP_TRY(hipStreamWaitEvent(s0, h->ev_ib[n % R], 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[4], s0));
+    hipLaunchKernelGGL(k_band_f64, dim3((cap + 255) / 256), dim3(256), 0, s0, D, g, cur, macro);
+    if (ev) HIP_TRY(hipEventRecord(ev[5], s0));
+    HIP_TRY(hipEventRecord(h->ev_step[n % R], s0));
+    HIP_TRY(hipGetLastError());
+    if (flag) h->last_flag_step[cur] = n;
     return 0;
 }
 
@@ -547,9 +713,9 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
     if (h->steps_done + n_steps > h->max_steps)
         return fail(LBMIB_E_ARG, "step %d beyond the motion table (last step %d)", h->steps_done + n_steps, h->max_steps);
     HIP_TRY(hipSetDevice(h->dev));
-    const int E = 7;
-    if (h->profiling && (int)h->evpool.size() < E * n_steps) {
-        while ((int)h->evpool.size() < E * n_steps) {
+    const int E = 11;
+    if (h->profiling) {
+        while (h->evpool.size() < (size_t)E * n_steps + 2) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             h->evpool.push_back(e);
@@ -557,16 +723,9 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (h->profiling) {
-        if (h->evpool.size() < (size_t)E * n_steps + 2) {
-            for (int i = 0; i < 2; ++i) {
-                hipEvent_t e;
-                HIP_TRY(hipEventCreate(&e));
-                h->evpool.push_back(e);
-            }
-        }
         t0 = h->evpool[(size_t)E * n_steps];
         t1 = h->evpool[(size_t)E * n_steps + 1];
-        HIP_TRY(hipEventRecord(t0, h->s0));
+        HIP_TRY(hipEventRecord(t0, h->s[0]));
     }
     for (int s = 0; s < n_steps; ++s) {
         int n = h->steps_done + 1;
@@ -575,12 +734,13 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
         if (rc) return rc;
         h->steps_done = n;
     }
-    if (h->profiling) {
-        HIP_TRY(hipStreamWaitEvent(h->s0, h->ev_step, 0));
-        HIP_TRY(hipEventRecord(t1, h->s0));
+    if (h->profiling) HIP_TRY(hipEventRecord(t1, h->s[0]));
+    // forces of the steps taken so far (the history is complete on return)
+    if (h->forces_upto < h->steps_done) {
+        int rc = enqueue_forces(h, h->forces_upto + 1, h->steps_done);
+        if (rc) return rc;
     }
-    HIP_TRY(hipStreamSynchronize(h->s0));
-    HIP_TRY(hipStreamSynchronize(h->s1));
+    for (hipStream_t st : h->s) HIP_TRY(hipStreamSynchronize(st));
     int err = 0;
     HIP_TRY(hipMemcpy(&err, h->D.err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) {
@@ -598,14 +758,16 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
             HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); h->tms[2] += ms;
             HIP_TRY(hipEventElapsedTime(&ms, ev[4], ev[5])); h->tms[3] += ms;
             HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[6])); h->tms[4] += ms;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[7], ev[8])); h->tms[5] += ms;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[9], ev[10])); h->tms[7] += ms;
         }
-        h->tms[5] += n_steps;
+        h->tms[6] += n_steps;
     }
     if (h->p.nan_guard) {
         int* flag;
         if (dalloc(h, &flag, 1)) return LBMIB_E_HIP;
         HIP_TRY(hipMemset(flag, 0, sizeof(int)));
-        hipLaunchKernelGGL(k_nan_check, dim3(1024), dim3(256), 0, h->s0,
+        hipLaunchKernelGGL(k_nan_check, dim3(1024), dim3(256), 0, h->s[0],
                            (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
         int f = 0;
         HIP_TRY(hipMemcpy(&f, flag, sizeof(int), hipMemcpyDeviceToHost));
@@ -714,11 +876,11 @@ int lbmib_residual(lbmib_handle* h, double* err)
         return fail(LBMIB_E_STATE, "velocity fields of steps %d and %d were not recorded", n - 1, n);
     HIP_TRY(hipSetDevice(h->dev));
     const int blocks = 1024;
-    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, h->s0, (const double*)h->D.macro[sc],
+    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, h->s[0], (const double*)h->D.macro[sc],
                        (const double*)h->D.macro[sp], h->D.nx, h->D.ny, h->D.N, h->D.red);
     std::vector<double> part(2 * blocks);
-    HIP_TRY(hipMemcpyAsync(part.data(), h->D.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->s0));
-    HIP_TRY(hipStreamSynchronize(h->s0));
+    HIP_TRY(hipMemcpyAsync(part.data(), h->D.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->s[0]));
+    HIP_TRY(hipStreamSynchronize(h->s[0]));
     double t1 = 0.0, t2 = 0.0;
     for (int b = 0; b < blocks; ++b) { t1 += part[2 * b]; t2 += part[2 * b + 1]; }
     *err = std::sqrt(t1) / (std::sqrt(t2) + 1e-30);
@@ -730,21 +892,24 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
     if (!h || !n) return fail(LBMIB_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(h->dev));
     HIP_TRY(hipDeviceSynchronize());
+    *n = 0;
+    if (h->steps_done == 0) return 0;
+    const Slot S = slot_of(h->D, (h->steps_done - 1) % h->D.nslot);
     int na = 0;
-    HIP_TRY(hipMemcpy(&na, h->D.nadj, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&na, S.nadj, sizeof(int), hipMemcpyDeviceToHost));
     *n = na;
     if (na == 0) return 0;
     if (pi || pj || wing) {
         std::vector<int> adj(3 * (size_t)na);
-        HIP_TRY(hipMemcpy(adj.data(), h->D.adj, sizeof(int) * adj.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(adj.data(), S.adj, sizeof(int) * adj.size(), hipMemcpyDeviceToHost));
         for (int a = 0; a < na; ++a) {
             if (pi) pi[a] = adj[3 * a];
             if (pj) pj[a] = adj[3 * a + 1];
             if (wing) wing[a] = adj[3 * a + 2];
         }
     }
-    if (glx) HIP_TRY(hipMemcpy(glx, h->D.glx, sizeof(double) * na, hipMemcpyDeviceToHost));
-    if (gly) HIP_TRY(hipMemcpy(gly, h->D.gly, sizeof(double) * na, hipMemcpyDeviceToHost));
+    if (glx) HIP_TRY(hipMemcpy(glx, S.glx, sizeof(double) * na, hipMemcpyDeviceToHost));
+    if (gly) HIP_TRY(hipMemcpy(gly, S.gly, sizeof(double) * na, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -756,10 +921,10 @@ int lbmib_set_profiling(lbmib_handle* h, int32_t on)
     return 0;
 }
 
-int lbmib_get_timings(lbmib_handle* h, double* ms6)
+int lbmib_get_timings(lbmib_handle* h, double* ms8)
 {
-    if (!h || !ms6) return fail(LBMIB_E_ARG, "null argument");
-    for (int i = 0; i < 6; ++i) ms6[i] = h->tms[i];
+    if (!h || !ms8) return fail(LBMIB_E_ARG, "null argument");
+    for (int i = 0; i < 8; ++i) ms8[i] = h->tms[i];
     return 0;
 }
 

@@ -11,10 +11,47 @@ namespace lbmib {
 constexpr int BOXMAX = 36;  // nodes in one marker's +-2.5 lattice-unit box (<= 6x6)
 constexpr int NCAP = 64;    // markers whose boxes may contain one IB node
 constexpr int KIN = 8;      // per wing per step: sin, cos, omega, sa, sau, sb, sbv, pad
+constexpr int MAXBW = 62;   // largest bandwidth of the banded IB solver
+constexpr int MAXSEG = 4096;
 
 enum ErrBits { ERR_GEOMETRY = 1, ERR_CAPACITY = 2, ERR_SOLVE = 4 };
 
-// All device pointers and scalars of one case.  Passed by value to kernels.
+// Offsets (in elements) of the per-step immersed-boundary arrays inside one
+// slot of the int and double pools.
+struct SlotInts {
+    long long claim;    // [gw*gh] first claiming segment of each window node, INT_MAX when free
+    long long nodeidx;  // [gw*gh] IB list index of each window node, or -1
+    long long box;      // [nseg][4] i1,i2,j1,j2 (1-based)
+    long long draw_idx; // [nseg][BOXMAX] box nodes in box order
+    long long nraw_s;   // [cap][NCAP] markers of each node, arrival order
+    long long adj;      // [cap][3] node i, j (0-based), wing
+    long long nadj;     // [1] node count
+    long long dcnt;     // [nseg] nodes in the segment's box
+    long long didx;     // [nseg][BOXMAX] their list indices, ascending
+    long long nmkcnt;   // [cap] markers whose box holds the node
+    long long nmk;      // [cap][NCAP] those markers, ascending
+    long long total;
+};
+struct SlotDoubles {
+    long long xb2, yb2; // [nseg] marker midpoints
+    long long ub, vb;   // [nseg] marker velocities
+    long long phi;      // [nseg][12] one-axis kernel values of the box columns / rows
+    long long draw_val; // [nseg][BOXMAX]
+    long long nraw_w;   // [cap][NCAP]
+    long long dval;     // [nseg][BOXMAX] kernel weights phi_x*phi_y, ascending node order
+    long long nmkv;     // [cap][NCAP]
+    long long G;        // [nseg][bw+1] band of D D^T, G[p][d] = A(p, p-d)
+    long long L;        // [nseg][bw+1] rows of the factor, L[k][d] = l(k,k-d), L[k][0] = 1/l(k,k)
+    long long LT;       // [nseg][bw+1] columns, LT[k][d] = l(k+d,k), LT[k][0] = 1/l(k,k)
+    long long rhoadj;   // [cap] rho at the nodes
+    long long glx, gly; // [cap] velocity corrections
+    long long total;
+};
+
+// All device pointers and scalars of one case, passed by value to kernels.
+// The immersed-boundary geometry of step n (node set, kernel weights, factor
+// of D D^T) lives in slot (n-1) % nslot.  Geometry depends only on the
+// prescribed kinematics, so it is built in batches of steps ahead of time.
 struct DevCase {
     int nx, ny;
     long long N;            // nx*ny
@@ -25,38 +62,52 @@ struct DevCase {
     int cap;                // IB node capacity
     int bw;                 // lower bandwidth of the zig-zag ordered IB matrix
     int steps_cap;          // rows of kin / hist
+    int nslot;
+    int gi0, gj0, gw, gh;   // lattice window (0-based origin, size) holding every IB node
     // lattice (population-planar, x fastest)
     double* f[2];
     double* f0L;            // stale boundary-ring f0 of column 1 and column nx: [9][ny]
     double* f0R;
     double* macro[2];       // rho, u, v planes (3*N) of the last two flagged steps
-    // immersed boundary
+    // marker input
     const double* xb0;
     const double* yb0;
     const double* kin;      // [steps_cap][nw][KIN]
-    double *xb2, *yb2, *ub, *vb;
-    int* box;               // [nseg][4] i1,i2,j1,j2 (1-based)
-    int* claim;             // [N] first claiming segment, INT_MAX when free
-    int* nodeidx;           // [N] IB list index or -1
-    int* segcnt;            // [nseg] nodes first claimed by each segment
-    int* adj;               // [cap][3] node i, j (0-based), wing
-    int* nadj;              // [1] node count of the current set
-    double *ustar, *vstar, *rhoadj;
-    int* dcnt;              // [nseg] nodes in the segment's box
-    int* didx;              // [nseg][BOXMAX] their list indices, ascending
-    double* dval;           // [nseg][BOXMAX] kernel weights phi_x*phi_y
-    int* nmkcnt;            // [cap] markers whose box holds the node
-    int* nmk;               // [cap][NCAP] those markers (sorted before use)
-    double* nmkv;           // [cap][NCAP] their weights
     const int* perm;        // [nseg] zig-zag position -> segment
     const int* iperm;       // [nseg] segment -> zig-zag position
+    // per-step slots
+    int* ipool;             // nslot * SlotInts::total
+    double* dpool;          // nslot * SlotDoubles::total
+    SlotInts io;
+    SlotDoubles dof;
+    // per-step solve (one step at a time)
     double* rhs;            // [2][nseg] U_B - D u*, zig-zag order
-    double* G;              // [nseg][bw+1] band of D D^T in zig-zag order, G[p][d] = A(p,p-d)
     double* y;              // [2][nseg] (D D^T)^-1 rhs, segment order
-    double *glx, *gly;      // [cap] velocity corrections
     double* hist;           // [steps_cap][3*nw] C_x, C_y, C_M per wing
     int* err;               // [1] ErrBits
     double* red;            // reduction scratch
 };
+
+// One slot's arrays.
+struct Slot {
+    int *claim, *nodeidx, *box, *draw_idx, *nraw_s, *adj, *nadj, *dcnt, *didx, *nmkcnt, *nmk;
+    double *xb2, *yb2, *ub, *vb, *phi, *draw_val, *nraw_w, *dval, *nmkv, *G, *L, *LT, *rhoadj, *glx, *gly;
+};
+
+__host__ __device__ inline Slot slot_of(const DevCase& D, int g)
+{
+    Slot S;
+    int* ib = D.ipool + (long long)g * D.io.total;
+    double* db = D.dpool + (long long)g * D.dof.total;
+    S.claim = ib + D.io.claim; S.nodeidx = ib + D.io.nodeidx; S.box = ib + D.io.box;
+    S.draw_idx = ib + D.io.draw_idx; S.nraw_s = ib + D.io.nraw_s; S.adj = ib + D.io.adj;
+    S.nadj = ib + D.io.nadj; S.dcnt = ib + D.io.dcnt; S.didx = ib + D.io.didx;
+    S.nmkcnt = ib + D.io.nmkcnt; S.nmk = ib + D.io.nmk;
+    S.xb2 = db + D.dof.xb2; S.yb2 = db + D.dof.yb2; S.ub = db + D.dof.ub; S.vb = db + D.dof.vb;
+    S.phi = db + D.dof.phi; S.draw_val = db + D.dof.draw_val; S.nraw_w = db + D.dof.nraw_w;
+    S.dval = db + D.dof.dval; S.nmkv = db + D.dof.nmkv; S.G = db + D.dof.G; S.L = db + D.dof.L;
+    S.LT = db + D.dof.LT; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
+    return S;
+}
 
 }  // namespace lbmib

@@ -4,10 +4,10 @@
 //   lattice:  k_collide_f64   fused pull-stream + moments + MRT collide of every
 //                             interior node, prev -> cur (HBM-bound, 144 B/node)
 //             k_edges_f64     characteristic/NEE outer boundary, prev -> cur ring
-//   IB chain: k_markers -> k_clear -> k_claim -> k_count -> k_fill -> k_nodes ->
-//             k_dvals -> k_assemble -> k_chol -> k_spread   (one small stream)
-//             k_band_f64      forced collide of the IB nodes, overwrites cur there
-//             k_forces        per-wing force / moment, fixed order
+//   IB:       geometry of the next step (node set, kernel weights, Cholesky
+//             factor of D D^T) on its own stream; per step u* -> D u* ->
+//             triangular solves -> spread + forced collide of the IB nodes;
+//             per-wing forces in fixed order
 // Nothing here uses floating-point atomics; every reduction has a fixed order.
 #include "lbmib_kernels.h"
 
@@ -44,9 +44,11 @@ __global__ __launch_bounds__(256) void k_collide_f64(const double* __restrict__ 
                                                      long long N, Rates R,
                                                      double* __restrict__ macro)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    // blocks start at multiples of 256 nodes so every wave writes whole cache
+    // lines; the ring nodes i = 0 and i = nx-1 belong to k_edges_f64
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y + 1;
-    if (i > nx - 2) return;
+    if (i < 1 || i > nx - 2) return;
     long long c = (long long)j * nx + i;
     double f[9];
     pull9(src, N, nx, c, f);
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(256) void k_collide_f64(const double* __restrict__ 
     }
     collide_node(f, rho, u, v, R, false, 0.0, 0.0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) dst[k * N + c] = f[k];
+    for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
 }
 
 // Moments of a boundary-ring node from its stale f0 (never streamed,
@@ -188,10 +190,6 @@ __global__ __launch_bounds__(256) void k_edges_f64(DevCase D, int cur, double cs
     }
 }
 
-// ---------------------------------------------------------------------------
-// immersed boundary
-// ---------------------------------------------------------------------------
-
 // Box search of move_boundary (airfoil.f90:290-321) for one axis: the 1-based
 // index range of nodes with |(o + (i-1)*sc - X)/sc| <= 2.5.
 __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, int& lo, int& hi)
@@ -210,11 +208,62 @@ __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, i
     hi = i2;
 }
 
-// Markers, marker velocities and boxes of step `step` (move_boundary
-// airfoil.f90:256-270 and boundary_condition.f90:1542-1553).  sin/cos of the
-// pitch angle come precomputed from the host's libm.
-__global__ __launch_bounds__(256) void k_markers(DevCase D, int step)
+// ---------------------------------------------------------------------------
+// immersed boundary
+//
+// Geometry of a batch of steps (one slot per step, blockIdx.y = step in the
+// batch), built ahead of time on its own stream because it depends only on
+// the prescribed kinematics:
+//   k_geo_clear -> k_geo_markers -> k_geo_fill -> k_geo_phi -> k_geo_pairs ->
+//   k_geo_dsort -> k_geo_nsort -> k_geo_assemble -> k_geo_factor
+// Per-step solve, on the critical path beside the lattice kernel:
+//   k_crit_rhs -> k_crit_solve -> (after the lattice kernel) k_band_f64
+// Per-wing forces of a batch of steps: k_forces.
+// ---------------------------------------------------------------------------
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, not for its outstanding global stores.
+__device__ __forceinline__ void lds_barrier()
 {
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__device__ __forceinline__ bool box_cell(const int* b, int q, int& i, int& j)
+{
+    int nyb = b[3] - b[2] + 1;
+    int n = (b[1] - b[0] + 1) * nyb;
+    if (nyb <= 0 || q >= n) return false;
+    i = b[0] + q / nyb;  // i-major, j-minor: the reference's loop order (airfoil.f90:322-323)
+    j = b[2] + q % nyb;
+    return true;
+}
+
+// window index of 1-based lattice node (i, j)
+__device__ __forceinline__ long long wcell(const DevCase& D, int i, int j)
+{
+    return (long long)(j - 1 - D.gj0) * D.gw + (i - 1 - D.gi0);
+}
+
+// Clear the slot's previous node set (airfoil.f90:276-283).
+__global__ __launch_bounds__(256) void k_geo_clear(DevCase D, int g0)
+{
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= *S.nadj) return;
+    long long c = (long long)(S.adj[3 * a + 1] - D.gj0) * D.gw + (S.adj[3 * a] - D.gi0);
+    S.claim[c] = INT_MAX;
+    S.nodeidx[c] = -1;
+}
+
+// Markers, marker velocities and boxes (move_boundary airfoil.f90:256-321,
+// boundary_condition.f90:1542-1553), then the first claim of every box node:
+// the smallest segment index wins, which is the segment that claims it first
+// in the reference's serial loop.  sin/cos of the pitch angle come
+// precomputed from the host's libm.
+__global__ __launch_bounds__(256) void k_geo_markers(DevCase D, int g0, int step0)
+{
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const int step = step0 + blockIdx.y;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.nseg) return;
     int w = s / D.spw;
@@ -228,193 +277,184 @@ __global__ __launch_bounds__(256) void k_markers(DevCase D, int step)
     double y1b = -sn * xb + ca * yb + sb + D.yc;
     double X = 0.5 * (x1b + x1a);
     double Y = 0.5 * (y1b + y1a);
-    D.xb2[s] = X;
-    D.yb2[s] = Y;
+    S.xb2[s] = X;
+    S.yb2[s] = Y;
     double xs = xb + xa, ys = yb + ya;
     double U = D.U;
-    D.ub[s] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + sau);
-    D.vb[s] = U * (-om * ca * 0.5 * xs - om * sn * 0.5 * ys + sbv);
+    S.ub[s] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + sau);
+    S.vb[s] = U * (-om * ca * 0.5 * xs - om * sn * 0.5 * ys + sbv);
     int i1, i2, j1, j2;
     axis_box(X, D.ox, D.scale, D.nx, i1, i2);
     axis_box(Y, D.oy, D.scale, D.ny, j1, j2);
     bool ok = i1 >= 4 && i2 <= D.nx - 3 && j1 >= 4 && j2 <= D.ny - 3 && i2 >= i1 && j2 >= j1 &&
-              (i2 - i1) < 6 && (j2 - j1) < 6;
+              (i2 - i1) < 6 && (j2 - j1) < 6 && i1 - 1 >= D.gi0 && i2 - 1 < D.gi0 + D.gw &&
+              j1 - 1 >= D.gj0 && j2 - 1 < D.gj0 + D.gh;
     if (!ok) {
         atomicOr(D.err, ERR_GEOMETRY);
-        i1 = 4; i2 = 3; j1 = 4; j2 = 3;  // empty box keeps every later access in bounds
+        i1 = D.gi0 + 2; i2 = D.gi0 + 1; j1 = D.gj0 + 2; j2 = D.gj0 + 1;  // empty box
     }
-    int* b = D.box + 4 * s;
+    int* b = S.box + 4 * s;
     b[0] = i1; b[1] = i2; b[2] = j1; b[3] = j2;
-}
-
-// Clear the previous node set (airfoil.f90:276-283).
-__global__ __launch_bounds__(256) void k_clear(DevCase D)
-{
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *D.nadj) return;
-    long long c = (long long)D.adj[3 * a + 1] * D.nx + D.adj[3 * a];
-    D.claim[c] = INT_MAX;
-    D.nodeidx[c] = -1;
-}
-
-__device__ __forceinline__ bool box_cell(const int* b, int q, int& i, int& j)
-{
-    int nyb = b[3] - b[2] + 1;
-    int n = (b[1] - b[0] + 1) * nyb;
-    if (nyb <= 0 || q >= n) return false;
-    i = b[0] + q / nyb;  // i-major, j-minor: the reference's loop order (airfoil.f90:322-323)
-    j = b[2] + q % nyb;
-    return true;
-}
-
-// First claim of every box node: the smallest segment index wins, which is
-// the segment that claims it first in the reference's serial loop.
-__global__ __launch_bounds__(256) void k_claim(DevCase D)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = t / BOXMAX, q = t % BOXMAX;
-    if (s >= D.nseg) return;
-    int i, j;
-    if (!box_cell(D.box + 4 * s, q, i, j)) return;
-    atomicMin(&D.claim[(long long)(j - 1) * D.nx + (i - 1)], s);
-}
-
-__global__ __launch_bounds__(256) void k_count(DevCase D)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= D.nseg) return;
-    const int* b = D.box + 4 * s;
-    int cnt = 0, i, j;
-    for (int q = 0; q < BOXMAX; ++q) {
-        if (!box_cell(b, q, i, j)) break;
-        if (D.claim[(long long)(j - 1) * D.nx + (i - 1)] == s) cnt++;
-    }
-    D.segcnt[s] = cnt;
+    for (int i = i1; i <= i2; ++i)
+        for (int j = j1; j <= j2; ++j) atomicMin(&S.claim[wcell(D, i, j)], s);
 }
 
 // Ordered compaction: the node list in the reference's order (segment-major,
-// then i, then j), the node -> list index grid and the node count.
-__global__ __launch_bounds__(1024) void k_fill(DevCase D)
+// then i, then j; airfoil.f90:322-331), the node -> list index grid, the node
+// count, and cleared marker-list counters.  One workgroup per slot.
+__global__ __launch_bounds__(1024) void k_geo_fill(DevCase D, int g0)
 {
+    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
     __shared__ int part[1024];
-    const int ns = D.nseg;
-    const int T = blockDim.x;
-    int chunk = (ns + T - 1) / T;
-    int s0 = threadIdx.x * chunk, s1 = min(ns, s0 + chunk);
+    __shared__ int s_total;
+    const int ns = D.nseg, T = blockDim.x, tid = threadIdx.x;
+    const int chunk = (ns + T - 1) / T;
+    const int s0 = tid * chunk, s1 = min(ns, s0 + chunk);
     int sum = 0;
-    for (int s = s0; s < s1; ++s) sum += D.segcnt[s];
-    part[threadIdx.x] = sum;
+    for (int s = s0; s < s1; ++s) {
+        const int* b = S.box + 4 * s;
+        int cl[BOXMAX];
+#pragma unroll
+        for (int q = 0; q < BOXMAX; ++q) {
+            int i, j;
+            cl[q] = box_cell(b, q, i, j) ? S.claim[wcell(D, i, j)] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < BOXMAX; ++q) sum += cl[q] == s;
+    }
+    part[tid] = sum;
     __syncthreads();
     for (int off = 1; off < T; off <<= 1) {
-        int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        int v = tid >= off ? part[tid - off] : 0;
         __syncthreads();
-        part[threadIdx.x] += v;
+        part[tid] += v;
         __syncthreads();
     }
-    int base = part[threadIdx.x] - sum;
-    if (threadIdx.x == T - 1) {
-        if (part[T - 1] > D.cap) atomicOr(D.err, ERR_CAPACITY);
-        *D.nadj = min(part[T - 1], D.cap);
-    }
+    int base = part[tid] - sum;
+    if (tid == T - 1) s_total = part[T - 1];
     for (int s = s0; s < s1; ++s) {
-        const int* b = D.box + 4 * s;
-        int w = s / D.spw, i, j;
+        const int* b = S.box + 4 * s;
+        int w = s / D.spw;
+        int cl[BOXMAX];
+#pragma unroll
         for (int q = 0; q < BOXMAX; ++q) {
-            if (!box_cell(b, q, i, j)) break;
-            long long c = (long long)(j - 1) * D.nx + (i - 1);
-            if (D.claim[c] == s) {
+            int i, j;
+            cl[q] = box_cell(b, q, i, j) ? S.claim[wcell(D, i, j)] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < BOXMAX; ++q) {
+            int i, j;
+            if (cl[q] == s && box_cell(b, q, i, j)) {
                 if (base < D.cap) {
-                    D.adj[3 * base] = i - 1;
-                    D.adj[3 * base + 1] = j - 1;
-                    D.adj[3 * base + 2] = w;
-                    D.nodeidx[c] = base;
+                    S.adj[3 * base] = i - 1;
+                    S.adj[3 * base + 1] = j - 1;
+                    S.adj[3 * base + 2] = w;
+                    S.nodeidx[wcell(D, i, j)] = base;
                 }
                 base++;
             }
         }
     }
+    __syncthreads();
+    const int total = s_total;
+    if (tid == 0) {
+        if (total > D.cap) atomicOr(D.err, ERR_CAPACITY);
+        *S.nadj = min(total, D.cap);
+    }
+    for (int a = tid; a < min(total, D.cap); a += T) S.nmkcnt[a] = 0;
 }
 
-// u*, v*, rho at the IB nodes (physical_variables at those nodes).
-__global__ __launch_bounds__(256) void k_nodes(DevCase D, int cur)
+// One-axis kernel values of every box column (q < 6) and row (q >= 6)
+// (boundary_condition.f90:1581-1600).
+__global__ __launch_bounds__(256) void k_geo_phi(DevCase D, int g0)
 {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *D.nadj) return;
-    const double* src = D.f[cur ^ 1];
-    long long c = (long long)D.adj[3 * a + 1] * D.nx + D.adj[3 * a];
-    double f[9], r, u, v;
-    pull9(src, D.N, D.nx, c, f);
-    moments(f, r, u, v);
-    D.rhoadj[a] = r;
-    D.ustar[a] = u;
-    D.vstar[a] = v;
-    D.nmkcnt[a] = 0;
-}
-
-// Per segment: kernel weights of its box nodes sorted by list index
-// (xx1 row, boundary_condition.f90:1558-1606), the interpolation D u*
-// (DGEMV, 1620-1621) in that order, the right-hand side U_B - D u* (1624-1629)
-// in zig-zag order, and registration in the nodes' marker lists.
-__global__ __launch_bounds__(64) void k_dvals(DevCase D)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = t / 12, q = t % 12;
     if (s >= D.nseg) return;
-    const int* b = D.box + 4 * s;
+    const int* b = S.box + 4 * s;
     const double sc = D.scale;
-    double px[6], py[6];
-    int nxb = b[1] - b[0] + 1, nyb = b[3] - b[2] + 1;
-    double X = D.xb2[s], Y = D.yb2[s];
-    for (int q = 0; q < nxb; ++q) {
-        double x0 = D.ox + sc * (double)(b[0] + q - 1);
-        px[q] = delta1(fabs((x0 - X) / sc), D.smooth);
-    }
-    for (int q = 0; q < nyb; ++q) {
-        double y0 = D.oy + sc * (double)(b[2] + q - 1);
-        py[q] = delta1(fabs((y0 - Y) / sc), D.smooth);
-    }
-    int idx[BOXMAX];
-    double val[BOXMAX];
-    int n = 0;
-    for (int qi = 0; qi < nxb; ++qi)
-        for (int qj = 0; qj < nyb; ++qj) {
-            long long c = (long long)(b[2] + qj - 1) * D.nx + (b[0] + qi - 1);
-            int node = D.nodeidx[c];
-            double w = px[qi] * py[qj];
-            // insertion by list index
-            int m = n++;
-            while (m > 0 && idx[m - 1] > node) { idx[m] = idx[m - 1]; val[m] = val[m - 1]; --m; }
-            idx[m] = node;
-            val[m] = w;
-            if (node >= 0) {
-                int slot = atomicAdd(&D.nmkcnt[node], 1);
-                if (slot < NCAP) {
-                    D.nmk[(long long)node * NCAP + slot] = s;
-                    D.nmkv[(long long)node * NCAP + slot] = w;
-                } else {
-                    atomicOr(D.err, ERR_CAPACITY);
-                }
-            } else {
-                atomicOr(D.err, ERR_CAPACITY);
-            }
+    double v = 0.0;
+    if (q < 6) {
+        if (q <= b[1] - b[0]) {
+            double x0 = D.ox + sc * (double)(b[0] + q - 1);
+            v = delta1(fabs((x0 - S.xb2[s]) / sc), D.smooth);
         }
-    double su = 0.0, sv = 0.0;
-    for (int a = 0; a < n; ++a) {
-        int node = idx[a] < 0 ? 0 : idx[a];
-        su = su + val[a] * D.ustar[node];
-        sv = sv + val[a] * D.vstar[node];
-        D.didx[s * BOXMAX + a] = idx[a];
-        D.dval[s * BOXMAX + a] = val[a];
+    } else if (q - 6 <= b[3] - b[2]) {
+        double y0 = D.oy + sc * (double)(b[2] + q - 6 - 1);
+        v = delta1(fabs((y0 - S.yb2[s]) / sc), D.smooth);
     }
-    D.dcnt[s] = n;
-    int p = D.iperm[s];
-    D.rhs[p] = D.ub[s] - su;
-    D.rhs[D.nseg + p] = D.vb[s] - sv;
+    S.phi[t] = v;
 }
 
-// Band of G = D D^T in zig-zag order: G(p,q) = sum over the nodes shared by
-// the boxes of segments perm[p], perm[q], ascending list index.
-__global__ __launch_bounds__(256) void k_assemble(DevCase D)
+// Entry xx1(s, node) = kernel1*kernel2 (boundary_condition.f90:1602) of every
+// box node, in box order, and registration of s in the node's marker list.
+__global__ __launch_bounds__(256) void k_geo_pairs(DevCase D, int g0)
 {
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = t / BOXMAX, q = t % BOXMAX;
+    if (s >= D.nseg) return;
+    const int* b = S.box + 4 * s;
+    int i, j;
+    if (!box_cell(b, q, i, j)) return;
+    int nyb = b[3] - b[2] + 1;
+    double w = S.phi[s * 12 + q / nyb] * S.phi[s * 12 + 6 + q % nyb];
+    int node = S.nodeidx[wcell(D, i, j)];
+    S.draw_idx[t] = node;
+    S.draw_val[t] = w;
+    if (node < 0) { atomicOr(D.err, ERR_CAPACITY); return; }
+    int slot = atomicAdd(&S.nmkcnt[node], 1);
+    if (slot < NCAP) {
+        S.nraw_s[(long long)node * NCAP + slot] = s;
+        S.nraw_w[(long long)node * NCAP + slot] = w;
+    } else {
+        atomicOr(D.err, ERR_CAPACITY);
+    }
+}
+
+// A row of xx1 in ascending node order: each entry moves to its rank.
+__global__ __launch_bounds__(256) void k_geo_dsort(DevCase D, int g0)
+{
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = t / BOXMAX, q = t % BOXMAX;
+    if (s >= D.nseg) return;
+    const int* b = S.box + 4 * s;
+    int cnt = max(0, (b[1] - b[0] + 1) * (b[3] - b[2] + 1));
+    if (q == 0) S.dcnt[s] = cnt;
+    if (q >= cnt) return;
+    const int* ix = S.draw_idx + s * BOXMAX;
+    int me = ix[q], rank = 0;
+    for (int r = 0; r < cnt; ++r) rank += ix[r] < me;
+    S.didx[s * BOXMAX + rank] = me < 0 ? 0 : me;
+    S.dval[s * BOXMAX + rank] = S.draw_val[t];
+}
+
+// Each node's marker list in ascending segment order (the order of the
+// spread sum); grid-stride over (node, entry).
+__global__ __launch_bounds__(256) void k_geo_nsort(DevCase D, int g0)
+{
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const long long total = (long long)(*S.nadj) * NCAP;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        int a = (int)(t / NCAP), q = (int)(t % NCAP);
+        int cnt = min(S.nmkcnt[a], NCAP);
+        if (q >= cnt) continue;
+        const int* ks = S.nraw_s + (long long)a * NCAP;
+        int me = ks[q], rank = 0;
+        for (int r = 0; r < cnt; ++r) rank += ks[r] < me;
+        S.nmk[(long long)a * NCAP + rank] = me;
+        S.nmkv[(long long)a * NCAP + rank] = S.nraw_w[t];
+    }
+}
+
+// Band of G = D D^T in zig-zag order: G(p, p-d) is the sum over the nodes
+// shared by the boxes of segments perm[p] and perm[p-d], ascending list index.
+__global__ __launch_bounds__(256) void k_geo_assemble(DevCase D, int g0)
+{
+    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
     const int W = D.bw + 1;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)D.nseg * W) return;
@@ -422,14 +462,14 @@ __global__ __launch_bounds__(256) void k_assemble(DevCase D)
     double sum = 0.0;
     if (q >= 0) {
         int k = D.perm[p], l = D.perm[q];
-        const int* bk = D.box + 4 * k;
-        const int* bl = D.box + 4 * l;
+        const int* bk = S.box + 4 * k;
+        const int* bl = S.box + 4 * l;
         if (!(bk[1] < bl[0] || bl[1] < bk[0] || bk[3] < bl[2] || bl[3] < bk[2])) {
-            const int* ik = D.didx + k * BOXMAX;
-            const int* il = D.didx + l * BOXMAX;
-            const double* vk = D.dval + k * BOXMAX;
-            const double* vl = D.dval + l * BOXMAX;
-            int nk = D.dcnt[k], nl = D.dcnt[l], a = 0, c = 0;
+            const int* ik = S.didx + k * BOXMAX;
+            const int* il = S.didx + l * BOXMAX;
+            const double* vk = S.dval + k * BOXMAX;
+            const double* vl = S.dval + l * BOXMAX;
+            int nk = S.dcnt[k], nl = S.dcnt[l], a = 0, c = 0;
             while (a < nk && c < nl) {
                 int xa = ik[a], xc = il[c];
                 if (xa < xc) a++;
@@ -438,150 +478,418 @@ __global__ __launch_bounds__(256) void k_assemble(DevCase D)
             }
         }
     }
-    D.G[t] = sum;
+    S.G[t] = sum;
 }
 
-// Banded Cholesky of G and the two triangular solves for both right-hand
-// sides, right-looking: element (i,j) receives its updates in increasing k.
-// One workgroup; the band lives in global memory (L2-resident).
-__global__ __launch_bounds__(1024) void k_chol(DevCase D)
+constexpr int WIN = 64;  // LDS window slots of the banded factorisation (rows k..k+bw, bw <= WIN-2)
+constexpr int CH = 32;   // rows / columns staged per chunk
+constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
+
+// Banded Cholesky of G, right-looking over a sliding LDS window of rows
+// k..k+bw (row r in slot r % WIN, entry d = column r-d).  Element (i,j)
+// receives its updates in increasing k, so the result equals the dense
+// right-looking algorithm bit for bit.  Columns are scaled by the reciprocal
+// of the pivot (LAPACK dpotf2).  Rows entering the window are staged through
+// LDS one chunk ahead; the factor leaves through plain stores that are never
+// waited on inside the loop.  Outputs L (rows) and LT (columns), both with
+// 1/l_kk in entry 0.  One workgroup per slot of the batch.
+__global__ __launch_bounds__(256) void k_geo_factor(DevCase D, int g0)
 {
-    const int n = D.nseg, bw = D.bw, W = bw + 1;
-    double* G = D.G;
-    double* x = D.rhs;
+    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
+    extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
+    const int n = D.nseg, bw = D.bw, W = bw + 1;
+    double* win = smem;
+    double* colv = win + WIN * WIN;
+    double* stage = colv + WIN;  // [2][CH][WIN]
+    int* pairs = (int*)(stage + 2 * CH * WIN);
+    double* L = S.L;
+    double* LT = S.LT;
+    const double* G = S.G;
+    for (int a = tid; a < bw; a += T)
+        for (int b = 0; b <= a; ++b) pairs[a * (a + 1) / 2 + b] = (a << 8) | b;
+    for (int e = tid; e < min(W, n) * W; e += T) {
+        int r = e / W, d = e % W;
+        win[(r % WIN) * WIN + d] = G[(long long)r * W + d];
+    }
+    // chunk c: rows W + c*CH .. W + c*CH + CH - 1, entering at steps c*CH .. c*CH + CH - 1
+    constexpr int PER = (CH * WIN + 255) / 256;
+    double pre[PER];
+    auto load_chunk = [&](int c) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + q * T;
+            int r = W + c * CH + e / W, d = e % W;
+            pre[q] = (e < CH * W && r < n) ? G[(long long)r * W + d] : 0.0;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + q * T;
+            if (e < CH * W) stage[buf * CH * WIN + (e / W) * WIN + e % W] = pre[q];
+        }
+    };
+    load_chunk(0);
+    store_chunk(0);
+    load_chunk(1);
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
-        int m = min(k + bw, n - 1) - k;
-        double akk = G[(long long)k * W];
-        double lkk = sqrt(akk);
-        __syncthreads();
-        if (tid == 0) {
-            G[(long long)k * W] = lkk;
-            if (!(akk > 0.0)) atomicOr(D.err, ERR_SOLVE);
+        const int c = k / CH;
+        if (k > 0 && k % CH == 0) {
+            store_chunk(c & 1);  // chunk c, prefetched during chunk c-1
+            load_chunk(c + 1);
         }
-        for (int t = tid; t < m; t += T) {
-            int i = k + 1 + t;
-            long long e = (long long)i * W + (i - k);
-            G[e] = G[e] / lkk;
+        const int m = min(k + bw, n - 1) - k;
+        const int sk = (k % WIN) * WIN;
+        const double akk = win[sk];
+        const double lkk = sqrt(akk);
+        const double rk = 1.0 / lkk;
+        if (tid < m) {
+            int i = k + 1 + tid;
+            int e = (i % WIN) * WIN + (i - k);
+            double l = win[e] * rk;
+            win[e] = l;
+            colv[tid] = l;
         }
-        __syncthreads();
-        for (int t = tid; t < m * m; t += T) {
-            int a = t / m, b = t % m;
-            if (b > a) continue;
+        lds_barrier();
+        if (tid == 0 && !(akk > 0.0)) atomicOr(D.err, ERR_SOLVE);
+        const int npairs = m * (m + 1) / 2;
+        for (int t = tid; t < npairs; t += T) {
+            int ab = pairs[t];
+            int a = ab >> 8, b = ab & 255;
             int i = k + 1 + a, j = k + 1 + b;
-            long long e = (long long)i * W + (i - j);
-            G[e] = G[e] - G[(long long)i * W + (i - k)] * G[(long long)j * W + (j - k)];
+            int e = (i % WIN) * WIN + (i - j);
+            win[e] = win[e] - colv[a] * colv[b];
         }
-        __syncthreads();
-    }
-    // forward L z = b
-    for (int k = 0; k < n; ++k) {
-        int m = min(k + bw, n - 1) - k;
-        if (tid < 2) x[tid * n + k] = x[tid * n + k] / G[(long long)k * W];
-        __syncthreads();
-        for (int t = tid; t < 2 * m; t += T) {
-            int r = t & 1, i = k + 1 + (t >> 1);
-            x[r * n + i] = x[r * n + i] - G[(long long)i * W + (i - k)] * x[r * n + k];
+        for (int d = tid; d < W; d += T) {
+            // column k of L, and row k (final now)
+            LT[(long long)k * W + d] = d == 0 ? rk : (d <= m ? colv[d - 1] : 0.0);
+            L[(long long)k * W + d] = d == 0 ? rk : (d <= k ? win[sk + d] : 0.0);
         }
-        __syncthreads();
+        // row k+W enters the window (its slot differs from every row in use)
+        const int rn = k + W;
+        if (rn < n)
+            for (int d = tid; d < W; d += T)
+                win[(rn % WIN) * WIN + d] = stage[(c & 1) * CH * WIN + (k - c * CH) * WIN + d];
+        lds_barrier();
     }
-    // backward L^T x = z
-    for (int k = n - 1; k >= 0; --k) {
-        int m = k - max(0, k - bw);
-        if (tid < 2) x[tid * n + k] = x[tid * n + k] / G[(long long)k * W];
-        __syncthreads();
-        for (int t = tid; t < 2 * m; t += T) {
-            int r = t & 1, i = k - 1 - (t >> 1);
-            x[r * n + i] = x[r * n + i] - G[(long long)k * W + (k - i)] * x[r * n + k];
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)b, lane);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Interpolation D u* (the DGEMVs, boundary_condition.f90:1620-1621) and the
+// right-hand side U_B - D u* (1624-1629) in zig-zag order.  Each thread
+// forms one product xx1(s,node) * u*(node) with u* taken from the moments of
+// the node's post-stream populations (physical_variables there); the
+// segment's products are then summed in ascending node order.
+constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
+__global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
+{
+    const Slot S = slot_of(D, g);
+    __shared__ double pu[RHS_SEG][BOXMAX], pv[RHS_SEG][BOXMAX];
+    const int sl = threadIdx.x / BOXMAX, q = threadIdx.x % BOXMAX;
+    const int s = blockIdx.x * RHS_SEG + sl;
+    const bool live = sl < RHS_SEG && s < D.nseg;
+    const int n = live ? S.dcnt[s] : 0;
+    if (live && q < n) {
+        int node = S.didx[s * BOXMAX + q];
+        double w = S.dval[s * BOXMAX + q];
+        long long c = (long long)S.adj[3 * node + 1] * D.nx + S.adj[3 * node];
+        double f[9], r, u, v;
+        pull9(D.f[cur ^ 1], D.N, D.nx, c, f);
+        moments(f, r, u, v);
+        pu[sl][q] = w * u;
+        pv[sl][q] = w * v;
+    }
+    __syncthreads();
+    if (live && q == 0) {
+        double su = 0.0, sv = 0.0;
+        for (int a = 0; a < n; ++a) { su = su + pu[sl][a]; sv = sv + pv[sl][a]; }
+        int p = D.iperm[s];
+        D.rhs[p] = S.ub[s] - su;
+        D.rhs[D.nseg + p] = S.vb[s] - sv;
+    }
+}
+
+// Forward L z = b and backward L^T x = z for both right-hand sides in one
+// wavefront.  With a window of WS rows (WS = 32: u in lanes 0-31, v in lanes
+// 32-63, bw <= 30; WS = 64: both in every lane, bw <= 62), lane h of a group
+// holds the running value of the row r == h (mod WS) of the active window.
+// The owner of row k forms z_k = b_k * (1/l_kk), every lane of the group
+// receives it by readlane and applies its l(r,k) z_k, so element r gets its
+// updates in increasing (forward) / decreasing (backward) k exactly as the
+// column-oriented algorithm applies them.  A lane's next row is read from
+// LDS right after it finishes the current one and swapped in a few steps
+// later, before that row's first update.  Columns / rows of L are staged in
+// LDS one chunk ahead and each step's entries are read one step ahead.
+template <int WS>
+__global__ __launch_bounds__(64) void k_crit_solve(DevCase D, int g)
+{
+    const Slot S = slot_of(D, g);
+    extern __shared__ double zb[];                // [2][n]
+    __shared__ double stage[2 * CH * WIN];        // [2][CH][WIN]
+    constexpr int NR = WS == 32 ? 1 : 2;          // right-hand sides per lane
+    const int lane = threadIdx.x, h = lane & (WS - 1);
+    const int r0 = WS == 32 ? (lane >> 5) : 0;    // first right-hand side of this lane
+    const int n = D.nseg, bw = D.bw, W = bw + 1;
+    constexpr int PER = (CH * WIN + 63) / 64;
+    double pre[PER];
+    const double* LT = S.LT;
+    const double* L = S.L;
+    for (int e = lane; e < 2 * n; e += 64) zb[e] = D.rhs[e];
+    auto load_chunk = [&](const double* src, int c) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = lane + q * 64;
+            int r = c * CH + e / W, d = e % W;
+            pre[q] = (c >= 0 && e < CH * W && r < n) ? src[(long long)r * W + d] : 0.0;
         }
-        __syncthreads();
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = lane + q * 64;
+            if (e < CH * W) stage[buf * CH * WIN + (e / W) * WIN + e % W] = pre[q];
+        }
+    };
+    auto bcast = [&](double v, int owner) -> double {
+        if (WS == 32) {
+            double a0 = readlane_f64(v, owner);
+            double a1 = readlane_f64(v, owner + 32);
+            return r0 ? a1 : a0;
+        }
+        return readlane_f64(v, owner);
+    };
+    const int take = (bw + 1) & (WS - 1);
+    const int nch = (n + CH - 1) / CH;
+    double b[NR], nb[NR];
+    // ---- forward ----
+    load_chunk(LT, 0);
+    store_chunk(0);
+    load_chunk(LT, 1);
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { b[r] = h < n ? zb[(r0 + r) * n + h] : 0.0; nb[r] = 0.0; }
+    for (int c = 0; c < nch; ++c) {
+        const double* colbase = stage + (c & 1) * CH * WIN;
+        double rk_n = colbase[0];
+        double lv_n = colbase[(h - c * CH) & (WS - 1)];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int k = c * CH + q;
+            if (k < n) {
+                const double rk = rk_n, lv = lv_n;
+                if (q + 1 < CH) {
+                    rk_n = colbase[(q + 1) * WIN];
+                    lv_n = colbase[(q + 1) * WIN + ((h - k - 1) & (WS - 1))];
+                }
+                const int d = (h - k) & (WS - 1);   // this lane's row is k + d
+                const int owner = k & (WS - 1);
+                double z[NR];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) z[r] = bcast(b[r] * rk, owner);
+                const bool upd = d >= 1 && d <= bw && k + d < n;
+                const bool tk = d == take && k + bw + 1 < n && k + bw + 1 >= WS;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double nv = b[r] - lv * z[r];
+                    b[r] = upd ? nv : b[r];
+                    b[r] = tk ? nb[r] : b[r];
+                }
+                if (d == 0) {
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        zb[(r0 + r) * n + k] = z[r];
+                        if (k + WS < n) nb[r] = zb[(r0 + r) * n + k + WS];
+                    }
+                }
+            }
+        }
+        if (c + 1 < nch) {
+            store_chunk((c + 1) & 1);
+            load_chunk(LT, c + 2);
+            lds_barrier();
+        }
     }
-    for (int p = tid; p < n; p += T) {
+    lds_barrier();
+    // ---- backward ----
+    {
+        int c = nch - 1;
+        load_chunk(L, c);
+        store_chunk(c & 1);
+        load_chunk(L, c - 1);
+        lds_barrier();
+        // lane holds the row r == h (mod WS) in [n-WS, n-1]
+        const int rr = (n - 1) - ((((n - 1 - h) % WS) + WS) % WS);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) { b[r] = rr >= 0 ? zb[(r0 + r) * n + rr] : 0.0; nb[r] = 0.0; }
+        for (; c >= 0; --c) {
+            const double* rowbase = stage + (c & 1) * CH * WIN;
+            double rk_n = rowbase[(CH - 1) * WIN];
+            double lv_n = rowbase[(CH - 1) * WIN + ((c * CH + CH - 1 - h) & (WS - 1))];
+#pragma unroll
+            for (int q = CH - 1; q >= 0; --q) {
+                const int k = c * CH + q;
+                const double rk = rk_n, lv = lv_n;
+                if (q > 0) {
+                    rk_n = rowbase[(q - 1) * WIN];
+                    lv_n = rowbase[(q - 1) * WIN + ((k - 1 - h) & (WS - 1))];
+                }
+                if (k < n) {
+                    const int d = (k - h) & (WS - 1);   // this lane's row is k - d
+                    const int owner = k & (WS - 1);
+                    double x[NR];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) x[r] = bcast(b[r] * rk, owner);
+                    const bool upd = d >= 1 && d <= bw && k - d >= 0;
+                    const bool tk = d == take && k - bw - 1 >= 0 && k - bw - 1 < n - WS;
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        double nv = b[r] - lv * x[r];
+                        b[r] = upd ? nv : b[r];
+                        b[r] = tk ? nb[r] : b[r];
+                    }
+                    if (d == 0) {
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) {
+                            zb[(r0 + r) * n + k] = x[r];
+                            if (k - WS >= 0) nb[r] = zb[(r0 + r) * n + k - WS];
+                        }
+                    }
+                }
+            }
+            if (c > 0) {
+                store_chunk((c - 1) & 1);
+                load_chunk(L, c - 2);
+                lds_barrier();
+            }
+        }
+    }
+    lds_barrier();
+    for (int p = lane; p < n; p += 64) {
         int s = D.perm[p];
-        D.y[s] = x[p];
-        D.y[n + s] = x[n + p];
+        D.y[s] = zb[p];
+        D.y[n + s] = zb[n + p];
     }
 }
+template __global__ void k_crit_solve<32>(DevCase, int);
+template __global__ void k_crit_solve<64>(DevCase, int);
 
-// Spread delta_u(i) = sum_k D(k,i) y(k), k ascending (the fixed-order gather
-// replacing the spread DGEMV, boundary_condition.f90:1639-1640).
-__global__ __launch_bounds__(256) void k_spread(DevCase D)
-{
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *D.nadj) return;
-    int cnt = min(D.nmkcnt[a], NCAP);
-    int ks[NCAP];
-    double ws[NCAP];
-    for (int q = 0; q < cnt; ++q) {
-        int s = D.nmk[(long long)a * NCAP + q];
-        double w = D.nmkv[(long long)a * NCAP + q];
-        int m = q;
-        while (m > 0 && ks[m - 1] > s) { ks[m] = ks[m - 1]; ws[m] = ws[m - 1]; --m; }
-        ks[m] = s;
-        ws[m] = w;
-    }
-    const int n = D.nseg;
-    double sx = 0.0, sy = 0.0;
-    for (int q = 0; q < cnt; ++q) {
-        sx = sx + ws[q] * D.y[ks[q]];
-        sy = sy + ws[q] * D.y[n + ks[q]];
-    }
-    D.glx[a] = sx;
-    D.gly[a] = sy;
-}
+size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + WIN + 2 * CH * WIN) + sizeof(int) * PAIRMAX; }
+size_t solve_smem_bytes(int nseg) { return sizeof(double) * 2 * (size_t)nseg; }
 
-// Velocity correction, body force and forced collision at the IB nodes
-// (boundary_condition.f90:1647-1657, solution.f90:598-638).  Runs after
-// k_collide_f64 and overwrites its unforced result at these nodes.
-__global__ __launch_bounds__(256) void k_band_f64(DevCase D, int cur, double* __restrict__ macro)
+// Spread delta_u = sum_k D(k,a) y(k) in ascending k (the fixed-order gather in
+// place of the spread DGEMV, boundary_condition.f90:1639-1640), velocity
+// correction and body force (1647-1657), and the forced collision
+// (solution.f90:598-638).  Runs after k_collide_f64 and overwrites its
+// unforced result at the IB nodes.
+__global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, double* __restrict__ macro)
 {
+    const Slot S = slot_of(D, g);
     int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *D.nadj) return;
-    const double* src = D.f[cur ^ 1];
-    double* dst = D.f[cur];
-    long long c = (long long)D.adj[3 * a + 1] * D.nx + D.adj[3 * a];
+    if (a >= *S.nadj) return;
+    const int cnt = min(S.nmkcnt[a], NCAP);
+    const int* km = S.nmk + (long long)a * NCAP;
+    const double* wm = S.nmkv + (long long)a * NCAP;
+    const int ns = D.nseg;
+    const int* ad = S.adj + 3 * a;
+    long long c = (long long)ad[1] * D.nx + ad[0];
     double f[9], r, us, vs;
-    pull9(src, D.N, D.nx, c, f);
+    pull9(D.f[cur ^ 1], D.N, D.nx, c, f);
+    double gx = 0.0, gy = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        int s = km[q];
+        double w = wm[q];
+        gx = gx + w * D.y[s];
+        gy = gy + w * D.y[ns + s];
+    }
+    S.glx[a] = gx;
+    S.gly[a] = gy;
     moments(f, r, us, vs);
-    double u = us + D.glx[a];
-    double v = vs + D.gly[a];
-    double gfx = 2.0 * r * D.glx[a];
-    double gfy = 2.0 * r * D.gly[a];
+    S.rhoadj[a] = r;
+    double u = us + gx;
+    double v = vs + gy;
+    double gfx = 2.0 * r * gx;
+    double gfy = 2.0 * r * gy;
     bool forced = (gfx != 0.0) || (gfy != 0.0);
     collide_node(f, r, u, v, D.R, forced, gfx, gfy);
+    double* dst = D.f[cur];
 #pragma unroll
     for (int k = 0; k < 9; ++k) dst[k * D.N + c] = f[k];
     if (macro) { macro[D.N + c] = u; macro[2 * D.N + c] = v; }
 }
 
-// Per-wing force and moment, sequential in node order
+// Per-wing force and moment, accumulated in node order
 // (boundary_condition.f90:1662-1675), and their normalisation
-// (solution.f90:327-330).
-__global__ void k_forces(DevCase D, int step)
+// (solution.f90:327-330).  One workgroup per step of a batch: 256 threads
+// form the per-node terms of a chunk into LDS, one thread advances the
+// running sums node by node.  The nodes of a wing are contiguous in the list
+// (segment-major order).
+constexpr int FCH = 1024;
+__global__ __launch_bounds__(256) void k_forces(DevCase D, int g0, int step0)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double F[8][2], M[8];
-    for (int w = 0; w < 8; ++w) { F[w][0] = 0.0; F[w][1] = 0.0; M[w] = 0.0; }
-    const int n = *D.nadj;
+    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
+    const int step = step0 + blockIdx.x;
+    __shared__ double terms[FCH][4];
+    __shared__ int wing[FCH];
+    const int tid = threadIdx.x;
+    const int na = *S.nadj;
     const double sc = D.scale;
-    for (int a = 0; a < n; ++a) {
-        int w = D.adj[3 * a + 2];
-        const double* kn = D.kin + ((long long)step * D.nw + w) * KIN;
-        double sa = kn[3], sb = kn[5];
-        double t1 = 2.0 * D.rhoadj[a] * D.glx[a];
-        double t2 = 2.0 * D.rhoadj[a] * D.gly[a];
-        F[w][0] = F[w][0] - t1;
-        F[w][1] = F[w][1] - t2;
-        double x0 = (D.ox + sc * (double)D.adj[3 * a] - (D.xc + sa)) / sc;
-        double y0 = (D.oy + sc * (double)D.adj[3 * a + 1] - (D.yc + sb)) / sc;
-        M[w] = M[w] - x0 * t2 + y0 * t1;
-    }
+    double F0 = 0.0, F1 = 0.0, M = 0.0;
+    int curw = 0;
+    double* hist = D.hist + (long long)step * 3 * D.nw;
     const double U = D.U;
-    double* h = D.hist + (long long)step * 3 * D.nw;
-    for (int w = 0; w < D.nw; ++w) {
-        h[3 * w + 0] = 2.0 * F[w][0] / (U * U * (1.0 / sc));
-        h[3 * w + 1] = 2.0 * F[w][1] / (U * U * (1.0 / sc));
-        h[3 * w + 2] = -2.0 * M[w] / (U * U * (1.0 / (sc * sc)));
+    auto flush = [&](int w) {
+        hist[3 * w + 0] = 2.0 * F0 / (U * U * (1.0 / sc));
+        hist[3 * w + 1] = 2.0 * F1 / (U * U * (1.0 / sc));
+        hist[3 * w + 2] = -2.0 * M / (U * U * (1.0 / (sc * sc)));
+    };
+    for (int base = 0; base < na; base += FCH) {
+        const int cnt = min(FCH, na - base);
+        for (int q = tid; q < cnt; q += 256) {
+            int a = base + q;
+            const int* ad = S.adj + 3 * a;
+            int w = ad[2];
+            const double* kn = D.kin + ((long long)step * D.nw + w) * KIN;
+            double sa = kn[3], sb = kn[5];
+            double rho = S.rhoadj[a];
+            double t1 = 2.0 * rho * S.glx[a];
+            double t2 = 2.0 * rho * S.gly[a];
+            double x0 = (D.ox + sc * (double)ad[0] - (D.xc + sa)) / sc;
+            double y0 = (D.oy + sc * (double)ad[1] - (D.yc + sb)) / sc;
+            terms[q][0] = t1;
+            terms[q][1] = t2;
+            terms[q][2] = x0 * t2;
+            terms[q][3] = y0 * t1;
+            wing[q] = w;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int q = 0; q < cnt; ++q) {
+                int wq = wing[q];
+                if (wq != curw) {
+                    flush(curw);
+                    for (int x = curw + 1; x < wq; ++x) { F0 = F1 = M = 0.0; flush(x); }
+                    curw = wq;
+                    F0 = F1 = M = 0.0;
+                }
+                F0 = F0 - terms[q][0];
+                F1 = F1 - terms[q][1];
+                M = M - terms[q][2] + terms[q][3];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        flush(curw);
+        for (int x = curw + 1; x < D.nw; ++x) { F0 = F1 = M = 0.0; flush(x); }
     }
 }
 

@@ -217,10 +217,10 @@ class LBMIB:
         _check(load().lbmib_set_profiling(self.h, 1 if on else 0))
 
     def timings(self) -> dict:
-        t = np.zeros(6)
+        t = np.zeros(8)
         _check(load().lbmib_get_timings(self.h, _d(t)))
-        return dict(step_ms=t[0], lattice_ms=t[1], ib_ms=t[2], ib_collide_ms=t[3],
-                    edges_ms=t[4], steps=int(t[5]))
+        return dict(step_ms=t[0], lattice_ms=t[1], ib_solve_ms=t[2], ib_collide_ms=t[3],
+                    edges_ms=t[4], ib_geometry_ms=t[5], ib_factor_ms=t[7], steps=int(t[6]))
 
 
 def version() -> str:

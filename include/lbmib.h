@@ -125,11 +125,14 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
 /* Per-phase device time in milliseconds accumulated since the last reset
  * (the reference's cpu_time table, main.f90:338-369, 420-433), measured with
  * HIP events on the streams the kernels run on: [0] whole steps, [1] fused
- * stream+moments+collide lattice kernel, [2] immersed boundary (markers through
- * spread), [3] IB-node collide, [4] outer boundary.  Enabled by
- * lbmib_set_profiling(h, 1); [5] = number of timed steps. */
+ * stream+moments+collide lattice kernel, [2] per-step IB solve (u*, D u*,
+ * triangular solves), [3] IB-node spread + forced collide, [4] outer
+ * boundary, [5] IB geometry of a later step (node set, kernel weights,
+ * band of D D^T; overlapped with the step), [6] number of timed steps,
+ * [7] factorisation of that band (overlapped).  Enabled by
+ * lbmib_set_profiling(h, 1). */
 int lbmib_set_profiling(lbmib_handle* h, int32_t on);
-int lbmib_get_timings(lbmib_handle* h, double* ms6);
+int lbmib_get_timings(lbmib_handle* h, double* ms8);
 
 /* Releases the handle and its device memory. */
 int lbmib_destroy(lbmib_handle* h);
