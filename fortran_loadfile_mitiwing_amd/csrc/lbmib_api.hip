@@ -21,8 +21,9 @@
 
 namespace lbmib {
 // kernels (lbmib_kernels.hip)
-__global__ void k_collide_f64(const double*, double*, int, int, long long, Rates, double*);
-__global__ void k_edges_f64(DevCase, int, double, double*);
+__global__ void k_lattice_f64(DevCase, int, int, double, double*, int);
+int lattice_ring_blocks(int nx, int ny);
+int lattice_blocks(int nx, int ny);
 __global__ void k_geo_clear(DevCase, int);
 __global__ void k_geo_markers(DevCase, int, int);
 __global__ void k_geo_fill(DevCase, int);
@@ -33,11 +34,13 @@ __global__ void k_geo_nsort(DevCase, int);
 __global__ void k_geo_assemble(DevCase, int);
 __global__ void k_geo_factor(DevCase, int);
 __global__ void k_crit_rhs(DevCase, int, int);
-template <int WS> __global__ void k_crit_solve(DevCase, int);
+template <int U> __global__ void k_crit_solve(DevCase, int);
+int solve_unit(int npad);
+int solve_threads();
 __global__ void k_band_f64(DevCase, int, int, double*);
 __global__ void k_forces(DevCase, int, int);
 size_t factor_smem_bytes();
-size_t solve_smem_bytes(int nseg);
+size_t solve_smem_bytes(int npad);
 __global__ void k_residual(const double*, const double*, int, int, long long, double*);
 __global__ void k_nan_check(const double*, long long, int*);
 }  // namespace lbmib
@@ -175,12 +178,13 @@ void zigzag(int nw, int spw, std::vector<int>& perm)
 struct lbmib_handle {
     lbmib_params p;
     DevCase D;
-    // s0 lattice; s1 per-step IB solve; s2 IB geometry of later batches;
-    // s3 forces.  With LBMIB_IB_CUS > 0, s1-s3 run on that many reserved CUs.
+    // s0 lattice; s1 triangular solves (on CUs of their own, see
+    // lbmib_create); s2 IB geometry of later batches and forces of earlier
+    // ones; s3 right-hand sides of the IB system.
     static constexpr int NSTREAM = 4;
     hipStream_t s[NSTREAM] = {};
     static constexpr int RING = 64;  // per-step events of step n live in slot n % RING
-    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {};
+    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {};
     static constexpr int BRING = 8;  // per-batch events of batch j live in slot j % BRING
     hipEvent_t ev_built[BRING] = {}, ev_forces[BRING] = {};
     int batch = 8;                   // steps per geometry batch (slots = 2 * batch)
@@ -198,10 +202,11 @@ struct lbmib_handle {
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
     // device allocations to free
     std::vector<void*> allocs;
-    // profiling
-    bool profiling = false;
-    std::vector<hipEvent_t> evpool;
-    double tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // profiling: phase events on every prof_every-th step (0: off)
+    int prof_every = 0;
+    std::vector<hipEvent_t> evpool, gevpool;
+    int gev_used = 0;
+    double tms[LBMIB_NTIMINGS] = {};
     int dev = 0;
 };
 
@@ -267,8 +272,9 @@ int lbmib_destroy(lbmib_handle* h)
         if (st) (void)hipStreamSynchronize(st);
     for (void* p : h->allocs) (void)hipFree(p);
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : h->gevpool) (void)hipEventDestroy(e);
     for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i]})
+        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i], h->ev_rhs[i]})
             if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < lbmib_handle::BRING; ++i)
         for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]})
@@ -323,33 +329,36 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     } while (0)
     HTRY(hipSetDevice(h->dev));
     {
-        // Optional CU split: the IB streams get LBMIB_IB_CUS CUs, the lattice
-        // stream the rest.  The mask bits are spread over the accelerator's 8
-        // XCDs and a queue whose mask leaves an XCD without CUs is not usable,
-        // so only whole multiples of 8 are taken.
+        // The triangular solves run as one workgroup whose critical wavefront
+        // must not queue behind the lattice kernel's memory traffic or wait
+        // for a CU that the lattice kernel keeps full, so stream s1 gets
+        // LBMIB_IB_CUS CUs (default 8, one per XCD) that no other stream uses.
+        // The mask bits are spread over the accelerator's 8 XCDs and a queue
+        // whose mask leaves an XCD without CUs is not usable, so only whole
+        // multiples of 8 are taken; 0 turns the split off.
         hipDeviceProp_t prop;
         HTRY(hipGetDeviceProperties(&prop, h->dev));
         const int ncu = prop.multiProcessorCount;
-        int ib_cus = 0;
+        int ib_cus = 8;
         if (const char* e = getenv("LBMIB_IB_CUS")) ib_cus = atoi(e);
-        ib_cus = (ib_cus + 7) / 8 * 8;
-        if (ib_cus > 0 && ib_cus < ncu / 2) {
+        ib_cus = std::max(0, ib_cus) / 8 * 8;
+        if (ib_cus > 0 && ib_cus < ncu / 2 && ncu % 8 == 0) {
             const int nwords = (ncu + 31) / 32;
             std::vector<uint32_t> lat(nwords, 0u), ib(nwords, 0u);
             for (int c = 0; c < ncu; ++c) {
                 if (c >= ncu - ib_cus) ib[c / 32] |= 1u << (c % 32);
                 else lat[c / 32] |= 1u << (c % 32);
             }
-            HTRY(hipExtStreamCreateWithCUMask(&h->s[0], nwords, lat.data()));
-            for (int i = 1; i < lbmib_handle::NSTREAM; ++i)
-                HTRY(hipExtStreamCreateWithCUMask(&h->s[i], nwords, ib.data()));
+            for (int i = 0; i < lbmib_handle::NSTREAM; ++i)
+                HTRY(hipExtStreamCreateWithCUMask(&h->s[i], nwords, i == 1 ? ib.data() : lat.data()));
         } else {
             for (int i = 0; i < lbmib_handle::NSTREAM; ++i)
                 HTRY(hipStreamCreateWithFlags(&h->s[i], hipStreamNonBlocking));
         }
     }
     for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t* e : {&h->ev_ib[i], &h->ev_step[i]}) HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        for (hipEvent_t* e : {&h->ev_ib[i], &h->ev_step[i], &h->ev_rhs[i]})
+            HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
     for (int i = 0; i < lbmib_handle::BRING; ++i)
         for (hipEvent_t* e : {&h->ev_built[i], &h->ev_forces[i]})
             HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
@@ -435,7 +444,7 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     D.bw = geometric_bandwidth(h);
     if (D.bw >= ns) D.bw = ns - 1;
     for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i]}) HTRY(hipEventRecord(e, h->s[0]));
+        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i], h->ev_rhs[i]}) HTRY(hipEventRecord(e, h->s[0]));
     for (int i = 0; i < lbmib_handle::BRING; ++i)
         for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]}) HTRY(hipEventRecord(e, h->s[0]));
     HTRY(hipStreamSynchronize(h->s[0]));
@@ -539,12 +548,12 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
     if (D.bw > MAXBW)
         return fail(LBMIB_E_ARG, "immersed-boundary bandwidth %d exceeds the banded solver's limit of %d "
                     "(wings too close together, or markers much denser than the lattice)", D.bw, MAXBW);
+    D.ws = 64;
+    D.npad = (D.nseg + 63) / 64 * 64;
     HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)factor_smem_bytes()));
-    HIP_TRY(hipFuncSetAttribute((const void*)k_crit_solve<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)solve_smem_bytes(D.nseg)));
-    HIP_TRY(hipFuncSetAttribute((const void*)k_crit_solve<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)solve_smem_bytes(D.nseg)));
+    HIP_TRY(hipFuncSetAttribute(solve_unit(D.npad) == 64 ? (const void*)k_crit_solve<64> : (const void*)k_crit_solve<32>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_smem_bytes(D.npad)));
     double* dkin;
     if (dalloc(h, &dkin, kin.size())) return LBMIB_E_HIP;
     HIP_TRY(hipMemcpy(dkin, kin.data(), sizeof(double) * kin.size(), hipMemcpyHostToDevice));
@@ -581,14 +590,16 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         dd.dval = o; o += al(ns * BOXMAX);
         dd.nmkv = o; o += al(cap * NCAP);
         dd.G = o; o += al(ns * W);
-        dd.L = o; o += al(ns * W);
-        dd.LT = o; o += al(ns * W);
+        dd.coef = o; o += al((long long)(2 * D.npad + 64) * 2 * D.ws);
+        dd.dinv = o; o += al(D.npad);
         dd.rhoadj = o; o += al(cap);
         dd.glx = o; o += al(cap);
         dd.gly = o; o += al(cap);
         dd.total = o;
         if (dalloc(h, &D.ipool, (size_t)D.nslot * io.total)) return LBMIB_E_HIP;
         if (dalloc(h, &D.dpool, (size_t)D.nslot * dd.total)) return LBMIB_E_HIP;
+        // the solve coefficients of the padding rows stay zero
+        HIP_TRY(hipMemset(D.dpool, 0, sizeof(double) * (size_t)D.nslot * dd.total));
         for (int g = 0; g < D.nslot; ++g) {
             Slot S = slot_of(D, g);
             if (fill_i32(S.claim, INT_MAX, (size_t)grid, h->s[0])) return LBMIB_E_HIP;
@@ -614,11 +625,13 @@ static int enqueue_batch(lbmib_handle* h, int j, hipEvent_t* ev)
     const int g0 = (n0 - 1) % D.nslot;
     hipStream_t st = h->s[2];
     if (j >= 2) {
+        // the lattice update of its last step reads the node grid; its
+        // forces wait for its IB-node collides
         const int last = (j - 1) * B;  // last step of batch j-2
         HIP_TRY(hipStreamWaitEvent(st, h->ev_step[last % lbmib_handle::RING], 0));
         HIP_TRY(hipStreamWaitEvent(st, h->ev_forces[(j - 2) % lbmib_handle::BRING], 0));
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[7], st));
+    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     const int ns = D.nseg, cap = D.cap;
     hipLaunchKernelGGL(k_geo_clear, dim3((cap + 255) / 256, nb), dim3(256), 0, st, D, g0);
     hipLaunchKernelGGL(k_geo_markers, dim3((ns + 255) / 256, nb), dim3(256), 0, st, D, g0, n0 - 1);
@@ -629,22 +642,22 @@ static int enqueue_batch(lbmib_handle* h, int j, hipEvent_t* ev)
     hipLaunchKernelGGL(k_geo_nsort, dim3(64, nb), dim3(256), 0, st, D, g0);
     long long nbw = (long long)ns * (D.bw + 1);
     hipLaunchKernelGGL(k_geo_assemble, dim3((unsigned)((nbw + 255) / 256), nb), dim3(256), 0, st, D, g0);
-    if (ev) HIP_TRY(hipEventRecord(ev[8], st));
-    if (ev) HIP_TRY(hipEventRecord(ev[9], st));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     hipLaunchKernelGGL(k_geo_factor, dim3(nb), dim3(256), factor_smem_bytes(), st, D, g0);
-    if (ev) HIP_TRY(hipEventRecord(ev[10], st));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     HIP_TRY(hipEventRecord(h->ev_built[j % lbmib_handle::BRING], st));
     h->built_batches = j + 1;
     return 0;
 }
 
-// Forces of steps first..last (one workgroup each) on s3.
+// Forces of steps first..last (one workgroup each) on s2, ahead of the
+// geometry of later batches.
 static int enqueue_forces(lbmib_handle* h, int first, int last)
 {
     if (last < first) return 0;
     DevCase& D = h->D;
-    hipStream_t st = h->s[3];
-    HIP_TRY(hipStreamWaitEvent(st, h->ev_step[last % lbmib_handle::RING], 0));
+    hipStream_t st = h->s[2];
+    HIP_TRY(hipStreamWaitEvent(st, h->ev_ib[last % lbmib_handle::RING], 0));
     hipLaunchKernelGGL(k_forces, dim3(last - first + 1), dim3(256), 0, st, D, (first - 1) % D.nslot, first - 1);
     const int j = (last - 1) / h->batch;
     HIP_TRY(hipEventRecord(h->ev_forces[j % lbmib_handle::BRING], st));
@@ -652,15 +665,23 @@ static int enqueue_forces(lbmib_handle* h, int first, int last)
     return 0;
 }
 
+// Step n.  f[n&1] is written from f[(n-1)&1]:
+//   s3  right-hand side of the IB system (after the lattice update and the IB
+//       nodes of step n-1, and the geometry of batch j)
+//   s1  triangular solves, then the forced collide of the IB nodes
+//   s0  lattice update of every other node (after the IB-node collide of step
+//       n-1: that wrote nodes it reads, and read the array it overwrites)
+// so the IB chain of step n runs beside the lattice update of step n.
+// ev (when profiling this step): [0]/[1] around the lattice update, [2] before
+// the right-hand side, [3] after the solves, [4] after the IB-node collide.
 static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
 {
     DevCase& D = h->D;
-    const int B = h->batch, R = lbmib_handle::RING;
+    const int B = h->batch, R = lbmib_handle::RING, BR = lbmib_handle::BRING;
     const int cur = n & 1, g = (n - 1) % D.nslot, j = (n - 1) / B;
     const int nx = D.nx, ny = D.ny;
     double* macro = flag ? D.macro[cur] : nullptr;
     const int ns = D.nseg, cap = D.cap;
-    bool timed = false;
     if ((n - 1) % B == 0) {
         // a new batch starts: forces of the previous one, geometry of the next
         if (h->forces_upto < n - 1) {
@@ -668,37 +689,42 @@ static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
             if (rc) return rc;
         }
         for (int jj = h->built_batches; jj <= j + 1; ++jj) {
-            bool t = ev && jj == j + 1;
-            int rc = enqueue_batch(h, jj, t ? ev : nullptr);
+            hipEvent_t* gev = nullptr;
+            if (h->prof_every > 0) {
+                while (h->gevpool.size() < (size_t)3 * (h->gev_used + 1)) {
+                    hipEvent_t e;
+                    HIP_TRY(hipEventCreate(&e));
+                    h->gevpool.push_back(e);
+                }
+                gev = &h->gevpool[(size_t)3 * h->gev_used++];
+            }
+            int rc = enqueue_batch(h, jj, gev);
             if (rc) return rc;
-            timed |= t;
         }
     }
-    if (ev && !timed)
-        for (int i = 7; i <= 10; ++i) HIP_TRY(hipEventRecord(ev[i], h->s[2]));
-    // per-step IB solve on s1: needs f of step n-1 and the geometry of batch j
-    hipStream_t s1 = h->s[1], s0 = h->s[0];
-    HIP_TRY(hipStreamWaitEvent(s1, h->ev_step[(n - 1) % R], 0));
-    HIP_TRY(hipStreamWaitEvent(s1, h->ev_built[j % lbmib_handle::BRING], 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], s1));
-    hipLaunchKernelGGL(k_crit_rhs, dim3((ns + 6) / 7), dim3(256), 0, s1, D, g, cur);
-    if (D.bw <= 30)
-        hipLaunchKernelGGL(k_crit_solve<32>, dim3(1), dim3(64), solve_smem_bytes(ns), s1, D, g);
+    hipStream_t s0 = h->s[0], s1 = h->s[1], s3 = h->s[3];
+    hipEvent_t built = h->ev_built[j % BR];
+    HIP_TRY(hipStreamWaitEvent(s3, h->ev_step[(n - 1) % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s3, h->ev_ib[(n - 1) % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s3, built, 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s3));
+    hipLaunchKernelGGL(k_crit_rhs, dim3((ns + 6) / 7), dim3(256), 0, s3, D, g, cur);
+    HIP_TRY(hipEventRecord(h->ev_rhs[n % R], s3));
+    HIP_TRY(hipStreamWaitEvent(s1, h->ev_rhs[n % R], 0));
+    if (solve_unit(D.npad) == 64)
+        hipLaunchKernelGGL(k_crit_solve<64>, dim3(1), dim3(solve_threads()), solve_smem_bytes(D.npad), s1, D, g);
     else
-        hipLaunchKernelGGL(k_crit_solve<64>, dim3(1), dim3(64), solve_smem_bytes(ns), s1, D, g);
+        hipLaunchKernelGGL(k_crit_solve<32>, dim3(1), dim3(solve_threads()), solve_smem_bytes(D.npad), s1, D, g);
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
+    hipLaunchKernelGGL(k_band_f64, dim3((cap + 255) / 256), dim3(256), 0, s1, D, g, cur, macro);
+    if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
     HIP_TRY(hipEventRecord(h->ev_ib[n % R], s1));
-    // lattice on s0
+    HIP_TRY(hipStreamWaitEvent(s0, h->ev_ib[(n - 1) % R], 0));
+    if ((n - 1) % B == 0) HIP_TRY(hipStreamWaitEvent(s0, built, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    hipLaunchKernelGGL(k_collide_f64, dim3((nx - 1 + 255) / 256, ny - 2), dim3(256), 0, s0,
-                       (const double*)D.f[cur ^ 1], D.f[cur], nx, ny, D.N, D.R, macro);
+    hipLaunchKernelGGL(k_lattice_f64, dim3(lattice_blocks(nx, ny)), dim3(256), 0, s0, D, g, cur, h->cs, macro,
+                       lattice_ring_blocks(nx, ny));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
-    hipLaunchKernelGGL(k_edges_f64, dim3((ny - 2 + nx + 255) / 256), dim3(256), 0, s0, D, cur, h->cs, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[6], s0));
-    HIP_TRY(hipStreamWaitEvent(s0, h->ev_ib[n % R], 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[4], s0));
-    hipLaunchKernelGGL(k_band_f64, dim3((cap + 255) / 256), dim3(256), 0, s0, D, g, cur, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[5], s0));
     HIP_TRY(hipEventRecord(h->ev_step[n % R], s0));
     HIP_TRY(hipGetLastError());
     if (flag) h->last_flag_step[cur] = n;
@@ -713,28 +739,31 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
     if (h->steps_done + n_steps > h->max_steps)
         return fail(LBMIB_E_ARG, "step %d beyond the motion table (last step %d)", h->steps_done + n_steps, h->max_steps);
     HIP_TRY(hipSetDevice(h->dev));
-    const int E = 11;
-    if (h->profiling) {
-        while (h->evpool.size() < (size_t)E * n_steps + 2) {
+    const int E = 5, every = h->prof_every;
+    const int nsamp = every > 0 ? (n_steps + every - 1) / every : 0;
+    if (every > 0) {
+        while (h->evpool.size() < (size_t)E * nsamp + 2) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             h->evpool.push_back(e);
         }
+        h->gev_used = 0;
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (h->profiling) {
-        t0 = h->evpool[(size_t)E * n_steps];
-        t1 = h->evpool[(size_t)E * n_steps + 1];
+    if (every > 0) {
+        t0 = h->evpool[(size_t)E * nsamp];
+        t1 = h->evpool[(size_t)E * nsamp + 1];
         HIP_TRY(hipEventRecord(t0, h->s[0]));
     }
     for (int s = 0; s < n_steps; ++s) {
         int n = h->steps_done + 1;
         bool flag = s >= n_steps - 2;
-        int rc = launch_step(h, n, flag, h->profiling ? &h->evpool[(size_t)E * s] : nullptr);
+        hipEvent_t* ev = every > 0 && s % every == 0 ? &h->evpool[(size_t)E * (s / every)] : nullptr;
+        int rc = launch_step(h, n, flag, ev);
         if (rc) return rc;
         h->steps_done = n;
     }
-    if (h->profiling) HIP_TRY(hipEventRecord(t1, h->s[0]));
+    if (every > 0) HIP_TRY(hipEventRecord(t1, h->s[0]));
     // forces of the steps taken so far (the history is complete on return)
     if (h->forces_upto < h->steps_done) {
         int rc = enqueue_forces(h, h->forces_upto + 1, h->steps_done);
@@ -748,20 +777,24 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
         if (err & ERR_CAPACITY) return fail(LBMIB_E_CAPACITY, "an immersed-boundary list overflowed");
         if (err & ERR_SOLVE) return fail(LBMIB_E_SOLVE, "immersed-boundary system not positive definite");
     }
-    if (h->profiling && n_steps > 0) {
+    if (every > 0 && n_steps > 0) {
         float ms;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         h->tms[0] += ms;
-        for (int s = 0; s < n_steps; ++s) {
-            hipEvent_t* ev = &h->evpool[(size_t)E * s];
-            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); h->tms[1] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); h->tms[2] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[4], ev[5])); h->tms[3] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[6])); h->tms[4] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[7], ev[8])); h->tms[5] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[9], ev[10])); h->tms[7] += ms;
+        h->tms[1] += n_steps;
+        for (int q = 0; q < nsamp; ++q) {
+            hipEvent_t* ev = &h->evpool[(size_t)E * q];
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); h->tms[3] += ms;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); h->tms[4] += ms;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4])); h->tms[5] += ms;
         }
-        h->tms[6] += n_steps;
+        h->tms[2] += nsamp;
+        for (int q = 0; q < h->gev_used; ++q) {
+            hipEvent_t* ev = &h->gevpool[(size_t)3 * q];
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); h->tms[6] += ms;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); h->tms[7] += ms;
+        }
+        h->tms[8] += h->gev_used;
     }
     if (h->p.nan_guard) {
         int* flag;
@@ -913,18 +946,20 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
     return 0;
 }
 
-int lbmib_set_profiling(lbmib_handle* h, int32_t on)
+int lbmib_set_profiling(lbmib_handle* h, int32_t every)
 {
     if (!h) return fail(LBMIB_E_ARG, "null handle");
-    h->profiling = on != 0;
+    if (every < 0) return fail(LBMIB_E_ARG, "profiling interval must be >= 0");
+    h->prof_every = every;
     for (double& t : h->tms) t = 0.0;
     return 0;
 }
 
-int lbmib_get_timings(lbmib_handle* h, double* ms8)
+int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n)
 {
-    if (!h || !ms8) return fail(LBMIB_E_ARG, "null argument");
-    for (int i = 0; i < 8; ++i) ms8[i] = h->tms[i];
+    if (!h || !t) return fail(LBMIB_E_ARG, "null argument");
+    for (int i = 0; i < n && i < LBMIB_NTIMINGS; ++i) t[i] = h->tms[i];
+    for (int i = LBMIB_NTIMINGS; i < n; ++i) t[i] = 0.0;
     return 0;
 }
 

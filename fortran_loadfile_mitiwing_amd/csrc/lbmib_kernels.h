@@ -41,8 +41,8 @@ struct SlotDoubles {
     long long dval;     // [nseg][BOXMAX] kernel weights phi_x*phi_y, ascending node order
     long long nmkv;     // [cap][NCAP]
     long long G;        // [nseg][bw+1] band of D D^T, G[p][d] = A(p, p-d)
-    long long L;        // [nseg][bw+1] rows of the factor, L[k][d] = l(k,k-d), L[k][0] = 1/l(k,k)
-    long long LT;       // [nseg][bw+1] columns, LT[k][d] = l(k+d,k), LT[k][0] = 1/l(k,k)
+    long long coef;     // [2*npad+64][ws][2] per-lane coefficients of the solve timeline
+    long long dinv;     // [npad] 1/d_k of G = L D L^T (zero in the padding)
     long long rhoadj;   // [cap] rho at the nodes
     long long glx, gly; // [cap] velocity corrections
     long long total;
@@ -61,6 +61,8 @@ struct DevCase {
     int nw, ppw, spw, nseg; // wings, points per wing, segments per wing, all segments
     int cap;                // IB node capacity
     int bw;                 // lower bandwidth of the zig-zag ordered IB matrix
+    int ws;                 // rows per lane group of the triangular solves (32 or 64)
+    int npad;               // nseg rounded up to a multiple of 64
     int steps_cap;          // rows of kin / hist
     int nslot;
     int gi0, gj0, gw, gh;   // lattice window (0-based origin, size) holding every IB node
@@ -91,7 +93,7 @@ struct DevCase {
 // One slot's arrays.
 struct Slot {
     int *claim, *nodeidx, *box, *draw_idx, *nraw_s, *adj, *nadj, *dcnt, *didx, *nmkcnt, *nmk;
-    double *xb2, *yb2, *ub, *vb, *phi, *draw_val, *nraw_w, *dval, *nmkv, *G, *L, *LT, *rhoadj, *glx, *gly;
+    double *xb2, *yb2, *ub, *vb, *phi, *draw_val, *nraw_w, *dval, *nmkv, *G, *coef, *dinv, *rhoadj, *glx, *gly;
 };
 
 __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
@@ -105,8 +107,8 @@ __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
     S.nmkcnt = ib + D.io.nmkcnt; S.nmk = ib + D.io.nmk;
     S.xb2 = db + D.dof.xb2; S.yb2 = db + D.dof.yb2; S.ub = db + D.dof.ub; S.vb = db + D.dof.vb;
     S.phi = db + D.dof.phi; S.draw_val = db + D.dof.draw_val; S.nraw_w = db + D.dof.nraw_w;
-    S.dval = db + D.dof.dval; S.nmkv = db + D.dof.nmkv; S.G = db + D.dof.G; S.L = db + D.dof.L;
-    S.LT = db + D.dof.LT; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
+    S.dval = db + D.dof.dval; S.nmkv = db + D.dof.nmkv; S.G = db + D.dof.G; S.coef = db + D.dof.coef;
+    S.dinv = db + D.dof.dinv; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
     return S;
 }
 

@@ -1,10 +1,10 @@
 // lbmib_kernels.hip -- gfx950 kernels of the FP64 drop-in timestep.
 //
 // One timestep (reference evolution, solution.f90:269-410) is
-//   lattice:  k_collide_f64   fused pull-stream + moments + MRT collide of every
-//                             interior node, prev -> cur (HBM-bound, 144 B/node)
-//             k_edges_f64     characteristic/NEE outer boundary, prev -> cur ring
-//   IB:       geometry of the next step (node set, kernel weights, Cholesky
+//   lattice:  k_lattice_f64   fused pull-stream + moments + MRT collide of every
+//                             interior node, prev -> cur (HBM-bound, 144 B/node),
+//                             and the characteristic/NEE outer boundary ring
+//   IB:       geometry of the next step (node set, kernel weights, L D L^T
 //             factor of D D^T) on its own stream; per step u* -> D u* ->
 //             triangular solves -> spread + forced collide of the IB nodes;
 //             per-wing forces in fixed order
@@ -33,35 +33,6 @@ __device__ __forceinline__ void pull9(const double* __restrict__ src, long long 
     f[6] = src[6 * N + c - nx + 1];
     f[7] = src[7 * N + c + nx + 1];
     f[8] = src[8 * N + c + nx - 1];
-}
-
-// Fused streaming + physical_variables + colission for all interior nodes
-// (solution.f90:428-643).  The IB nodes are recomputed with forcing by
-// k_band_f64 afterwards; everywhere else gforce is zero and the force moments
-// vanish exactly.  Optionally records rho/u/v (download / residual support).
-__global__ __launch_bounds__(256) void k_collide_f64(const double* __restrict__ src,
-                                                     double* __restrict__ dst, int nx, int ny,
-                                                     long long N, Rates R,
-                                                     double* __restrict__ macro)
-{
-    // blocks start at multiples of 256 nodes so every wave writes whole cache
-    // lines; the ring nodes i = 0 and i = nx-1 belong to k_edges_f64
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y + 1;
-    if (i < 1 || i > nx - 2) return;
-    long long c = (long long)j * nx + i;
-    double f[9];
-    pull9(src, N, nx, c, f);
-    double rho, u, v;
-    moments(f, rho, u, v);
-    if (macro) {
-        macro[c] = rho;
-        macro[N + c] = u;
-        macro[2 * N + c] = v;
-    }
-    collide_node(f, rho, u, v, R, false, 0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
 }
 
 // Moments of a boundary-ring node from its stale f0 (never streamed,
@@ -134,20 +105,20 @@ __device__ void edge_column_state(const DevCase& D, const double* src, double cs
 }
 
 // boundary_condition1 (boundary_condition.f90:288-858) with no periodic
-// edges.  Threads 0..ny-3 take the left and right columns at rows 1..ny-2;
-// threads ny-2..ny-2+nx-1 take the top and bottom rows at column x.  The top
-// and bottom characteristic updates and corner states of the reference are
-// overwritten by rho = rho(.,Ny-1), u = v = 0 before use (658-661, 785-788),
-// so only that effective state is computed.  Corner columns of the walls read
-// the left/right results at rows 1 and ny-2, recomputed here.
-__global__ __launch_bounds__(256) void k_edges_f64(DevCase D, int cur, double cs,
-                                                   double* __restrict__ macro)
+// edges, for ring thread t.  Threads 0..ny-3 take the left and right columns
+// at rows 1..ny-2; threads ny-2..ny-2+nx-1 take the top and bottom rows at
+// column x.  The top and bottom characteristic updates and corner states of
+// the reference are overwritten by rho = rho(.,Ny-1), u = v = 0 before use
+// (658-661, 785-788), so only that effective state is computed.  Corner
+// columns of the walls read the left/right results at rows 1 and ny-2,
+// recomputed here.  Reads only the previous populations and the stale ring
+// f0, so it runs beside the interior update.
+__device__ void ring_thread(const DevCase& D, int cur, double cs, double* __restrict__ macro, int t)
 {
     const int nx = D.nx, ny = D.ny;
     const long long N = D.N;
     const double* src = D.f[cur ^ 1];
     double* dst = D.f[cur];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ny - 2) {
         int j = t + 1;
         for (int col = 0; col < 2; ++col) {
@@ -189,6 +160,53 @@ __global__ __launch_bounds__(256) void k_edges_f64(DevCase D, int cur, double cs
         if (macro) { macro[c] = re; macro[N + c] = ue; macro[2 * N + c] = ve; }
     }
 }
+
+// One lattice update prev -> cur: fused streaming + physical_variables +
+// colission of every interior node (solution.f90:428-643) and the outer
+// boundary (boundary_condition1).  The first rblocks blocks take the ring,
+// the rest the interior rows in blocks that start at multiples of 256 nodes,
+// so every wave writes whole cache lines.  The IB nodes of the step (slot g)
+// are skipped: k_band_f64 collides them with their forcing, concurrently.
+// Everywhere else gforce is zero and the force moments vanish exactly.
+// Optionally records rho/u/v (download / residual support).
+__global__ __launch_bounds__(256) void k_lattice_f64(DevCase D, int g, int cur, double cs,
+                                                     double* __restrict__ macro, int rblocks)
+{
+    int b = blockIdx.x;
+    if (b < rblocks) {
+        ring_thread(D, cur, cs, macro, b * blockDim.x + threadIdx.x);
+        return;
+    }
+    b -= rblocks;
+    const int nx = D.nx;
+    const long long N = D.N;
+    const int gx = (nx - 1 + 255) / 256;
+    const int i = (b % gx) * 256 + threadIdx.x;
+    const int j = b / gx + 1;
+    if (i < 1 || i > nx - 2) return;
+    const unsigned wi = (unsigned)(i - D.gi0), wj = (unsigned)(j - D.gj0);
+    if (wi < (unsigned)D.gw && wj < (unsigned)D.gh) {
+        const int* nodeidx = D.ipool + (long long)g * D.io.total + D.io.nodeidx;
+        if (nodeidx[(long long)wj * D.gw + wi] >= 0) return;
+    }
+    const long long c = (long long)j * nx + i;
+    double f[9];
+    pull9(D.f[cur ^ 1], N, nx, c, f);
+    double rho, u, v;
+    moments(f, rho, u, v);
+    if (macro) {
+        macro[c] = rho;
+        macro[N + c] = u;
+        macro[2 * N + c] = v;
+    }
+    collide_node(f, rho, u, v, D.R, false, 0.0, 0.0);
+    double* dst = D.f[cur];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
+}
+
+int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
+int lattice_blocks(int nx, int ny) { return lattice_ring_blocks(nx, ny) + (nx - 1 + 255) / 256 * (ny - 2); }
 
 // Box search of move_boundary (airfoil.f90:290-321) for one axis: the 1-based
 // index range of nodes with |(o + (i-1)*sc - X)/sc| <= 2.5.
@@ -485,26 +503,32 @@ constexpr int WIN = 64;  // LDS window slots of the banded factorisation (rows k
 constexpr int CH = 32;   // rows / columns staged per chunk
 constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
 
-// Banded Cholesky of G, right-looking over a sliding LDS window of rows
-// k..k+bw (row r in slot r % WIN, entry d = column r-d).  Element (i,j)
-// receives its updates in increasing k, so the result equals the dense
-// right-looking algorithm bit for bit.  Columns are scaled by the reciprocal
-// of the pivot (LAPACK dpotf2).  Rows entering the window are staged through
-// LDS one chunk ahead; the factor leaves through plain stores that are never
-// waited on inside the loop.  Outputs L (rows) and LT (columns), both with
-// 1/l_kk in entry 0.  One workgroup per slot of the batch.
+// Banded L D L^T of G, right-looking over a sliding LDS window of rows
+// k..k+bw.  Step k takes the pivot d_k = a(k,k), forms the column
+// l(i,k) = t_i * (1/d_k) from its unscaled entries t_i and updates the
+// trailing band a(i,j) -= l(i,k) * t_j.  Each entry receives its updates in
+// increasing k, so the result equals the dense right-looking algorithm bit
+// for bit.  Rows entering the window are staged through LDS one chunk ahead;
+// results leave through plain stores that are never waited on inside the
+// loop.  Besides 1/d_k the kernel writes the coefficients of step k of both
+// triangular solves in the per-lane order of k_crit_solve (see there), the
+// forward step at timeline position k, the backward one at 2*npad-1-k:
+//   forward  [h] = {l(k+e1,k), l(k+e2,k)}, e1 = h-q (h > q), e2 = ws+h-q (h <= q)
+//   backward [h] = {l(k,k-e1), l(k,k-e2)}, e1 = q-h (h < q), e2 = ws+q-h (h >= q)
+// with q = k mod ws, zero outside the band.  One workgroup per slot.
 __global__ __launch_bounds__(256) void k_geo_factor(DevCase D, int g0)
 {
     const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
     extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
-    const int n = D.nseg, bw = D.bw, W = bw + 1;
+    const int n = D.nseg, bw = D.bw, W = bw + 1, ws = D.ws;
     double* win = smem;
     double* colv = win + WIN * WIN;
-    double* stage = colv + WIN;  // [2][CH][WIN]
+    double* colt = colv + WIN;
+    double* stage = colt + WIN;  // [2][CH][WIN]
     int* pairs = (int*)(stage + 2 * CH * WIN);
-    double* L = S.L;
-    double* LT = S.LT;
+    double* coef = S.coef;
+    const int np = D.npad;
     const double* G = S.G;
     for (int a = tid; a < bw; a += T)
         for (int b = 0; b <= a; ++b) pairs[a * (a + 1) / 2 + b] = (a << 8) | b;
@@ -543,14 +567,15 @@ __global__ __launch_bounds__(256) void k_geo_factor(DevCase D, int g0)
         const int m = min(k + bw, n - 1) - k;
         const int sk = (k % WIN) * WIN;
         const double akk = win[sk];
-        const double lkk = sqrt(akk);
-        const double rk = 1.0 / lkk;
+        const double rk = 1.0 / akk;
         if (tid < m) {
             int i = k + 1 + tid;
             int e = (i % WIN) * WIN + (i - k);
-            double l = win[e] * rk;
+            double t = win[e];
+            double l = t * rk;
             win[e] = l;
             colv[tid] = l;
+            colt[tid] = t;
         }
         lds_barrier();
         if (tid == 0 && !(akk > 0.0)) atomicOr(D.err, ERR_SOLVE);
@@ -560,12 +585,23 @@ __global__ __launch_bounds__(256) void k_geo_factor(DevCase D, int g0)
             int a = ab >> 8, b = ab & 255;
             int i = k + 1 + a, j = k + 1 + b;
             int e = (i % WIN) * WIN + (i - j);
-            win[e] = win[e] - colv[a] * colv[b];
+            win[e] = win[e] - colv[a] * colt[b];
         }
-        for (int d = tid; d < W; d += T) {
-            // column k of L, and row k (final now)
-            LT[(long long)k * W + d] = d == 0 ? rk : (d <= m ? colv[d - 1] : 0.0);
-            L[(long long)k * W + d] = d == 0 ? rk : (d <= k ? win[sk + d] : 0.0);
+        // solve coefficients of step k: column k (final now) and row k (final
+        // since step k-1)
+        {
+            const int q = k & (ws - 1);
+            const int hh = (tid >> 1) & (ws - 1), j = tid & 1;
+            if (tid < 2 * ws) {
+                const int e = j == 0 ? hh - q : ws + hh - q;
+                const bool ok = (j == 0 ? hh > q : hh <= q) && e <= m;
+                coef[(long long)k * 2 * ws + tid] = ok ? colv[e - 1] : 0.0;
+            } else if (tid < 4 * ws) {
+                const int e = j == 0 ? q - hh : ws + q - hh;
+                const bool ok = (j == 0 ? hh < q : hh >= q) && e <= min(bw, k);
+                coef[(long long)(2 * np - 1 - k) * 2 * ws + tid - 2 * ws] = ok ? win[sk + e] : 0.0;
+            }
+            if (tid == 0) S.dinv[k] = rk;
         }
         // row k+W enters the window (its slot differs from every row in use)
         const int rn = k + W;
@@ -592,6 +628,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
 __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
 {
+    __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const Slot S = slot_of(D, g);
     __shared__ double pu[RHS_SEG][BOXMAX], pv[RHS_SEG][BOXMAX];
     const int sl = threadIdx.x / BOXMAX, q = threadIdx.x % BOXMAX;
@@ -618,180 +655,183 @@ __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
     }
 }
 
-// Forward L z = b and backward L^T x = z for both right-hand sides in one
-// wavefront.  With a window of WS rows (WS = 32: u in lanes 0-31, v in lanes
-// 32-63, bw <= 30; WS = 64: both in every lane, bw <= 62), lane h of a group
-// holds the running value of the row r == h (mod WS) of the active window.
-// The owner of row k forms z_k = b_k * (1/l_kk), every lane of the group
-// receives it by readlane and applies its l(r,k) z_k, so element r gets its
-// updates in increasing (forward) / decreasing (backward) k exactly as the
-// column-oriented algorithm applies them.  A lane's next row is read from
-// LDS right after it finishes the current one and swapped in a few steps
-// later, before that row's first update.  Columns / rows of L are staged in
-// LDS one chunk ahead and each step's entries are read one step ahead.
-template <int WS>
-__global__ __launch_bounds__(64) void k_crit_solve(DevCase D, int g)
+// Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
+// right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
+// 64: lane h keeps both right-hand sides of row c*64 + h of the active chunk
+// c ("cur") and of row (c+1)*64 + h (forward) / (c-1)*64 + h (backward) of
+// its neighbour ("nxt"), which covers every row within bw <= 64 of the pivot.
+// Step k (q = k mod 64) takes the finished value of row k from lane q by
+// readlane and every lane applies
+//   cur -= c1 * z_k,   nxt -= c2 * z_k
+// with the two coefficients k_geo_factor laid out for this lane (zero outside
+// the band).  A zero coefficient leaves a finite value unchanged, so every
+// row receives exactly the oracle's updates in increasing (forward) /
+// decreasing (backward) k, and the step has no lane-dependent control flow.
+//
+// The 2*npad steps form one timeline (forward steps, then backward steps);
+// their coefficients lie in that order in coef, 1 KB per step.  Wavefront 0
+// runs the steps and touches only LDS and registers: a global load in its
+// instruction stream would wait for the CU's vector-memory queue, which the
+// lattice kernel keeps full, and stall the whole dependency chain.  The
+// SV_NPROD producer wavefronts stream units of U steps from HBM into a
+// double-buffered LDS ring, loading each unit two units before it is used;
+// one workgroup barrier per unit hands the buffers over.
+constexpr int SV_NPROD = 8;  // producer wavefronts
+typedef double sv_d2 __attribute__((ext_vector_type(2)));  // register-friendly pair
+constexpr int SV_P = 4;      // steps of coefficients read ahead from LDS, divides U
+
+template <bool FWD, int U>
+__device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane, int i0, double (&cur)[2],
+                                        double (&nxt)[2])
 {
-    const Slot S = slot_of(D, g);
-    extern __shared__ double zb[];                // [2][n]
-    __shared__ double stage[2 * CH * WIN];        // [2][CH][WIN]
-    constexpr int NR = WS == 32 ? 1 : 2;          // right-hand sides per lane
-    const int lane = threadIdx.x, h = lane & (WS - 1);
-    const int r0 = WS == 32 ? (lane >> 5) : 0;    // first right-hand side of this lane
-    const int n = D.nseg, bw = D.bw, W = bw + 1;
-    constexpr int PER = (CH * WIN + 63) / 64;
-    double pre[PER];
-    const double* LT = S.LT;
-    const double* L = S.L;
-    for (int e = lane; e < 2 * n; e += 64) zb[e] = D.rhs[e];
-    auto load_chunk = [&](const double* src, int c) {
+    double2 cf[SV_P];
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = lane + q * 64;
-            int r = c * CH + e / W, d = e % W;
-            pre[q] = (c >= 0 && e < CH * W && r < n) ? src[(long long)r * W + d] : 0.0;
-        }
-    };
-    auto store_chunk = [&](int buf) {
+    for (int i = 0; i < SV_P; ++i) cf[i] = rb[i * 64 + lane];
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = lane + q * 64;
-            if (e < CH * W) stage[buf * CH * WIN + (e / W) * WIN + e % W] = pre[q];
-        }
-    };
-    auto bcast = [&](double v, int owner) -> double {
-        if (WS == 32) {
-            double a0 = readlane_f64(v, owner);
-            double a1 = readlane_f64(v, owner + 32);
-            return r0 ? a1 : a0;
-        }
-        return readlane_f64(v, owner);
-    };
-    const int take = (bw + 1) & (WS - 1);
-    const int nch = (n + CH - 1) / CH;
-    double b[NR], nb[NR];
-    // ---- forward ----
-    load_chunk(LT, 0);
-    store_chunk(0);
-    load_chunk(LT, 1);
-    lds_barrier();
-#pragma unroll
-    for (int r = 0; r < NR; ++r) { b[r] = h < n ? zb[(r0 + r) * n + h] : 0.0; nb[r] = 0.0; }
-    for (int c = 0; c < nch; ++c) {
-        const double* colbase = stage + (c & 1) * CH * WIN;
-        double rk_n = colbase[0];
-        double lv_n = colbase[(h - c * CH) & (WS - 1)];
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-            const int k = c * CH + q;
-            if (k < n) {
-                const double rk = rk_n, lv = lv_n;
-                if (q + 1 < CH) {
-                    rk_n = colbase[(q + 1) * WIN];
-                    lv_n = colbase[(q + 1) * WIN + ((h - k - 1) & (WS - 1))];
-                }
-                const int d = (h - k) & (WS - 1);   // this lane's row is k + d
-                const int owner = k & (WS - 1);
-                double z[NR];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) z[r] = bcast(b[r] * rk, owner);
-                const bool upd = d >= 1 && d <= bw && k + d < n;
-                const bool tk = d == take && k + bw + 1 < n && k + bw + 1 >= WS;
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    double nv = b[r] - lv * z[r];
-                    b[r] = upd ? nv : b[r];
-                    b[r] = tk ? nb[r] : b[r];
-                }
-                if (d == 0) {
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) {
-                        zb[(r0 + r) * n + k] = z[r];
-                        if (k + WS < n) nb[r] = zb[(r0 + r) * n + k + WS];
-                    }
-                }
-            }
-        }
-        if (c + 1 < nch) {
-            store_chunk((c + 1) & 1);
-            load_chunk(LT, c + 2);
-            lds_barrier();
-        }
-    }
-    lds_barrier();
-    // ---- backward ----
-    {
-        int c = nch - 1;
-        load_chunk(L, c);
-        store_chunk(c & 1);
-        load_chunk(L, c - 1);
-        lds_barrier();
-        // lane holds the row r == h (mod WS) in [n-WS, n-1]
-        const int rr = (n - 1) - ((((n - 1 - h) % WS) + WS) % WS);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) { b[r] = rr >= 0 ? zb[(r0 + r) * n + rr] : 0.0; nb[r] = 0.0; }
-        for (; c >= 0; --c) {
-            const double* rowbase = stage + (c & 1) * CH * WIN;
-            double rk_n = rowbase[(CH - 1) * WIN];
-            double lv_n = rowbase[(CH - 1) * WIN + ((c * CH + CH - 1 - h) & (WS - 1))];
-#pragma unroll
-            for (int q = CH - 1; q >= 0; --q) {
-                const int k = c * CH + q;
-                const double rk = rk_n, lv = lv_n;
-                if (q > 0) {
-                    rk_n = rowbase[(q - 1) * WIN];
-                    lv_n = rowbase[(q - 1) * WIN + ((k - 1 - h) & (WS - 1))];
-                }
-                if (k < n) {
-                    const int d = (k - h) & (WS - 1);   // this lane's row is k - d
-                    const int owner = k & (WS - 1);
-                    double x[NR];
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) x[r] = bcast(b[r] * rk, owner);
-                    const bool upd = d >= 1 && d <= bw && k - d >= 0;
-                    const bool tk = d == take && k - bw - 1 >= 0 && k - bw - 1 < n - WS;
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) {
-                        double nv = b[r] - lv * x[r];
-                        b[r] = upd ? nv : b[r];
-                        b[r] = tk ? nb[r] : b[r];
-                    }
-                    if (d == 0) {
-#pragma unroll
-                        for (int r = 0; r < NR; ++r) {
-                            zb[(r0 + r) * n + k] = x[r];
-                            if (k - WS >= 0) nb[r] = zb[(r0 + r) * n + k - WS];
-                        }
-                    }
-                }
-            }
-            if (c > 0) {
-                store_chunk((c - 1) & 1);
-                load_chunk(L, c - 2);
-                lds_barrier();
-            }
-        }
-    }
-    lds_barrier();
-    for (int p = lane; p < n; p += 64) {
-        int s = D.perm[p];
-        D.y[s] = zb[p];
-        D.y[n + s] = zb[n + p];
+    for (int i = 0; i < U; ++i) {
+        const double2 c = cf[i % SV_P];
+        if (i + SV_P < U) cf[i % SV_P] = rb[(i + SV_P) * 64 + lane];
+        const int q = FWD ? i0 + i : 63 - i0 - i;
+        const double z0 = readlane_f64(cur[0], q);
+        const double z1 = readlane_f64(cur[1], q);
+        cur[0] = cur[0] - c.x * z0;
+        cur[1] = cur[1] - c.x * z1;
+        nxt[0] = nxt[0] - c.y * z0;
+        nxt[1] = nxt[1] - c.y * z1;
+        // keep this step's updates in this step
+        asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(nxt[0]), "+v"(nxt[1]));
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
-template __global__ void k_crit_solve<32>(DevCase, int);
-template __global__ void k_crit_solve<64>(DevCase, int);
 
-size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + WIN + 2 * CH * WIN) + sizeof(int) * PAIRMAX; }
-size_t solve_smem_bytes(int nseg) { return sizeof(double) * 2 * (size_t)nseg; }
+template <int U>
+__global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(DevCase D, int g)
+{
+    constexpr int UPC = 64 / U;              // units per lane chunk
+    constexpr int SPW = U / SV_NPROD;        // steps per producer and unit
+    __shared__ double2 ring[2 * U * 64];     // two units
+    extern __shared__ double zb[];           // [2][npad] values, then [npad] 1/d
+    const Slot S = slot_of(D, g);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int n = D.nseg, np = D.npad, nch = np / 64, nun = 2 * nch * UPC;
+    double* dv = zb + 2 * np;
+    for (int e = tid; e < 2 * np; e += blockDim.x) {
+        const int r = e >= np, p = e - r * np;
+        zb[e] = p < n ? D.rhs[r * n + p] : 0.0;
+    }
+    for (int p = tid; p < np; p += blockDim.x) dv[p] = S.dinv[p];
+    if (wave > 0) {
+        // ---- producer: unit u, steps u*U + w*SPW + s ----
+        const int w = wave - 1;
+        const sv_d2* coef = (const sv_d2*)S.coef + (long long)w * SPW * 64 + lane;
+        sv_d2* rg = (sv_d2*)ring;
+        sv_d2 A[SPW], B[SPW];
+        // units past the end reload the last one and are never read
+#define SV_LOAD(R, U_)                                                              \
+    {                                                                               \
+        const long long u_ = min((U_), nun - 1);                                    \
+        _Pragma("unroll") for (int s = 0; s < SPW; ++s)                             \
+            R[s] = coef[(u_ * U + s) * 64];                                         \
+    }
+#define SV_STORE(R, U_)                                                             \
+    {                                                                               \
+        _Pragma("unroll") for (int s = 0; s < SPW; ++s)                             \
+            rg[((U_) & 1) * U * 64 + (w * SPW + s) * 64 + lane] = R[s];             \
+    }
+        SV_LOAD(A, 0)
+        SV_LOAD(B, 1)
+        SV_STORE(A, 0)
+        SV_LOAD(A, 2)
+        __syncthreads();
+        for (int u = 0; u < nun; u += 2) {
+            SV_STORE(B, u + 1)
+            SV_LOAD(B, u + 3)
+            __syncthreads();
+            SV_STORE(A, u + 2)
+            SV_LOAD(A, u + 4)
+            __syncthreads();
+        }
+#undef SV_LOAD
+#undef SV_STORE
+    } else {
+        // ---- the chain ----
+        __syncthreads();
+        double cur[2], nxt[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            cur[r] = zb[r * np + lane];
+            nxt[r] = nch > 1 ? zb[r * np + 64 + lane] : 0.0;
+        }
+        // forward, then x = z * (1/d)
+        for (int c = 0; c < nch; ++c) {
+#pragma unroll
+            for (int j = 0; j < UPC; ++j) {
+                const int u = c * UPC + j;
+                sv_unit<true, U>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                if (j == UPC - 1) {
+                    const int row = c * 64 + lane;
+                    const double di = dv[row];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        zb[r * np + row] = cur[r] * di;
+                        cur[r] = nxt[r];
+                        nxt[r] = c + 2 < nch ? zb[r * np + row + 128] : 0.0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // backward; lane h starts on rows (nch-1)*64 + h and (nch-2)*64 + h,
+        // which it finished itself
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            cur[r] = zb[r * np + (nch - 1) * 64 + lane];
+            nxt[r] = nch > 1 ? zb[r * np + (nch - 2) * 64 + lane] : 0.0;
+        }
+        for (int c = nch - 1; c >= 0; --c) {
+#pragma unroll
+            for (int j = 0; j < UPC; ++j) {
+                const int u = (2 * nch - 1 - c) * UPC + j;
+                sv_unit<false, U>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                if (j == UPC - 1) {
+                    const int row = c * 64 + lane;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        zb[r * np + row] = cur[r];
+                        cur[r] = nxt[r];
+                        nxt[r] = c >= 2 ? zb[r * np + row - 128] : 0.0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < n; p += blockDim.x) {
+        const int s = D.perm[p];
+        D.y[s] = zb[p];
+        D.y[n + s] = zb[np + p];
+    }
+}
+template __global__ void k_crit_solve<64>(DevCase, int);
+template __global__ void k_crit_solve<32>(DevCase, int);
+
+// The 64-step staging unit when its two LDS buffers (128 KB) fit beside the
+// right-hand sides, the 32-step one otherwise.
+int solve_unit(int npad) { return 3 * 8 * (size_t)npad + 2 * 64 * 64 * 16 <= 160 * 1024 ? 64 : 32; }
+int solve_threads() { return 64 * (1 + SV_NPROD); }
+
+size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + 2 * WIN + 2 * CH * WIN) + sizeof(int) * PAIRMAX; }
+size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
 
 // Spread delta_u = sum_k D(k,a) y(k) in ascending k (the fixed-order gather in
 // place of the spread DGEMV, boundary_condition.f90:1639-1640), velocity
 // correction and body force (1647-1657), and the forced collision
-// (solution.f90:598-638).  Runs after k_collide_f64 and overwrites its
-// unforced result at the IB nodes.
+// (solution.f90:598-638) of the IB nodes, which k_lattice_f64 leaves out;
+// the two run concurrently.
 __global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, double* __restrict__ macro)
 {
+    __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const Slot S = slot_of(D, g);
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= *S.nadj) return;
@@ -823,7 +863,7 @@ __global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, dou
     double* dst = D.f[cur];
 #pragma unroll
     for (int k = 0; k < 9; ++k) dst[k * D.N + c] = f[k];
-    if (macro) { macro[D.N + c] = u; macro[2 * D.N + c] = v; }
+    if (macro) { macro[c] = r; macro[D.N + c] = u; macro[2 * D.N + c] = v; }
 }
 
 // Per-wing force and moment, accumulated in node order

@@ -35,6 +35,7 @@ LIB_PATH = os.path.join(_HERE, "liblbmib.so")
 
 PREC_FP64 = 0
 PREC_FP32DEV = 1
+NTIMINGS = 9  # LBMIB_NTIMINGS
 
 
 class LBMIBError(RuntimeError):
@@ -83,7 +84,7 @@ def load(path: str = LIB_PATH):
     L.lbmib_residual.argtypes = [vp, _dp]
     L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
-    L.lbmib_get_timings.argtypes = [vp, _dp]
+    L.lbmib_get_timings.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_destroy.argtypes = [vp]
     L.lbmib_last_error.restype = C.c_char_p
     L.lbmib_version.restype = C.c_char_p
@@ -213,14 +214,18 @@ class LBMIB:
                                 w.ctypes.data_as(_ip), _d(gx), _d(gy)))
         return dict(i=pi, j=pj, wing=w, glx=gx, gly=gy)
 
-    def set_profiling(self, on: bool = True):
-        _check(load().lbmib_set_profiling(self.h, 1 if on else 0))
+    def set_profiling(self, every: int = 1):
+        """Record phase timings on every ``every``-th step (0: off); resets them."""
+        _check(load().lbmib_set_profiling(self.h, int(every)))
 
     def timings(self) -> dict:
-        t = np.zeros(8)
-        _check(load().lbmib_get_timings(self.h, _d(t)))
-        return dict(step_ms=t[0], lattice_ms=t[1], ib_solve_ms=t[2], ib_collide_ms=t[3],
-                    edges_ms=t[4], ib_geometry_ms=t[5], ib_factor_ms=t[7], steps=int(t[6]))
+        """Averages of the recorded phases (milliseconds) and their counts."""
+        t = np.zeros(NTIMINGS)
+        _check(load().lbmib_get_timings(self.h, _d(t), NTIMINGS))
+        ns, nb = max(t[2], 1.0), max(t[8], 1.0)
+        return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / ns, ib_solve_ms=t[4] / ns,
+                    ib_collide_ms=t[5] / ns, geometry_batch_ms=t[6] / nb, factor_batch_ms=t[7] / nb,
+                    steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]))
 
 
 def version() -> str:

@@ -122,17 +122,23 @@ int lbmib_residual(lbmib_handle* h, double* err);
 int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_t* wing,
                    double* glx, double* gly);
 
-/* Per-phase device time in milliseconds accumulated since the last reset
- * (the reference's cpu_time table, main.f90:338-369, 420-433), measured with
- * HIP events on the streams the kernels run on: [0] whole steps, [1] fused
- * stream+moments+collide lattice kernel, [2] per-step IB solve (u*, D u*,
- * triangular solves), [3] IB-node spread + forced collide, [4] outer
- * boundary, [5] IB geometry of a later step (node set, kernel weights,
- * band of D D^T; overlapped with the step), [6] number of timed steps,
- * [7] factorisation of that band (overlapped).  Enabled by
- * lbmib_set_profiling(h, 1). */
-int lbmib_set_profiling(lbmib_handle* h, int32_t on);
-int lbmib_get_timings(lbmib_handle* h, double* ms8);
+/* Device timing of the hot path (the reference's cpu_time table,
+ * main.f90:338-369, 420-433), measured with HIP events on the streams the
+ * kernels run on.  lbmib_set_profiling(h, every) resets the counters and,
+ * for every > 0, records the phase events of every every-th step and of each
+ * geometry batch (0 turns profiling off; events cost a few microseconds per
+ * sampled step).  lbmib_get_timings copies min(n, LBMIB_NTIMINGS) values:
+ *   [0] ms of all steps taken while profiling   [1] those steps
+ *   [2] sampled steps                            [3] ms in the lattice update
+ *       (fused stream + collide + outer boundary, all nodes but the IB nodes)
+ *   [4] ms from the IB right-hand side to the end of the triangular solves
+ *   [5] ms in the forced collide of the IB nodes ([3]-[5] summed over the
+ *       sampled steps)
+ *   [6] ms of geometry batches before the factorisation, [7] ms of their
+ *       factorisations, [8] batches timed (geometry overlaps the steps). */
+#define LBMIB_NTIMINGS 9
+int lbmib_set_profiling(lbmib_handle* h, int32_t every);
+int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 
 /* Releases the handle and its device memory. */
 int lbmib_destroy(lbmib_handle* h);

@@ -26,9 +26,10 @@
  *     pivot), division by the pivot, column-oriented triangular solves.
  *   - solver=0 (default) solves the mathematically identical SPD system
  *     (D D^T) y = b, since A = D D^T S and delta_u = D^T S A^-1 b = D^T y
- *     (S cancels), by Cholesky in a zig-zag marker order (LE outwards along
- *     both surfaces), pivots applied through their reciprocals as LAPACK's
- *     dpotf2 does.  The tests pin it against solver=1 and numpy.
+ *     (S cancels), by square-root-free Cholesky (L D L^T) in a zig-zag
+ *     marker order (LE outwards along both surfaces), pivots applied
+ *     through their reciprocals.  The tests pin it against solver=1 and
+ *     numpy.
  *   - asin (delta kernel) is evaluated with fdlibm's published e_asin method
  *     (rational approximation, Sun Microsystems 1993), implemented with plain
  *     IEEE operations so that the GPU evaluates the identical sequence.
@@ -725,46 +726,49 @@ static void zigzag_perm(int nw, int spw, int* perm)
     }
 }
 
-/* Cholesky of the dense symmetric matrix a (n x n row-major, lower triangle
- * used), right-looking, element (i,j) updated in increasing k.  As LAPACK's
- * dpotf2 does, the column is scaled by the reciprocal of the pivot; rinv[k]
- * keeps 1/l_kk for the solves.  bw bounds the lower bandwidth of the profile
- * (n-1 = dense).  Returns 0 or -5. */
-static int cholesky(double* a, int n, int bw, double* rinv)
+/* Square-root-free Cholesky (L D L^T) of the symmetric matrix a (n x n
+ * row-major, lower triangle used), right-looking, element (i,j) updated in
+ * increasing k: with t_i = a(i,k) before scaling and l_i = t_i * (1/d_k),
+ * a(i,j) -= l_i * t_j.  dinv[k] keeps 1/d_k for the solves; L (unit lower)
+ * replaces the strict lower triangle.  bw bounds the lower bandwidth of the
+ * profile (n-1 = dense).  Returns 0 or -5 (pivot not positive). */
+static int ldlt(double* a, int n, int bw, double* dinv)
 {
+    double* t = (double*)malloc(sizeof(double) * (size_t)(bw + 1));
     for (int k = 0; k < n; ++k) {
         double d = a[(size_t)k * n + k];
-        if (!(d > 0.0)) return -5;
-        double lkk = sqrt(d);
-        double r = 1.0 / lkk;
-        a[(size_t)k * n + k] = lkk;
-        rinv[k] = r;
+        if (!(d > 0.0)) { free(t); return -5; }
+        double r = 1.0 / d;
+        dinv[k] = r;
         int iend = k + bw < n - 1 ? k + bw : n - 1;
-        for (int i = k + 1; i <= iend; ++i) a[(size_t)i * n + k] = a[(size_t)i * n + k] * r;
+        for (int i = k + 1; i <= iend; ++i) {
+            t[i - k - 1] = a[(size_t)i * n + k];
+            a[(size_t)i * n + k] = t[i - k - 1] * r;
+        }
         for (int j = k + 1; j <= iend; ++j) {
-            double ljk = a[(size_t)j * n + k];
+            double tj = t[j - k - 1];
             for (int i = j; i <= iend; ++i) {
                 double lik = a[(size_t)i * n + k];
-                a[(size_t)i * n + j] = a[(size_t)i * n + j] - lik * ljk;
+                a[(size_t)i * n + j] = a[(size_t)i * n + j] - lik * tj;
             }
         }
     }
+    free(t);
     return 0;
 }
 
-/* forward L z = b then backward L^T x = z, column oriented (nrhs columns of
- * length n stored consecutively); the pivots enter through their reciprocals */
-static void chol_solve(const double* a, const double* rinv, int n, int bw, double* b, int nrhs)
+/* forward L z = b, scaling x = D^-1 z, backward L^T y = x, column oriented
+ * (nrhs columns of length n stored consecutively) */
+static void ldlt_solve(const double* a, const double* dinv, int n, int bw, double* b, int nrhs)
 {
     for (int r = 0; r < nrhs; ++r) {
         double* x = b + (size_t)r * n;
         for (int k = 0; k < n; ++k) {
-            x[k] = x[k] * rinv[k];
             int iend = k + bw < n - 1 ? k + bw : n - 1;
             for (int i = k + 1; i <= iend; ++i) x[i] = x[i] - a[(size_t)i * n + k] * x[k];
         }
+        for (int k = 0; k < n; ++k) x[k] = x[k] * dinv[k];
         for (int k = n - 1; k >= 0; --k) {
-            x[k] = x[k] * rinv[k];
             int ibeg = k - bw > 0 ? k - bw : 0;
             for (int i = ibeg; i < k; ++i) x[i] = x[i] - a[(size_t)k * n + i] * x[k];
         }
@@ -824,14 +828,14 @@ int orc_solve(const double* G, const double* S, const double* b, int n,
             for (int q = 0; q < n; ++q) a[(size_t)p * n + q] = G[(size_t)perm[p] * n + perm[q]];
         for (int r = 0; r < 2; ++r)
             for (int p = 0; p < n; ++p) x[r * n + p] = b[r * n + perm[p]];
-        double* rinv = (double*)malloc(sizeof(double) * n);
-        rc = cholesky(a, n, n - 1, rinv);
+        double* dinv = (double*)malloc(sizeof(double) * n);
+        rc = ldlt(a, n, n - 1, dinv);
         if (rc == 0) {
-            chol_solve(a, rinv, n, n - 1, x, 2);
+            ldlt_solve(a, dinv, n, n - 1, x, 2);
             for (int r = 0; r < 2; ++r)
                 for (int p = 0; p < n; ++p) y[r * n + perm[p]] = x[r * n + p];
         }
-        free(rinv);
+        free(dinv);
     } else {
         for (int p = 0; p < n; ++p)
             for (int q = 0; q < n; ++q) a[(size_t)p * n + q] = G[(size_t)p * n + q] * S[q];
@@ -1019,14 +1023,14 @@ static int immersed_boundary(orc_case* c)
         double* x = (double*)malloc(sizeof(double) * 2 * ns);
         for (int r = 0; r < 2; ++r)
             for (int p = 0; p < ns; ++p) x[r * ns + p] = rhs[r * ns + perm[p]];
-        double* rinv = (double*)malloc(sizeof(double) * ns);
-        rc = cholesky(G, ns, bw, rinv);
+        double* dinv = (double*)malloc(sizeof(double) * ns);
+        rc = ldlt(G, ns, bw, dinv);
         if (rc == 0) {
-            chol_solve(G, rinv, ns, bw, x, 2);
+            ldlt_solve(G, dinv, ns, bw, x, 2);
             for (int r = 0; r < 2; ++r)
                 for (int p = 0; p < ns; ++p) y[r * ns + perm[p]] = x[r * ns + p];
         }
-        free(rinv);
+        free(dinv);
         free(x);
         free(perm);
     }

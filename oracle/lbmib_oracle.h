@@ -31,7 +31,7 @@ typedef struct orc_params {
     int smooth;            /* 1: smoothed 3-point kernel, 0: Peskin 4-point (boundary_condition.f90:1530,1563-1600) */
     int n_wings;           /* wing_count (main.f90:18) */
     int pts_per_wing;      /* boundary_points_per_wing, closed polygon (main.f90:21) */
-    int solver;            /* 0: SPD Cholesky of D D^T in zig-zag order; 1: dense LU with partial pivoting of D D^T S */
+    int solver;            /* 0: L D L^T of D D^T in zig-zag order; 1: dense LU with partial pivoting of D D^T S */
     int dense_ib;          /* 1: literal dense Nseg x Nadj matrices; 0: box-sparse evaluation (bit-identical) */
     int n_threads;         /* OpenMP threads for the lattice passes (results do not depend on it) */
 } orc_params;
@@ -96,7 +96,7 @@ int orc_motion_step_table(const double* motion, int rows, int ncols, double dt,
                           double* out, int max_steps);
 /* Solve the IB system for a given dense symmetric G (n x n row-major), rhs b
  * (n x 2, column k at b+k*n), scaling S (arc lengths), with solver 0 (SPD
- * Cholesky in the given order perm) or 1 (LU of G*S with partial pivoting).
+ * L D L^T in the given order perm) or 1 (LU of G*S with partial pivoting).
  * Result y satisfies G y = b (solver 0) or the c = S^-1 y equivalent is
  * returned already multiplied by S (solver 1), i.e. both return y = G^-1 b. */
 int orc_solve(const double* G, const double* S, const double* b, int n,
