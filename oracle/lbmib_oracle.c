@@ -663,6 +663,51 @@ static void physical_variables(orc_case* c)
         }
 }
 
+/* One node of colission (solution.f90:573-643): m = M f, relaxation towards
+ * the incompressible moment equilibrium with jx = rho*u, Guo forcing in
+ * moment space when a force acts, f = M^-1 m*. */
+static void collide_node(const double* f, double rt, double ut, double vt, const double* s,
+                         double gfx, double gfy, const double* minv, double* out)
+{
+    double temp[Q], temp1[Q], fp[Q];
+    for (int k = 0; k < Q; ++k) temp[k] = f[k];
+    matmulvoc(temp1, &MATM[0][0], temp);
+    temp[0] = 0.0;
+    temp[1] = -2.0 * rt + 3.0 * (rt * ut * rt * ut + rt * vt * rt * vt);
+    temp[2] = rt - 3.0 * (rt * ut * rt * ut + rt * vt * rt * vt);
+    temp[3] = 0.0;
+    temp[4] = -rt * ut;
+    temp[5] = 0.0;
+    temp[6] = -rt * vt;
+    temp[7] = (rt * ut * rt * ut - rt * vt * rt * vt);
+    temp[8] = (rt * ut * rt * vt);
+    int forced = (gfx != 0.0) || (gfy != 0.0);
+    if (forced) {
+        fp[0] = 0.0;
+        fp[1] = 6.0 * (1.0 - 0.5 * s[1]) * (ut * gfx + vt * gfy);
+        fp[2] = -6.0 * (1.0 - 0.5 * s[2]) * (ut * gfx + vt * gfy);
+        fp[3] = gfx;
+        fp[4] = -(1.0 - 0.5 * s[4]) * gfx;
+        fp[5] = gfy;
+        fp[6] = -(1.0 - 0.5 * s[6]) * gfy;
+        fp[7] = 2.0 * (1.0 - 0.5 * s[7]) * (ut * gfx - vt * gfy);
+        fp[8] = (1.0 - 0.5 * s[8]) * (ut * gfy + vt * gfx);
+        for (int k = 0; k < Q; ++k) temp[k] = temp1[k] + (temp[k] - temp1[k]) * s[k] + fp[k];
+    } else {
+        /* fp == +-0: adding it is an exact no-op */
+        for (int k = 0; k < Q; ++k) temp[k] = temp1[k] + (temp[k] - temp1[k]) * s[k];
+    }
+    matmulvoc(out, minv, temp);
+}
+
+void orc_collide_node(const double f[9], double rho, double u, double v, const double s[9],
+                      double gfx, double gfy, double out[9])
+{
+    double minv[Q * Q];
+    nizhen(&MATM[0][0], minv, Q);
+    collide_node(f, rho, u, v, s, gfx, gfy, minv, out);
+}
+
 /* colission (solution.f90:573-643) on the interior; gforce is zero away from
  * the IB nodes so the force moments vanish there exactly. */
 static void colission(orc_case* c)
@@ -675,38 +720,10 @@ static void colission(orc_case* c)
     for (int j = 1; j < ny - 1; ++j)
         for (int i = 1; i < nx - 1; ++i) {
             size_t cc = (size_t)j * nx + i;
-            double temp[Q], temp1[Q], fp[Q];
-            for (int k = 0; k < Q; ++k) temp[k] = c->f[k * N + cc];
-            matmulvoc(temp1, &MATM[0][0], temp);
-            double ut = c->u[cc], vt = c->v[cc], rt = c->rho[cc];
-            temp[0] = 0.0;
-            temp[1] = -2.0 * rt + 3.0 * (rt * ut * rt * ut + rt * vt * rt * vt);
-            temp[2] = rt - 3.0 * (rt * ut * rt * ut + rt * vt * rt * vt);
-            temp[3] = 0.0;
-            temp[4] = -rt * ut;
-            temp[5] = 0.0;
-            temp[6] = -rt * vt;
-            temp[7] = (rt * ut * rt * ut - rt * vt * rt * vt);
-            temp[8] = (rt * ut * rt * vt);
-            double gfx = c->gforce[cc], gfy = c->gforce[N + cc];
-            int forced = (gfx != 0.0) || (gfy != 0.0);
-            if (forced) {
-                fp[0] = 0.0;
-                fp[1] = 6.0 * (1.0 - 0.5 * s[1]) * (ut * gfx + vt * gfy);
-                fp[2] = -6.0 * (1.0 - 0.5 * s[2]) * (ut * gfx + vt * gfy);
-                fp[3] = gfx;
-                fp[4] = -(1.0 - 0.5 * s[4]) * gfx;
-                fp[5] = gfy;
-                fp[6] = -(1.0 - 0.5 * s[6]) * gfy;
-                fp[7] = 2.0 * (1.0 - 0.5 * s[7]) * (ut * gfx - vt * gfy);
-                fp[8] = (1.0 - 0.5 * s[8]) * (ut * gfy + vt * gfx);
-                for (int k = 0; k < Q; ++k) temp[k] = temp1[k] + (temp[k] - temp1[k]) * s[k] + fp[k];
-            } else {
-                /* fp == +-0: adding it is an exact no-op */
-                for (int k = 0; k < Q; ++k) temp[k] = temp1[k] + (temp[k] - temp1[k]) * s[k];
-            }
-            matmulvoc(temp1, minv, temp);
-            for (int k = 0; k < Q; ++k) c->f[k * N + cc] = temp1[k];
+            double fin[Q], fout[Q];
+            for (int k = 0; k < Q; ++k) fin[k] = c->f[k * N + cc];
+            collide_node(fin, c->rho[cc], c->u[cc], c->v[cc], s, c->gforce[cc], c->gforce[N + cc], minv, fout);
+            for (int k = 0; k < Q; ++k) c->f[k * N + cc] = fout[k];
         }
 }
 

@@ -91,6 +91,9 @@ double orc_asin(double x);                        /* asin used by the delta kern
 double orc_delta(double r, int smooth);           /* one-axis kernel (boundary_condition.f90:1563-1600) */
 double orc_cs(void);                              /* sqrt(3.0)/3.0d0 with REAL(4) sqrt (boundary_condition.f90:304) */
 double orc_feq(double rh, double u1, double v1, int k);  /* D2Q9feq (D2Q9model.f90:35-46), k 0-based */
+/* one node of colission (solution.f90:573-643): f, rho, u, v, rates, Guo force -> f */
+void orc_collide_node(const double f[9], double rho, double u, double v, const double s[9],
+                      double gfx, double gfy, double out[9]);
 /* Motion step table (mode.f90:245-269) on a dt grid: out has nsteps x ncols. */
 int orc_motion_step_table(const double* motion, int rows, int ncols, double dt,
                           double* out, int max_steps);

@@ -67,6 +67,7 @@ def lib():
         L.orc_feq.restype = C.c_double
         L.orc_motion_step_table.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_solve.argtypes = [_dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp]
+        L.orc_collide_node.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -105,6 +106,14 @@ def cs() -> float:
 
 def feq(rh, u, v, k):
     return lib().orc_feq(rh, u, v, k)
+
+
+def collide_node(f, rho: float, u: float, v: float, s, gfx: float = 0.0, gfy: float = 0.0) -> np.ndarray:
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = np.zeros(9)
+    lib().orc_collide_node(_d(f), rho, u, v, _d(s), gfx, gfy, _d(out))
+    return out
 
 
 def motion_step_table(motion: np.ndarray, dt: float) -> np.ndarray:
