@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE_F64 = 144
 PROFILE_EVERY = 16
+# HBM bytes per k_lattice_f64 launch on the 4096x2048 workload, from
+# rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
+# device copy measured the same way (the lattice traffic summary in profiles/)
+LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
@@ -56,7 +60,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (the known-size kernel that scales PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs one case on one GPU; --gpus %d is not supported "
               "(the path shards by case across GPUs, see the design notes)" % args.gpus, file=sys.stderr)
@@ -98,8 +113,9 @@ def main():
                                % (case.name, p.nx, p.ny, 1.0 / p.scale, p.pts_per_wing - 1),
                    "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_collide_f64", "bytes_per_launch": interior * BYTES_PER_NODE_F64,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": LATTICE_TRAFFIC_BYTES.get((p.nx, p.ny)),
+                     "kernel": "k_lattice_f64", "bytes_per_launch": interior * BYTES_PER_NODE_F64,
                      "mean_launch_ms": round(lat_s * 1e3, 5)},
         "whole_step_hbm_frac": round(value * 1e6 * BYTES_PER_NODE_F64 / (HBM_PEAK_GBS * 1e9), 4),
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},
