@@ -12,6 +12,28 @@
  * plain IEEE double, evaluated left to right exactly as the Fortran source is
  * written, compiled with -ffp-contract=off (no FMA contraction) so that the
  * GPU path, built under the same rule, can be compared bit for bit.
+ *
+ * How each part is pinned (tests/test_oracle.py, tests/test_cases.py):
+ *   weights, cs, M^-1, asin, both delta kernels, the collision (mass, force,
+ *   He-Luo equilibrium fixed point, Guo forcing moments), feq, the motion
+ *   table, the IB solvers (numpy, LU vs L D L^T, elimination order), the IB
+ *   node set (brute force), marker positions and velocities (closed-form
+ *   rotation, finite differences), the IB correction (D (u* + du) = U_B),
+ *   dense vs sparse assembly (bit-identical), symmetric-airfoil zero lift,
+ *   mirrored-plunge lift, the force-row layout and erranalysis.
+ * PARITY UNPINNED:
+ *   - boundary_condition1 (boundary_condition.f90:288-858): the characteristic
+ *     + non-equilibrium-extrapolation outer boundary, its overwrites and the
+ *     stale-ring / corner semantics are a line-by-line restatement; nothing
+ *     but the reference itself fixes those ad-hoc formulas, and the reference
+ *     does not build here (it needs mpi, cublas and cusolverDn modules).  The
+ *     symmetric and mirrored whole-step tests exercise it only through the
+ *     symmetries it must keep.
+ *   - the round-off order of the reference's cuBLAS DGEMM/DGEMV and cuSOLVER
+ *     DGETRF/DGETRS calls (NVHPC 23.11 / CUDA 12.3, closed source): solver=1
+ *     restates them as textbook LAPACK; the GPU path and solver=0 use a banded
+ *     L D L^T of the same SPD system, pinned against solver=1 and numpy to
+ *     1e-12 (fields) / 1e-10 (force history), not bit for bit.
  */
 #ifndef LBMIB_ORACLE_H
 #define LBMIB_ORACLE_H
