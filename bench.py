@@ -26,6 +26,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE_F64 = 144
+BYTES_PER_NODE_F32 = 72
+WORKLOADS = {
+    0: "400x200 lattice, NACA0012 chord 40 lu, 128 markers, Re 100, plunge 0.5c, St 0.3, U 0.05",
+    1: "4096x2048 lattice, NACA0012 chord 256 lu, 1024 markers, Re 1000, plunge 0.5c + pitch 30deg "
+       "(90deg lead), St 0.3, U 0.05",
+    2: "8192x4096 lattice, NACA0012 chord 512 lu, 2048 markers, Re 2000, plunge 0.5c, St 0.3, U 0.05",
+}
 PROFILE_EVERY = 16
 # HBM bytes per k_lattice_f64 launch on the 4096x2048 workload, from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
@@ -60,6 +67,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (the known-size kernel that scales PMC byte counters)")
     args = ap.parse_args()
@@ -78,9 +87,10 @@ def main():
         sys.exit(2)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
+    fp32 = args.precision == "fp32"
     case = cases.config(args.config, n_steps=K + W + 8)
     p = case.params
-    g = lbmib.LBMIB(case, device=0)
+    g = lbmib.LBMIB(case, device=0, precision=lbmib.PREC_FP32DEV if fp32 else lbmib.PREC_FP64)
     if W:
         g.evolution(W)
     # phase events on every PROFILE_EVERY-th timed step: they time the lattice
@@ -92,11 +102,12 @@ def main():
     tm = g.timings()
     cells = p.nx * p.ny
     interior = (p.nx - 2) * (p.ny - 2)
+    bpn = BYTES_PER_NODE_F32 if fp32 else BYTES_PER_NODE_F64
     lat_s = tm["lattice_ms"] * 1e-3
-    achieved = interior * BYTES_PER_NODE_F64 / lat_s / 1e9
+    achieved = interior * bpn / lat_s / 1e9
     value = cells * K / dt / 1e6
     out = {
-        "metric": "MLUPS (whole FP64 timestep incl. immersed boundary)",
+        "metric": "MLUPS (whole %s timestep incl. immersed boundary)" % ("FP32-deviation" if fp32 else "FP64"),
         "value": round(value, 2),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -106,18 +117,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 lattice, f64 immersed boundary" if fp32 else "f64",
         "data": "synthetic (generated NACA0012 markers and sinusoidal kinematics)",
-        "config": {"workload": "%s: %dx%d lattice, NACA0012 chord %g lu, %d markers, Re 1000, "
-                               "plunge 0.5c + pitch 30deg (90deg lead), St 0.3, U 0.05, FP64 drop-in"
-                               % (case.name, p.nx, p.ny, 1.0 / p.scale, p.pts_per_wing - 1),
+        "config": {"workload": "%s: %s, %s" % (case.name, WORKLOADS.get(args.config, ""),
+                                                "FP32 deviation form" if fp32 else "FP64 drop-in"),
                    "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": LATTICE_TRAFFIC_BYTES.get((p.nx, p.ny)),
-                     "kernel": "k_lattice_f64", "bytes_per_launch": interior * BYTES_PER_NODE_F64,
-                     "mean_launch_ms": round(lat_s * 1e3, 5)},
-        "whole_step_hbm_frac": round(value * 1e6 * BYTES_PER_NODE_F64 / (HBM_PEAK_GBS * 1e9), 4),
+                     "traffic": None if fp32 else LATTICE_TRAFFIC_BYTES.get((p.nx, p.ny)),
+                     "kernel": "k_lattice<%s>" % ("float" if fp32 else "double"),
+                     "bytes_per_launch": interior * bpn, "mean_launch_ms": round(lat_s * 1e3, 5)},
+        "whole_step_hbm_frac": round(value * 1e6 * bpn / (HBM_PEAK_GBS * 1e9), 4),
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},
     }
     g.close()

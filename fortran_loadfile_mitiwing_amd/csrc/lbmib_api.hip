@@ -21,7 +21,7 @@
 
 namespace lbmib {
 // kernels (lbmib_kernels.hip)
-__global__ void k_lattice_f64(DevCase, int, int, double, double*, int);
+template <typename T> __global__ void k_lattice(DevCase, int, int, double, double*, int);
 int lattice_ring_blocks(int nx, int ny);
 int lattice_blocks(int nx, int ny);
 __global__ void k_geo_clear(DevCase, int);
@@ -33,16 +33,16 @@ __global__ void k_geo_dsort(DevCase, int);
 __global__ void k_geo_nsort(DevCase, int);
 __global__ void k_geo_assemble(DevCase, int);
 __global__ void k_geo_factor(DevCase, int);
-__global__ void k_crit_rhs(DevCase, int, int);
+template <typename T> __global__ void k_crit_rhs(DevCase, int, int);
 template <int U> __global__ void k_crit_solve(DevCase, int);
 int solve_unit(int npad);
 int solve_threads();
-__global__ void k_band_f64(DevCase, int, int, double*);
+template <typename T> __global__ void k_band(DevCase, int, int, double*);
 __global__ void k_forces(DevCase, int, int);
 size_t factor_smem_bytes();
 size_t solve_smem_bytes(int npad);
 __global__ void k_residual(const double*, const double*, int, int, long long, double*);
-__global__ void k_nan_check(const double*, long long, int*);
+template <typename T> __global__ void k_nan_check(const T*, long long, int*);
 }  // namespace lbmib
 
 using namespace lbmib;
@@ -293,7 +293,8 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     if (p->n_wings < 1 || p->n_wings > 8) return fail(LBMIB_E_ARG, "wing_count %d out of range 1..8", p->n_wings);
     if (p->pts_per_wing < 4) return fail(LBMIB_E_ARG, "need at least 4 boundary points per wing");
     if (!(p->scale > 0.0)) return fail(LBMIB_E_ARG, "scale must be positive");
-    if (p->precision != LBMIB_PREC_FP64) return fail(LBMIB_E_ARG, "precision %d not available in this build", p->precision);
+    if (p->precision != LBMIB_PREC_FP64 && p->precision != LBMIB_PREC_FP32DEV)
+        return fail(LBMIB_E_ARG, "precision %d unknown (LBMIB_PREC_FP64 or LBMIB_PREC_FP32DEV)", p->precision);
     if (p->smooth != 0 && p->smooth != 1) return fail(LBMIB_E_ARG, "smooth must be 0 or 1");
     if ((long long)p->n_wings * (p->pts_per_wing - 1) > MAXSEG)
         return fail(LBMIB_E_ARG, "at most %d marker segments supported", MAXSEG);
@@ -371,6 +372,22 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     D.nx = nx; D.ny = ny; D.N = N;
     D.ox = p->ox; D.oy = p->oy; D.scale = p->scale; D.U = p->U; D.xc = p->xc; D.yc = p->yc;
     for (int k = 0; k < 9; ++k) D.R.s[k] = p->s[k];
+    // weights and the FP32 deviation-form constants (rest moments of W)
+    D.prec = p->precision;
+    D.rho0 = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        D.W[k] = feq_ref(1.0, 0.0, 0.0, k);
+        D.rho0 += D.W[k];
+        D.K.s[k] = (float)p->s[k];
+    }
+    {
+        const double* W = D.W;
+        const double e_w = -4 * W[0] - W[1] - W[2] - W[3] - W[4] + 2 * (W[5] + W[6] + W[7] + W[8]);
+        const double eps_w = 4 * W[0] - 2 * (W[1] + W[2] + W[3] + W[4]) + W[5] + W[6] + W[7] + W[8];
+        D.K.c0 = (float)(-D.rho0);
+        D.K.c1 = (float)(-2.0 * D.rho0 - e_w);
+        D.K.c2 = (float)(D.rho0 - eps_w);
+    }
     D.smooth = p->smooth;
     D.nw = nw; D.ppw = ppw; D.spw = ppw - 1; D.nseg = nw * (ppw - 1);
     D.cap = D.nseg * BOXMAX;
@@ -395,8 +412,13 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     for (int q = 0; q < D.nseg; ++q) h->iperm[h->perm[q]] = q;
 
     // lattice
-    TRY(dalloc(h, &D.f[0], 9 * (size_t)N));
-    TRY(dalloc(h, &D.f[1], 9 * (size_t)N));
+    if (D.prec == LBMIB_PREC_FP64) {
+        TRY(dalloc(h, &D.f[0], 9 * (size_t)N));
+        TRY(dalloc(h, &D.f[1], 9 * (size_t)N));
+    } else {
+        TRY(dalloc(h, &D.g[0], 9 * (size_t)N));
+        TRY(dalloc(h, &D.g[1], 9 * (size_t)N));
+    }
     TRY(dalloc(h, &D.f0L, 9 * (size_t)ny));
     TRY(dalloc(h, &D.f0R, 9 * (size_t)ny));
     TRY(dalloc(h, &D.macro[0], 3 * (size_t)N));
@@ -406,10 +428,17 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
         double feq0[9];
         for (int k = 0; k < 9; ++k) feq0[k] = feq_ref(1.0, p->U, 0.0, k);
         std::vector<double> plane((size_t)N);
+        std::vector<float> plane32(D.prec == LBMIB_PREC_FP64 ? 0 : (size_t)N);
         for (int k = 0; k < 9; ++k) {
-            std::fill(plane.begin(), plane.end(), feq0[k]);
-            HTRY(hipMemcpy(D.f[0] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-            HTRY(hipMemcpy(D.f[1] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+            if (D.prec == LBMIB_PREC_FP64) {
+                std::fill(plane.begin(), plane.end(), feq0[k]);
+                HTRY(hipMemcpy(D.f[0] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+                HTRY(hipMemcpy(D.f[1] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+            } else {
+                std::fill(plane32.begin(), plane32.end(), (float)(feq0[k] - D.W[k]));
+                HTRY(hipMemcpy(D.g[0] + (size_t)k * N, plane32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+                HTRY(hipMemcpy(D.g[1] + (size_t)k * N, plane32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+            }
         }
         std::vector<double> col(9 * (size_t)ny);
         for (int k = 0; k < 9; ++k)
@@ -438,9 +467,9 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     D.perm = dperm; D.iperm = diperm;
     TRY(dalloc(h, &D.rhs, 2 * (size_t)ns));
     TRY(dalloc(h, &D.y, 2 * (size_t)ns));
-    TRY(dalloc(h, &D.err, 1));
+    TRY(dalloc(h, &D.err, 2));  // [0] ErrBits, [1] NaN-guard flag
     TRY(dalloc(h, &D.red, 2 * 4096));
-    TRY(fill_i32(D.err, 0, 1, h->s[0]));
+    TRY(fill_i32(D.err, 0, 2, h->s[0]));
     D.bw = geometric_bandwidth(h);
     if (D.bw >= ns) D.bw = ns - 1;
     for (int i = 0; i < lbmib_handle::RING; ++i)
@@ -708,7 +737,8 @@ static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
     HIP_TRY(hipStreamWaitEvent(s3, h->ev_ib[(n - 1) % R], 0));
     HIP_TRY(hipStreamWaitEvent(s3, built, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[2], s3));
-    hipLaunchKernelGGL(k_crit_rhs, dim3((ns + 6) / 7), dim3(256), 0, s3, D, g, cur);
+    const bool f64 = D.prec == LBMIB_PREC_FP64;
+    hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3((ns + 6) / 7), dim3(256), 0, s3, D, g, cur);
     HIP_TRY(hipEventRecord(h->ev_rhs[n % R], s3));
     HIP_TRY(hipStreamWaitEvent(s1, h->ev_rhs[n % R], 0));
     if (solve_unit(D.npad) == 64)
@@ -716,13 +746,13 @@ static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
     else
         hipLaunchKernelGGL(k_crit_solve<32>, dim3(1), dim3(solve_threads()), solve_smem_bytes(D.npad), s1, D, g);
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
-    hipLaunchKernelGGL(k_band_f64, dim3((cap + 255) / 256), dim3(256), 0, s1, D, g, cur, macro);
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3((cap + 255) / 256), dim3(256), 0, s1, D, g, cur, macro);
     if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
     HIP_TRY(hipEventRecord(h->ev_ib[n % R], s1));
     HIP_TRY(hipStreamWaitEvent(s0, h->ev_ib[(n - 1) % R], 0));
     if ((n - 1) % B == 0) HIP_TRY(hipStreamWaitEvent(s0, built, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    hipLaunchKernelGGL(k_lattice_f64, dim3(lattice_blocks(nx, ny)), dim3(256), 0, s0, D, g, cur, h->cs, macro,
+    hipLaunchKernelGGL(f64 ? k_lattice<double> : k_lattice<float>, dim3(lattice_blocks(nx, ny)), dim3(256), 0, s0, D, g, cur, h->cs, macro,
                        lattice_ring_blocks(nx, ny));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     HIP_TRY(hipEventRecord(h->ev_step[n % R], s0));
@@ -797,15 +827,16 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
         h->tms[8] += h->gev_used;
     }
     if (h->p.nan_guard) {
-        int* flag;
-        if (dalloc(h, &flag, 1)) return LBMIB_E_HIP;
-        HIP_TRY(hipMemset(flag, 0, sizeof(int)));
-        hipLaunchKernelGGL(k_nan_check, dim3(1024), dim3(256), 0, h->s[0],
-                           (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
+        int* flag = h->D.err + 1;
+        if (fill_i32(flag, 0, 1, h->s[0])) return LBMIB_E_HIP;
+        if (h->D.prec == LBMIB_PREC_FP64)
+            hipLaunchKernelGGL(k_nan_check<double>, dim3(1024), dim3(256), 0, h->s[0],
+                               (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
+        else
+            hipLaunchKernelGGL(k_nan_check<float>, dim3(1024), dim3(256), 0, h->s[0],
+                               (const float*)h->D.g[h->steps_done & 1], 9 * h->D.N, flag);
         int f = 0;
         HIP_TRY(hipMemcpy(&f, flag, sizeof(int), hipMemcpyDeviceToHost));
-        h->allocs.pop_back();
-        (void)hipFree(flag);
         if (f) return fail(LBMIB_E_NAN, "NaN in the populations after step %d", h->steps_done);
     }
     return 0;
@@ -887,7 +918,15 @@ int lbmib_download(lbmib_handle* h, double* rho, double* u, double* v, double* f
     if (rho) HIP_TRY(hipMemcpy(rho, m, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (u) HIP_TRY(hipMemcpy(u, m + N, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (v) HIP_TRY(hipMemcpy(v, m + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost));
-    if (f) HIP_TRY(hipMemcpy(f, h->D.f[slot], sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    if (f && h->D.prec == LBMIB_PREC_FP64)
+        HIP_TRY(hipMemcpy(f, h->D.f[slot], sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+    if (f && h->D.prec != LBMIB_PREC_FP64) {
+        std::vector<float> g((size_t)N);
+        for (int k = 0; k < 9; ++k) {
+            HIP_TRY(hipMemcpy(g.data(), h->D.g[slot] + (size_t)k * N, sizeof(float) * N, hipMemcpyDeviceToHost));
+            for (long long c = 0; c < N; ++c) f[(size_t)k * N + c] = h->D.W[k] + (double)g[c];
+        }
+    }
     return 0;
 }
 
@@ -896,7 +935,17 @@ int lbmib_upload_state(lbmib_handle* h, const double* f)
     if (!h || !f) return fail(LBMIB_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(h->dev));
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(h->D.f[h->steps_done & 1], f, sizeof(double) * 9 * h->D.N, hipMemcpyHostToDevice));
+    const long long N = h->D.N;
+    const int slot = h->steps_done & 1;
+    if (h->D.prec == LBMIB_PREC_FP64) {
+        HIP_TRY(hipMemcpy(h->D.f[slot], f, sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> g((size_t)N);
+        for (int k = 0; k < 9; ++k) {
+            for (long long c = 0; c < N; ++c) g[c] = (float)(f[(size_t)k * N + c] - h->D.W[k]);
+            HIP_TRY(hipMemcpy(h->D.g[slot] + (size_t)k * N, g.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+        }
+    }
     return 0;
 }
 

@@ -186,6 +186,63 @@ LBM_HD void nee_node(double rhe, double ue, double ve, double rhn, double un, do
     mrt_inverse(t3, out);
 }
 
+// ---- FP32 deviation form ----
+// Populations stored as g = f - W with W the reference's REAL(4) weights
+// (the rest state, rho0 = sum W).  The collision is linear in f, so in moment
+// space it relaxes m(g) = M g towards m_eq(rho, j) - M W:
+//   k=1: -2 drho + 3 j^2 + c1,  k=2: drho - 3 j^2 + c2  (c1 = -2 rho0 - (M W)_1,
+//   c2 = rho0 - (M W)_2, round-off sized), k=4: -jx, k=6: -jy, k=7: jx^2 - jy^2,
+//   k=8: jx jy, k=0: -rho0 and k=3,5: 0 (the reference writes zero equilibria
+//   there and relaxes them with s = 0), with drho = sum g and j = sum e g.
+struct DevConst {
+    float s[9];
+    float c0, c1, c2;
+};
+
+// f = M^-1 m in FP32 with the exact rational inverse M^T diag(1/9, 1/36, 1/36,
+// 1/6, 1/12, 1/6, 1/12, 1/4, 1/4).
+LBM_HD void mrt_inverse_f32(const float* m, float* f)
+{
+    const float a0 = m[0] * (1.0f / 9.0f);
+    const float r1 = m[1] * (1.0f / 36.0f), r2 = m[2] * (1.0f / 18.0f);
+    const float p3 = m[3] * (1.0f / 6.0f), p4 = m[4] * (1.0f / 6.0f);
+    const float p5 = m[5] * (1.0f / 6.0f), p6 = m[6] * (1.0f / 6.0f), p7 = m[7] * 0.25f;
+    const float s1 = m[1] * (1.0f / 18.0f), s2 = m[2] * (1.0f / 36.0f);
+    const float q4 = m[4] * (1.0f / 12.0f), q6 = m[6] * (1.0f / 12.0f), p8 = m[8] * 0.25f;
+    f[0] = a0 - m[1] * (1.0f / 9.0f) + m[2] * (1.0f / 9.0f);
+    f[1] = a0 - r1 - r2 + p3 - p4 + p7;
+    f[2] = a0 - r1 - r2 + p5 - p6 - p7;
+    f[3] = a0 - r1 - r2 - p3 + p4 + p7;
+    f[4] = a0 - r1 - r2 - p5 + p6 - p7;
+    f[5] = a0 + s1 + s2 + p3 + q4 + p5 + q6 + p8;
+    f[6] = a0 + s1 + s2 - p3 - q4 + p5 + q6 - p8;
+    f[7] = a0 + s1 + s2 - p3 - q4 - p5 - q6 + p8;
+    f[8] = a0 + s1 + s2 + p3 + q4 - p5 - q6 - p8;
+}
+
+// Unforced collision of one node in deviation form; returns drho, jx, jy.
+LBM_HD void collide_dev_f32(float* g, const DevConst& K, float& drho, float& jx, float& jy)
+{
+    float m[9], t[9];
+    mrt_forward(g, m);
+    drho = m[0];
+    jx = m[3];
+    jy = m[5];
+    const float jxx = jx * jx, jyy = jy * jy, j2 = jxx + jyy;
+    t[0] = K.c0;
+    t[1] = -2.0f * drho + 3.0f * j2 + K.c1;
+    t[2] = drho - 3.0f * j2 + K.c2;
+    t[3] = 0.0f;
+    t[4] = -jx;
+    t[5] = 0.0f;
+    t[6] = -jy;
+    t[7] = jxx - jyy;
+    t[8] = jx * jy;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) t[k] = m[k] + (t[k] - m[k]) * K.s[k];
+    mrt_inverse_f32(t, g);
+}
+
 // ---- delta kernel ----
 
 // asin by fdlibm's e_asin method (Sun Microsystems, 1993): x + x^3 R(x^2)

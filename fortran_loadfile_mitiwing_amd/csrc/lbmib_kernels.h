@@ -66,8 +66,14 @@ struct DevCase {
     int steps_cap;          // rows of kin / hist
     int nslot;
     int gi0, gj0, gw, gh;   // lattice window (0-based origin, size) holding every IB node
-    // lattice (population-planar, x fastest)
+    // lattice (population-planar, x fastest): FP64 populations f, or in the
+    // FP32 deviation form g = f - W
+    int prec;               // LBMIB_PREC_FP64 / LBMIB_PREC_FP32DEV
     double* f[2];
+    float* g[2];
+    double W[9];            // the reference's REAL(4) weights, as doubles
+    double rho0;            // sum of W
+    DevConst K;             // FP32 collision constants
     double* f0L;            // stale boundary-ring f0 of column 1 and column nx: [9][ny]
     double* f0R;
     double* macro[2];       // rho, u, v planes (3*N) of the last two flagged steps

@@ -1,9 +1,11 @@
-// lbmib_kernels.hip -- gfx950 kernels of the FP64 drop-in timestep.
+// lbmib_kernels.hip -- gfx950 kernels of the timestep: the FP64 drop-in and
+// the FP32 deviation form (populations stored as f - W in FP32).
 //
 // One timestep (reference evolution, solution.f90:269-410) is
-//   lattice:  k_lattice_f64   fused pull-stream + moments + MRT collide of every
-//                             interior node, prev -> cur (HBM-bound, 144 B/node),
-//                             and the characteristic/NEE outer boundary ring
+//   lattice:  k_lattice<T>   fused pull-stream + moments + MRT collide of every
+//                            interior node, prev -> cur (HBM-bound: 144 B/node in
+//                            FP64, 72 B/node in FP32), and the characteristic/NEE
+//                            outer boundary ring (FP64 arithmetic in both forms)
 //   IB:       geometry of the next step (node set, kernel weights, L D L^T
 //             factor of D D^T) on its own stream; per step u* -> D u* ->
 //             triangular solves -> spread + forced collide of the IB nodes;
@@ -19,20 +21,32 @@ namespace lbmib {
 // lattice
 // ---------------------------------------------------------------------------
 
+// Stored population -> FP64 population: FP64 storage holds f itself, the
+// FP32 deviation form holds g = f - W.
+__device__ __forceinline__ double lift(double v, double) { return v; }
+__device__ __forceinline__ double lift(float v, double w) { return w + (double)v; }
+__device__ __forceinline__ void lower(double f, double, double* dst) { *dst = f; }
+__device__ __forceinline__ void lower(double f, double w, float* dst) { *dst = (float)(f - w); }
+
+template <typename T> __device__ __forceinline__ T* pops(const DevCase& D, int i);
+template <> __device__ __forceinline__ double* pops<double>(const DevCase& D, int i) { return D.f[i]; }
+template <> __device__ __forceinline__ float* pops<float>(const DevCase& D, int i) { return D.g[i]; }
+
 // Pull the nine populations of interior node (i,j) from prev
-// (streaming, solution.f90:438-454: f0(i,j,k) = f(i-ex,j-ey,k)).
-__device__ __forceinline__ void pull9(const double* __restrict__ src, long long N, int nx,
-                                      long long c, double* f)
+// (streaming, solution.f90:438-454: f0(i,j,k) = f(i-ex,j-ey,k)), in FP64.
+template <typename T>
+__device__ __forceinline__ void pull9(const T* __restrict__ src, long long N, int nx, long long c,
+                                      double* f, const double* W)
 {
-    f[0] = src[c];
-    f[1] = src[1 * N + c - 1];
-    f[2] = src[2 * N + c - nx];
-    f[3] = src[3 * N + c + 1];
-    f[4] = src[4 * N + c + nx];
-    f[5] = src[5 * N + c - nx - 1];
-    f[6] = src[6 * N + c - nx + 1];
-    f[7] = src[7 * N + c + nx + 1];
-    f[8] = src[8 * N + c + nx - 1];
+    f[0] = lift(src[c], W[0]);
+    f[1] = lift(src[1 * N + c - 1], W[1]);
+    f[2] = lift(src[2 * N + c - nx], W[2]);
+    f[3] = lift(src[3 * N + c + 1], W[3]);
+    f[4] = lift(src[4 * N + c + nx], W[4]);
+    f[5] = lift(src[5 * N + c - nx - 1], W[5]);
+    f[6] = lift(src[6 * N + c - nx + 1], W[6]);
+    f[7] = lift(src[7 * N + c + nx + 1], W[7]);
+    f[8] = lift(src[8 * N + c + nx - 1], W[8]);
 }
 
 // Moments of a boundary-ring node from its stale f0 (never streamed,
@@ -82,7 +96,8 @@ __device__ __forceinline__ void right_char(double cs, double tr, double tu, doub
 
 // Edge state of column `col` (0: left, 1: right) at interior row j: the
 // characteristic-updated rho/u/v and the neighbour's f0 and moments.
-__device__ void edge_column_state(const DevCase& D, const double* src, double cs, int col, int j,
+template <typename T>
+__device__ void edge_column_state(const DevCase& D, const T* src, double cs, int col, int j,
                                   double& ro, double& uo, double& vo, double* f0n, double& rn,
                                   double& un, double& vn)
 {
@@ -94,9 +109,9 @@ __device__ void edge_column_state(const DevCase& D, const double* src, double cs
     int i2 = col == 0 ? 1 : nx - 2;
     int i3 = col == 0 ? 2 : nx - 3;
     double f3[9], r3, u3, v3;
-    pull9(src, N, nx, (long long)j * nx + i2, f0n);
+    pull9(src, N, nx, (long long)j * nx + i2, f0n, D.W);
     moments(f0n, rn, un, vn);
-    pull9(src, N, nx, (long long)j * nx + i3, f3);
+    pull9(src, N, nx, (long long)j * nx + i3, f3, D.W);
     moments(f3, r3, u3, v3);
     if (col == 0)
         left_char(cs, tr, tu, tv, rn, un, vn, r3, u3, v3, ro, uo, vo);
@@ -113,12 +128,13 @@ __device__ void edge_column_state(const DevCase& D, const double* src, double cs
 // columns of the walls read the left/right results at rows 1 and ny-2,
 // recomputed here.  Reads only the previous populations and the stale ring
 // f0, so it runs beside the interior update.
+template <typename T>
 __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __restrict__ macro, int t)
 {
     const int nx = D.nx, ny = D.ny;
     const long long N = D.N;
-    const double* src = D.f[cur ^ 1];
-    double* dst = D.f[cur];
+    const T* src = pops<T>(D, cur ^ 1);
+    T* dst = pops<T>(D, cur);
     if (t < ny - 2) {
         int j = t + 1;
         for (int col = 0; col < 2; ++col) {
@@ -128,7 +144,7 @@ __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __rest
             int i = col == 0 ? 0 : nx - 1;
             long long c = (long long)j * nx + i;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) dst[k * N + c] = out[k];
+            for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], dst + k * N + c);
             if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
         }
         return;
@@ -149,14 +165,14 @@ __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __rest
 #pragma unroll
             for (int k = 0; k < 9; ++k) f0n[k] = f0c[k * ny + jn];
         } else {
-            pull9(src, N, nx, (long long)jn * nx + x, f0n);
+            pull9(src, N, nx, (long long)jn * nx + x, f0n, D.W);
             moments(f0n, rn, un, vn);
         }
         double re = rn, ue = 0.0, ve = 0.0, out[9];
         nee_node(re, ue, ve, rn, un, vn, f0n, D.R, out);
         long long c = (long long)je * nx + x;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) dst[k * N + c] = out[k];
+        for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], dst + k * N + c);
         if (macro) { macro[c] = re; macro[N + c] = ue; macro[2 * N + c] = ve; }
     }
 }
@@ -166,15 +182,16 @@ __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __rest
 // boundary (boundary_condition1).  The first rblocks blocks take the ring,
 // the rest the interior rows in blocks that start at multiples of 256 nodes,
 // so every wave writes whole cache lines.  The IB nodes of the step (slot g)
-// are skipped: k_band_f64 collides them with their forcing, concurrently.
+// are skipped: k_band collides them with their forcing, concurrently.
 // Everywhere else gforce is zero and the force moments vanish exactly.
 // Optionally records rho/u/v (download / residual support).
-__global__ __launch_bounds__(256) void k_lattice_f64(DevCase D, int g, int cur, double cs,
-                                                     double* __restrict__ macro, int rblocks)
+template <typename T>
+__global__ __launch_bounds__(256) void k_lattice(DevCase D, int g, int cur, double cs,
+                                                 double* __restrict__ macro, int rblocks)
 {
     int b = blockIdx.x;
     if (b < rblocks) {
-        ring_thread(D, cur, cs, macro, b * blockDim.x + threadIdx.x);
+        ring_thread<T>(D, cur, cs, macro, b * blockDim.x + threadIdx.x);
         return;
     }
     b -= rblocks;
@@ -190,20 +207,47 @@ __global__ __launch_bounds__(256) void k_lattice_f64(DevCase D, int g, int cur, 
         if (nodeidx[(long long)wj * D.gw + wi] >= 0) return;
     }
     const long long c = (long long)j * nx + i;
-    double f[9];
-    pull9(D.f[cur ^ 1], N, nx, c, f);
-    double rho, u, v;
-    moments(f, rho, u, v);
-    if (macro) {
-        macro[c] = rho;
-        macro[N + c] = u;
-        macro[2 * N + c] = v;
-    }
-    collide_node(f, rho, u, v, D.R, false, 0.0, 0.0);
-    double* dst = D.f[cur];
+    const T* __restrict__ src = pops<T>(D, cur ^ 1);
+    T* __restrict__ dst = pops<T>(D, cur);
+    if constexpr (sizeof(T) == 8) {
+        double f[9];
+        pull9(src, N, nx, c, f, D.W);
+        double rho, u, v;
+        moments(f, rho, u, v);
+        if (macro) {
+            macro[c] = rho;
+            macro[N + c] = u;
+            macro[2 * N + c] = v;
+        }
+        collide_node(f, rho, u, v, D.R, false, 0.0, 0.0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
+        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
+    } else {
+        // FP32 deviation form: pull, collide and store g = f - W in FP32
+        float g[9];
+        g[0] = src[c];
+        g[1] = src[1 * N + c - 1];
+        g[2] = src[2 * N + c - nx];
+        g[3] = src[3 * N + c + 1];
+        g[4] = src[4 * N + c + nx];
+        g[5] = src[5 * N + c - nx - 1];
+        g[6] = src[6 * N + c - nx + 1];
+        g[7] = src[7 * N + c + nx + 1];
+        g[8] = src[8 * N + c + nx - 1];
+        float drho, jx, jy;
+        collide_dev_f32(g, D.K, drho, jx, jy);
+        if (macro) {
+            const double rho = D.rho0 + (double)drho;
+            macro[c] = rho;
+            macro[N + c] = (double)jx / rho;
+            macro[2 * N + c] = (double)jy / rho;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(g[k], dst + k * N + c);
+    }
 }
+template __global__ void k_lattice<double>(DevCase, int, int, double, double*, int);
+template __global__ void k_lattice<float>(DevCase, int, int, double, double*, int);
 
 int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
 int lattice_blocks(int nx, int ny) { return lattice_ring_blocks(nx, ny) + (nx - 1 + 255) / 256 * (ny - 2); }
@@ -235,7 +279,7 @@ __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, i
 //   k_geo_clear -> k_geo_markers -> k_geo_fill -> k_geo_phi -> k_geo_pairs ->
 //   k_geo_dsort -> k_geo_nsort -> k_geo_assemble -> k_geo_factor
 // Per-step solve, on the critical path beside the lattice kernel:
-//   k_crit_rhs -> k_crit_solve -> (after the lattice kernel) k_band_f64
+//   k_crit_rhs -> k_crit_solve -> k_band (beside the lattice kernel)
 // Per-wing forces of a batch of steps: k_forces.
 // ---------------------------------------------------------------------------
 
@@ -626,6 +670,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 // the node's post-stream populations (physical_variables there); the
 // segment's products are then summed in ascending node order.
 constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
+template <typename T>
 __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
@@ -640,7 +685,7 @@ __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
         double w = S.dval[s * BOXMAX + q];
         long long c = (long long)S.adj[3 * node + 1] * D.nx + S.adj[3 * node];
         double f[9], r, u, v;
-        pull9(D.f[cur ^ 1], D.N, D.nx, c, f);
+        pull9(pops<T>(D, cur ^ 1), D.N, D.nx, c, f, D.W);
         moments(f, r, u, v);
         pu[sl][q] = w * u;
         pv[sl][q] = w * v;
@@ -654,6 +699,8 @@ __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
         D.rhs[D.nseg + p] = S.vb[s] - sv;
     }
 }
+template __global__ void k_crit_rhs<double>(DevCase, int, int);
+template __global__ void k_crit_rhs<float>(DevCase, int, int);
 
 // Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
 // right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
@@ -827,9 +874,10 @@ size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
 // Spread delta_u = sum_k D(k,a) y(k) in ascending k (the fixed-order gather in
 // place of the spread DGEMV, boundary_condition.f90:1639-1640), velocity
 // correction and body force (1647-1657), and the forced collision
-// (solution.f90:598-638) of the IB nodes, which k_lattice_f64 leaves out;
+// (solution.f90:598-638) of the IB nodes, which k_lattice leaves out;
 // the two run concurrently.
-__global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, double* __restrict__ macro)
+template <typename T>
+__global__ __launch_bounds__(256) void k_band(DevCase D, int g, int cur, double* __restrict__ macro)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const Slot S = slot_of(D, g);
@@ -842,7 +890,7 @@ __global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, dou
     const int* ad = S.adj + 3 * a;
     long long c = (long long)ad[1] * D.nx + ad[0];
     double f[9], r, us, vs;
-    pull9(D.f[cur ^ 1], D.N, D.nx, c, f);
+    pull9(pops<T>(D, cur ^ 1), D.N, D.nx, c, f, D.W);
     double gx = 0.0, gy = 0.0;
     for (int q = 0; q < cnt; ++q) {
         int s = km[q];
@@ -860,11 +908,13 @@ __global__ __launch_bounds__(256) void k_band_f64(DevCase D, int g, int cur, dou
     double gfy = 2.0 * r * gy;
     bool forced = (gfx != 0.0) || (gfy != 0.0);
     collide_node(f, r, u, v, D.R, forced, gfx, gfy);
-    double* dst = D.f[cur];
+    T* dst = pops<T>(D, cur);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) dst[k * D.N + c] = f[k];
+    for (int k = 0; k < 9; ++k) lower(f[k], D.W[k], dst + k * D.N + c);
     if (macro) { macro[c] = r; macro[D.N + c] = u; macro[2 * D.N + c] = v; }
 }
+template __global__ void k_band<double>(DevCase, int, int, double*);
+template __global__ void k_band<float>(DevCase, int, int, double*);
 
 // Per-wing force and moment, accumulated in node order
 // (boundary_condition.f90:1662-1675), and their normalisation
@@ -969,12 +1019,14 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ mc,
 }
 
 // NaN guard over the populations.
-__global__ __launch_bounds__(256) void k_nan_check(const double* __restrict__ f, long long n,
-                                                   int* __restrict__ flag)
+template <typename T>
+__global__ __launch_bounds__(256) void k_nan_check(const T* __restrict__ f, long long n, int* __restrict__ flag)
 {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x)
         if (f[t] != f[t]) { *flag = 1; return; }
 }
+template __global__ void k_nan_check<double>(const double*, long long, int*);
+template __global__ void k_nan_check<float>(const float*, long long, int*);
 
 }  // namespace lbmib
