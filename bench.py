@@ -14,9 +14,19 @@ bytes per interior node (9 FP64 populations read + 9 written) over its mean
 launch time, timed with HIP events on the stream it runs on.
 cpu_baseline = the CPU oracle (C restatement of the reference step, OpenMP
 over the host cores) on the first steps of the same workload.
+
+--form batch: the parameter sweep of 96 seeded cases of uneven lattice sizes
+(512..2048 x half, 4*chord markers, random Re / heave / pitch / phase /
+Strouhal), all advanced together as one FP32-deviation batch with the
+autotuned launch shape, against the same cases run one after another on the
+plain FP64 path.  value = cases per hour of the batch; the JSON also holds
+the sequential rate, and per-case C_L / C_D errors of the batch against the
+FP64 runs.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -60,6 +70,110 @@ def cpu_baseline(case, seconds_budget=20.0, threads=None):
                       "oracle/liblbmib_oracle.so, %d OpenMP threads" % (n, case.params.nx, case.params.ny, threads)}
 
 
+def coefficient_errors(h32, h64, skip_frac=0.5):
+    """Errors of an FP32 force history against the FP64 one over the run's
+    last (1 - skip_frac): mean C_D relative, mean C_L relative to the C_L
+    r.m.s. (a symmetric section's mean C_L is near zero), peak |C_L|."""
+    k = int(len(h64) * skip_frac)
+    cd32, cd64 = h32[k:, 1], h64[k:, 1]
+    cl32, cl64 = h32[k:, 2], h64[k:, 2]
+    rms = float(np.sqrt(np.mean(cl64 ** 2)))
+    return {"mean_CD": float(abs(cd32.mean() - cd64.mean()) / abs(cd64.mean())),
+            "mean_CL": float(abs(cl32.mean() - cl64.mean()) / rms),
+            "peak_CL": float(abs(np.abs(cl32).max() - np.abs(cl64).max()) / np.abs(cl64).max())}
+
+
+def run_batch(cs, prec, steps, tune, chunk=1000, label=""):
+    """Advance all cases together; returns (seconds, histories, timings, shape)."""
+    from fortran_loadfile_mitiwing_amd import lbmib
+    sims = [lbmib.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib.Batch()
+    for s_ in sims:
+        b.add(s_)
+    shape = b.autotune(trial_steps=32)["best"] if tune else b.shape()
+    print("[batch] %s shape %s" % (label, shape), file=sys.stderr, flush=True)
+    b.set_profiling(PROFILE_EVERY)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        b.step(n)
+        done += n
+        print("[batch] %s %d steps, %.2f s" % (label, done, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    tm = b.timings()
+    hist = [s_.history() for s_ in sims]
+    b.close()
+    for s_ in sims:
+        s_.close()
+    return dt, hist, tm, shape
+
+
+def bench_batch(args):
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    steps = args.sweep_steps
+    cs = cases.sweep_cases(args.cases, n_steps=steps)
+    nodes = sum(c.params.nx * c.params.ny for c in cs)
+    interior = sum((c.params.nx - 2) * (c.params.ny - 2) for c in cs)
+    print("[batch] %d cases, %d lattice nodes, %d steps each" % (len(cs), nodes, steps), file=sys.stderr, flush=True)
+    t_batch, h32, tm, shape = run_batch(cs, lbmib.PREC_FP32DEV, steps, not args.no_tune, label="fp32")
+    # FP64 reference histories of the same cases (the batch's per-case
+    # arithmetic is the single-case path's, bit for bit)
+    t_b64, h64, _, _ = run_batch(cs, lbmib.PREC_FP64, steps, False, label="fp64")
+    errs = [coefficient_errors(a, b_) for a, b_ in zip(h32, h64)]
+    # sequential plain FP64 runs: setup + 20 untimed + 200 timed steps per
+    # case, the run time taken as setup + steps x (ms per timed step)
+    t_seq, seq_w, seq_k = 0.0, 20, 200
+    for i, c in enumerate(cs):
+        t0 = time.perf_counter()
+        g = lbmib.LBMIB(c)
+        t_setup = time.perf_counter() - t0
+        g.evolution(seq_w)
+        t0 = time.perf_counter()
+        g.evolution(seq_k)
+        t_seq += t_setup + (time.perf_counter() - t0) / seq_k * steps
+        g.close()
+        if i % 16 == 15:
+            print("[batch] sequential FP64 timing: %d cases" % (i + 1), file=sys.stderr, flush=True)
+    worst = {k: max(e[k] for e in errs) for k in errs[0]}
+    lat_s = tm["lattice_ms"] * 1e-3
+    achieved = interior * BYTES_PER_NODE_F32 / lat_s / 1e9
+    out = {
+        "metric": "sweep cases/hr (batched FP32-deviation form, %d cases x %d steps)" % (len(cs), steps),
+        "value": round(len(cs) / t_batch * 3600.0, 1),
+        "unit": "cases/hr",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(t_batch / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 lattice, f64 immersed boundary",
+        "data": "synthetic (seeded sweep of NACA0012 cases, seed %d)" % cases.SWEEP_SEED,
+        "config": {"workload": "parameter sweep: %d cases, nx in 512..2048 step 128, ny = nx/2, chord nx/16, "
+                               "4*chord markers, Re 100..1000, heave 0.25..1c, pitch 0..45 deg, phase 0..180 deg, "
+                               "St 0.1..0.5, %d steps each" % (len(cs), steps),
+                   "lattice_nodes": nodes},
+        "sequential_plain_fp64_cases_per_hr": round(len(cs) / t_seq * 3600.0, 1),
+        "sequential_plain_timing": "per case: setup + %d untimed + %d timed steps, scaled to %d steps"
+                                   % (seq_w, seq_k, steps),
+        "speedup_vs_sequential_plain": round(t_seq / t_batch, 3),
+        "batched_fp64_cases_per_hr": round(len(cs) / t_b64 * 3600.0, 1),
+        "aggregate_MLUPS": round(nodes * steps / t_batch / 1e6, 1),
+        "accuracy_vs_fp64_worst_case": {k: float("%.3g" % v) for k, v in worst.items()},
+        "accuracy_window": "last half of each run",
+        "cases_within_0.5pct": int(sum(all(v <= 0.005 for v in e.values()) for e in errs)),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_lattice<float,%d> over all cases" % shape["nodes_per_thread"],
+                     "bytes_per_launch": interior * BYTES_PER_NODE_F32, "mean_launch_ms": round(lat_s * 1e3, 5)},
+        "phases_ms": {k: round(v, 5) for k, v in tm.items()},
+        "launch_shape": shape,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +183,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
+    ap.add_argument("--form", choices=["step", "batch"], default="step",
+                    help="step: one case's timestep (default); batch: the 96-case parameter sweep")
+    ap.add_argument("--cases", type=int, default=96, help="--form batch: cases in the sweep")
+    ap.add_argument("--sweep-steps", type=int, default=10000, help="--form batch: steps per case")
+    ap.add_argument("--no-tune", action="store_true", help="--form batch: keep the default launch shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (the known-size kernel that scales PMC byte counters)")
     args = ap.parse_args()
@@ -85,6 +204,8 @@ def main():
         print("bench.py: this build runs one case on one GPU; --gpus %d is not supported "
               "(the path shards by case across GPUs, see the design notes)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.form == "batch":
+        return bench_batch(args)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
     fp32 = args.precision == "fp32"

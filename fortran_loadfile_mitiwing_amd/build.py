@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["lbmib_kernels.hip", "lbmib_api.hip"]
+HIP_SOURCES = ["lbmib_kernels.hip", "lbmib_api.hip", "lbmib_engine.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
                "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-Wall", "-Wno-unused-function"]

@@ -258,3 +258,62 @@ def reference_boundary_case(boundary_xy: Tuple[np.ndarray, np.ndarray], nx: int 
     p = CaseParams(nx=nx, ny=ny, ox=ox, oy=oy, scale=scale, U=U, s=s, n_wings=1,
                    pts_per_wing=len(xb))
     return Case(p, np.asarray(xb), np.asarray(yb), motion, n_steps, "reference_boundary")
+
+
+SWEEP_SEED = 20261203
+
+
+def draw_case(rng: np.random.Generator, n_steps: int, nx_max: int = 2048, U: float = 0.05,
+              name: str = "") -> Case:
+    """One parameter-sweep case drawn from ``rng``:
+
+    * nx uniform in {512, 640, ..., nx_max}, ny = nx/2, chord nx/16 lattice units
+      with 4 markers per lattice unit of chord (4*chord markers);
+    * Re uniform in [100, 1000] on the chord; U = 0.05 lattice units per step;
+    * heave amplitude uniform in [0.25, 1.0] chords, pitch amplitude uniform in
+      [0, 45] deg about the quarter chord, pitch leading heave by a phase
+      uniform in [0, 180] deg;
+    * Strouhal number uniform in [0.1, 0.5], defined on twice the heave
+      amplitude: f = St U / (2 h0).
+    """
+    nx = int(rng.choice(np.arange(512, nx_max + 1, 128)))
+    Re = float(rng.uniform(100.0, 1000.0))
+    heave = float(rng.uniform(0.25, 1.0))
+    pitch = float(rng.uniform(0.0, 45.0))
+    phase = float(rng.uniform(0.0, 180.0))
+    st = float(rng.uniform(0.1, 0.5))
+    chord = nx / 16.0
+    return make_case(nx, nx // 2, chord, int(4 * chord), Re, n_steps, U=U, heave=heave, pitch_deg=pitch,
+                     phase_deg=phase, strouhal=st,
+                     name=name or "sweep_%dx%d_Re%.0f_h%.2f_p%.0f_ph%.0f_St%.2f" % (nx, nx // 2, Re, heave, pitch,
+                                                                                   phase, st))
+
+
+def sweep_cases(n_cases: int = 96, n_steps: int = 10000, seed: int = SWEEP_SEED, nx_max: int = 2048) -> List[Case]:
+    """The batched-form workload: ``n_cases`` independent cases drawn by
+    :func:`draw_case` from ``numpy.random.default_rng(seed)``."""
+    rng = np.random.default_rng(seed)
+    return [draw_case(rng, n_steps, nx_max) for _ in range(n_cases)]
+
+
+@dataclasses.dataclass
+class Request:
+    """One request of the serving workload: a case and its arrival time (s)."""
+    case: Case
+    arrival: float
+    index: int
+
+
+def serving_requests(n_requests: int = 500, rate_per_s: float = 1.0, seed: int = SWEEP_SEED + 1,
+                     nx_max: int = 4096, steps_lo: int = 2000, steps_hi: int = 20000) -> List[Request]:
+    """The serving workload: cases drawn as :func:`draw_case` with nx up to
+    ``nx_max`` and a step count log-uniform in [steps_lo, steps_hi], arriving
+    as a Poisson process of rate ``rate_per_s`` (exponential gaps)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        steps = int(round(math.exp(rng.uniform(math.log(steps_lo), math.log(steps_hi)))))
+        out.append(Request(draw_case(rng, steps, nx_max, name="req%03d" % k), t, k))
+    return out

@@ -1,10 +1,8 @@
 // lbmib_api.hip -- host side of liblbmib: the C ABI declared in include/lbmib.h.
 //
-// Owns the device state of a case, builds the per-step kinematics from the
-// motion table exactly as the reference does (mode.f90:199-283), and drives
-// the kernels of lbmib_kernels.hip on two HIP streams per handle:
-//   s0: lattice kernel, outer-boundary kernel, IB-node collide
-//   s1: immersed-boundary chain (runs beside the lattice kernel)
+// Owns the device state of a case and builds the per-step kinematics from
+// the motion table exactly as the reference does (mode.f90:199-283).  Steps
+// run on the case's own engine (lbmib_engine.hip), or on a batch's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,38 +14,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/lbmib.h"
-#include "lbmib_kernels.h"
-
-namespace lbmib {
-// kernels (lbmib_kernels.hip)
-template <typename T> __global__ void k_lattice(DevCase, int, int, double, double*, int);
-int lattice_ring_blocks(int nx, int ny);
-int lattice_blocks(int nx, int ny);
-__global__ void k_geo_clear(DevCase, int);
-__global__ void k_geo_markers(DevCase, int, int);
-__global__ void k_geo_fill(DevCase, int);
-__global__ void k_geo_phi(DevCase, int);
-__global__ void k_geo_pairs(DevCase, int);
-__global__ void k_geo_dsort(DevCase, int);
-__global__ void k_geo_nsort(DevCase, int);
-__global__ void k_geo_assemble(DevCase, int);
-__global__ void k_geo_factor(DevCase, int);
-template <typename T> __global__ void k_crit_rhs(DevCase, int, int);
-template <int U> __global__ void k_crit_solve(DevCase, int);
-int solve_unit(int npad);
-int solve_threads();
-template <typename T> __global__ void k_band(DevCase, int, int, double*);
-__global__ void k_forces(DevCase, int, int);
-size_t factor_smem_bytes();
-size_t solve_smem_bytes(int npad);
-__global__ void k_residual(const double*, const double*, int, int, long long, double*);
-template <typename T> __global__ void k_nan_check(const T*, long long, int*);
-}  // namespace lbmib
+#include "lbmib_host.h"
 
 using namespace lbmib;
 
-namespace {
+namespace lbmib {
 
 thread_local std::string g_err;
 
@@ -62,12 +33,13 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(LBMIB_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
-    } while (0)
+}  // namespace lbmib
+
+using namespace lbmib;
+
+namespace {
+
+constexpr int GEO_BATCH = 8;  // steps per geometry batch (slots = 2 * batch)
 
 // Gauss-Jordan inverse of the MRT transform as nizhen computes it
 // (solution.f90:1462-1517), row-major.
@@ -175,41 +147,6 @@ void zigzag(int nw, int spw, std::vector<int>& perm)
 
 }  // namespace
 
-struct lbmib_handle {
-    lbmib_params p;
-    DevCase D;
-    // s0 lattice; s1 triangular solves (on CUs of their own, see
-    // lbmib_create); s2 IB geometry of later batches and forces of earlier
-    // ones; s3 right-hand sides of the IB system.
-    static constexpr int NSTREAM = 4;
-    hipStream_t s[NSTREAM] = {};
-    static constexpr int RING = 64;  // per-step events of step n live in slot n % RING
-    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {};
-    static constexpr int BRING = 8;  // per-batch events of batch j live in slot j % BRING
-    hipEvent_t ev_built[BRING] = {}, ev_forces[BRING] = {};
-    int batch = 8;                   // steps per geometry batch (slots = 2 * batch)
-    int built_batches = 0;           // batches whose geometry has been enqueued
-    int forces_upto = 0;             // steps whose forces have been enqueued
-    std::vector<double> xb0, yb0, arc_l;  // polygon after the (xc-xp, yc-yp) shift
-    std::vector<int> perm, iperm;
-    double cs = 0.0;
-    // kinematics
-    bool have_motion = false;
-    std::vector<double> step_rows;  // per step: the motion row used (ncols)
-    int ncols = 0, max_steps = 0;
-    double motion_end_time = 0.0;
-    int steps_done = 0;
-    int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
-    // device allocations to free
-    std::vector<void*> allocs;
-    // profiling: phase events on every prof_every-th step (0: off)
-    int prof_every = 0;
-    std::vector<hipEvent_t> evpool, gevpool;
-    int gev_used = 0;
-    double tms[LBMIB_NTIMINGS] = {};
-    int dev = 0;
-};
-
 namespace {
 
 template <typename T>
@@ -224,11 +161,10 @@ int dalloc(lbmib_handle* h, T** ptr, size_t count)
     return 0;
 }
 
-int fill_i32(int* d, int v, size_t n, hipStream_t s)
+int fill_i32(int* d, int v, size_t n)
 {
     std::vector<int> hv(n, v);
-    HIP_TRY(hipMemcpyAsync(d, hv.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(d, hv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -268,19 +204,11 @@ int lbmib_destroy(lbmib_handle* h)
 {
     if (!h) return 0;
     (void)hipSetDevice(h->dev);
-    for (hipStream_t st : h->s)
-        if (st) (void)hipStreamSynchronize(st);
+    if (h->batch) (void)lbmib_batch_remove(h->batch, h);
+    engine_destroy(h->own);
+    h->own = nullptr;
+    (void)hipDeviceSynchronize();
     for (void* p : h->allocs) (void)hipFree(p);
-    for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
-    for (hipEvent_t e : h->gevpool) (void)hipEventDestroy(e);
-    for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i], h->ev_rhs[i]})
-            if (e) (void)hipEventDestroy(e);
-    for (int i = 0; i < lbmib_handle::BRING; ++i)
-        for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]})
-            if (e) (void)hipEventDestroy(e);
-    for (hipStream_t st : h->s)
-        if (st) (void)hipStreamDestroy(st);
     delete h;
     return 0;
 }
@@ -329,42 +257,10 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
         }                                                                         \
     } while (0)
     HTRY(hipSetDevice(h->dev));
-    {
-        // The triangular solves run as one workgroup whose critical wavefront
-        // must not queue behind the lattice kernel's memory traffic or wait
-        // for a CU that the lattice kernel keeps full, so stream s1 gets
-        // LBMIB_IB_CUS CUs (default 8, one per XCD) that no other stream uses.
-        // The mask bits are spread over the accelerator's 8 XCDs and a queue
-        // whose mask leaves an XCD without CUs is not usable, so only whole
-        // multiples of 8 are taken; 0 turns the split off.
-        hipDeviceProp_t prop;
-        HTRY(hipGetDeviceProperties(&prop, h->dev));
-        const int ncu = prop.multiProcessorCount;
-        int ib_cus = 8;
-        if (const char* e = getenv("LBMIB_IB_CUS")) ib_cus = atoi(e);
-        ib_cus = std::max(0, ib_cus) / 8 * 8;
-        if (ib_cus > 0 && ib_cus < ncu / 2 && ncu % 8 == 0) {
-            const int nwords = (ncu + 31) / 32;
-            std::vector<uint32_t> lat(nwords, 0u), ib(nwords, 0u);
-            for (int c = 0; c < ncu; ++c) {
-                if (c >= ncu - ib_cus) ib[c / 32] |= 1u << (c % 32);
-                else lat[c / 32] |= 1u << (c % 32);
-            }
-            for (int i = 0; i < lbmib_handle::NSTREAM; ++i)
-                HTRY(hipExtStreamCreateWithCUMask(&h->s[i], nwords, i == 1 ? ib.data() : lat.data()));
-        } else {
-            for (int i = 0; i < lbmib_handle::NSTREAM; ++i)
-                HTRY(hipStreamCreateWithFlags(&h->s[i], hipStreamNonBlocking));
-        }
-    }
-    for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t* e : {&h->ev_ib[i], &h->ev_step[i], &h->ev_rhs[i]})
-            HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    for (int i = 0; i < lbmib_handle::BRING; ++i)
-        for (hipEvent_t* e : {&h->ev_built[i], &h->ev_forces[i]})
-            HTRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    if (const char* e = getenv("LBMIB_GEO_BATCH")) h->batch = std::max(1, std::min(64, atoi(e)));
-
+    TRY(engine_create(h->dev, &h->own));
+    h->own->members.push_back(h);
+    h->in_xb0.assign(xb0, xb0 + (size_t)p->n_wings * p->pts_per_wing);
+    h->in_yb0.assign(yb0, yb0 + (size_t)p->n_wings * p->pts_per_wing);
     const int nx = p->nx, ny = p->ny, nw = p->n_wings, ppw = p->pts_per_wing;
     const long long N = (long long)nx * ny;
     DevCase& D = h->D;
@@ -390,9 +286,9 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     }
     D.smooth = p->smooth;
     D.nw = nw; D.ppw = ppw; D.spw = ppw - 1; D.nseg = nw * (ppw - 1);
-    D.cap = D.nseg * BOXMAX;
-    D.nslot = 2 * h->batch;
-    h->cs = (double)std::sqrt(3.0f) / 3.0;  // REAL(4) sqrt (boundary_condition.f90:304)
+    D.batch = GEO_BATCH;
+    D.nslot = 2 * GEO_BATCH;
+    D.cs = (double)std::sqrt(3.0f) / 3.0;  // REAL(4) sqrt (boundary_condition.f90:304)
 
     // polygon after read_boundary's shift (airfoil.f90:235-241)
     int npts = nw * ppw;
@@ -406,6 +302,20 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     for (int i = 0; i < npts; ++i) {
         h->xb0[i] = xb0[i] + (p->xc - p->xp);
         h->yb0[i] = yb0[i] + (p->yc - p->yp);
+    }
+    // IB node capacity: a node belongs to the set when it lies in the
+    // +-2.5 lattice-unit box (at most 6 x 6 nodes) of a marker; sweeping that
+    // box along a contour of length P covers at most 6 (|cos a| + |sin a|) <=
+    // 8.49 nodes per lattice unit of arc, so 10 P + 256 per wing bounds the
+    // set; a set that still overflows reports LBMIB_E_CAPACITY.
+    {
+        double cap = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            double P = 0.0;
+            for (int i = 0; i < ppw - 1; ++i) P += h->arc_l[w * (ppw - 1) + i];
+            cap += 10.0 * P + 256.0;
+        }
+        D.cap = (int)std::min((double)D.nseg * BOXMAX, std::ceil(cap));
     }
     zigzag(nw, D.spw, h->perm);
     h->iperm.assign(D.nseg, 0);
@@ -469,14 +379,9 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     TRY(dalloc(h, &D.y, 2 * (size_t)ns));
     TRY(dalloc(h, &D.err, 2));  // [0] ErrBits, [1] NaN-guard flag
     TRY(dalloc(h, &D.red, 2 * 4096));
-    TRY(fill_i32(D.err, 0, 2, h->s[0]));
+    TRY(fill_i32(D.err, 0, 2));
     D.bw = geometric_bandwidth(h);
     if (D.bw >= ns) D.bw = ns - 1;
-    for (int i = 0; i < lbmib_handle::RING; ++i)
-        for (hipEvent_t e : {h->ev_ib[i], h->ev_step[i], h->ev_rhs[i]}) HTRY(hipEventRecord(e, h->s[0]));
-    for (int i = 0; i < lbmib_handle::BRING; ++i)
-        for (hipEvent_t e : {h->ev_built[i], h->ev_forces[i]}) HTRY(hipEventRecord(e, h->s[0]));
-    HTRY(hipStreamSynchronize(h->s[0]));
     *out = h;
     return 0;
 #undef TRY
@@ -494,6 +399,7 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
             return fail(LBMIB_E_ARG, "motion table time column must be non-decreasing (row %d)", r + 1);
     if (h->have_motion) return fail(LBMIB_E_STATE, "the motion table can be set once per handle");
     HIP_TRY(hipSetDevice(h->dev));
+    h->motion_in.assign(rows, rows + (size_t)nrows * ncols);
     MotionTable m;
     m.t.assign(rows, rows + (size_t)nrows * ncols);
     m.rows = nrows; m.ncols = ncols;
@@ -579,10 +485,6 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
                     "(wings too close together, or markers much denser than the lattice)", D.bw, MAXBW);
     D.ws = 64;
     D.npad = (D.nseg + 63) / 64 * 64;
-    HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)factor_smem_bytes()));
-    HIP_TRY(hipFuncSetAttribute(solve_unit(D.npad) == 64 ? (const void*)k_crit_solve<64> : (const void*)k_crit_solve<32>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_smem_bytes(D.npad)));
     double* dkin;
     if (dalloc(h, &dkin, kin.size())) return LBMIB_E_HIP;
     HIP_TRY(hipMemcpy(dkin, kin.data(), sizeof(double) * kin.size(), hipMemcpyHostToDevice));
@@ -631,215 +533,53 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         HIP_TRY(hipMemset(D.dpool, 0, sizeof(double) * (size_t)D.nslot * dd.total));
         for (int g = 0; g < D.nslot; ++g) {
             Slot S = slot_of(D, g);
-            if (fill_i32(S.claim, INT_MAX, (size_t)grid, h->s[0])) return LBMIB_E_HIP;
-            if (fill_i32(S.nodeidx, -1, (size_t)grid, h->s[0])) return LBMIB_E_HIP;
-            if (fill_i32(S.nadj, 0, 1, h->s[0])) return LBMIB_E_HIP;
+            if (fill_i32(S.claim, INT_MAX, (size_t)grid)) return LBMIB_E_HIP;
+            if (fill_i32(S.nodeidx, -1, (size_t)grid)) return LBMIB_E_HIP;
+            if (fill_i32(S.nadj, 0, 1)) return LBMIB_E_HIP;
         }
     }
     h->have_motion = true;
     return 0;
 }
 
-// Geometry of batch j (steps j*B+1 .. j*B+B, slots (j*B .. j*B+B-1) % nslot)
-// on s2: node sets, kernel weights, bands of D D^T and their factors, one
-// slot per step in parallel.  The slots were last used by batch j-2, which
-// must have finished its IB collides and forces.
-static int enqueue_batch(lbmib_handle* h, int j, hipEvent_t* ev)
-{
-    DevCase& D = h->D;
-    const int B = h->batch;
-    const int n0 = j * B + 1;
-    const int nb = std::min(B, h->max_steps - n0 + 1);
-    if (nb <= 0) return 0;
-    const int g0 = (n0 - 1) % D.nslot;
-    hipStream_t st = h->s[2];
-    if (j >= 2) {
-        // the lattice update of its last step reads the node grid; its
-        // forces wait for its IB-node collides
-        const int last = (j - 1) * B;  // last step of batch j-2
-        HIP_TRY(hipStreamWaitEvent(st, h->ev_step[last % lbmib_handle::RING], 0));
-        HIP_TRY(hipStreamWaitEvent(st, h->ev_forces[(j - 2) % lbmib_handle::BRING], 0));
-    }
-    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
-    const int ns = D.nseg, cap = D.cap;
-    hipLaunchKernelGGL(k_geo_clear, dim3((cap + 255) / 256, nb), dim3(256), 0, st, D, g0);
-    hipLaunchKernelGGL(k_geo_markers, dim3((ns + 255) / 256, nb), dim3(256), 0, st, D, g0, n0 - 1);
-    hipLaunchKernelGGL(k_geo_fill, dim3(nb), dim3(1024), 0, st, D, g0);
-    hipLaunchKernelGGL(k_geo_phi, dim3((ns * 12 + 255) / 256, nb), dim3(256), 0, st, D, g0);
-    hipLaunchKernelGGL(k_geo_pairs, dim3((ns * BOXMAX + 255) / 256, nb), dim3(256), 0, st, D, g0);
-    hipLaunchKernelGGL(k_geo_dsort, dim3((ns * BOXMAX + 255) / 256, nb), dim3(256), 0, st, D, g0);
-    hipLaunchKernelGGL(k_geo_nsort, dim3(64, nb), dim3(256), 0, st, D, g0);
-    long long nbw = (long long)ns * (D.bw + 1);
-    hipLaunchKernelGGL(k_geo_assemble, dim3((unsigned)((nbw + 255) / 256), nb), dim3(256), 0, st, D, g0);
-    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    hipLaunchKernelGGL(k_geo_factor, dim3(nb), dim3(256), factor_smem_bytes(), st, D, g0);
-    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    HIP_TRY(hipEventRecord(h->ev_built[j % lbmib_handle::BRING], st));
-    h->built_batches = j + 1;
-    return 0;
-}
+}  // extern "C"
 
-// Forces of steps first..last (one workgroup each) on s2, ahead of the
-// geometry of later batches.
-static int enqueue_forces(lbmib_handle* h, int first, int last)
-{
-    if (last < first) return 0;
-    DevCase& D = h->D;
-    hipStream_t st = h->s[2];
-    HIP_TRY(hipStreamWaitEvent(st, h->ev_ib[last % lbmib_handle::RING], 0));
-    hipLaunchKernelGGL(k_forces, dim3(last - first + 1), dim3(256), 0, st, D, (first - 1) % D.nslot, first - 1);
-    const int j = (last - 1) / h->batch;
-    HIP_TRY(hipEventRecord(h->ev_forces[j % lbmib_handle::BRING], st));
-    h->forces_upto = last;
-    return 0;
-}
+namespace lbmib {
 
-// Step n.  f[n&1] is written from f[(n-1)&1]:
-//   s3  right-hand side of the IB system (after the lattice update and the IB
-//       nodes of step n-1, and the geometry of batch j)
-//   s1  triangular solves, then the forced collide of the IB nodes
-//   s0  lattice update of every other node (after the IB-node collide of step
-//       n-1: that wrote nodes it reads, and read the array it overwrites)
-// so the IB chain of step n runs beside the lattice update of step n.
-// ev (when profiling this step): [0]/[1] around the lattice update, [2] before
-// the right-hand side, [3] after the solves, [4] after the IB-node collide.
-static int launch_step(lbmib_handle* h, int n, bool flag, hipEvent_t* ev)
+int nan_guard_check(lbmib_handle* h)
 {
-    DevCase& D = h->D;
-    const int B = h->batch, R = lbmib_handle::RING, BR = lbmib_handle::BRING;
-    const int cur = n & 1, g = (n - 1) % D.nslot, j = (n - 1) / B;
-    const int nx = D.nx, ny = D.ny;
-    double* macro = flag ? D.macro[cur] : nullptr;
-    const int ns = D.nseg, cap = D.cap;
-    if ((n - 1) % B == 0) {
-        // a new batch starts: forces of the previous one, geometry of the next
-        if (h->forces_upto < n - 1) {
-            int rc = enqueue_forces(h, h->forces_upto + 1, n - 1);
-            if (rc) return rc;
-        }
-        for (int jj = h->built_batches; jj <= j + 1; ++jj) {
-            hipEvent_t* gev = nullptr;
-            if (h->prof_every > 0) {
-                while (h->gevpool.size() < (size_t)3 * (h->gev_used + 1)) {
-                    hipEvent_t e;
-                    HIP_TRY(hipEventCreate(&e));
-                    h->gevpool.push_back(e);
-                }
-                gev = &h->gevpool[(size_t)3 * h->gev_used++];
-            }
-            int rc = enqueue_batch(h, jj, gev);
-            if (rc) return rc;
-        }
-    }
-    hipStream_t s0 = h->s[0], s1 = h->s[1], s3 = h->s[3];
-    hipEvent_t built = h->ev_built[j % BR];
-    HIP_TRY(hipStreamWaitEvent(s3, h->ev_step[(n - 1) % R], 0));
-    HIP_TRY(hipStreamWaitEvent(s3, h->ev_ib[(n - 1) % R], 0));
-    HIP_TRY(hipStreamWaitEvent(s3, built, 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], s3));
-    const bool f64 = D.prec == LBMIB_PREC_FP64;
-    hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3((ns + 6) / 7), dim3(256), 0, s3, D, g, cur);
-    HIP_TRY(hipEventRecord(h->ev_rhs[n % R], s3));
-    HIP_TRY(hipStreamWaitEvent(s1, h->ev_rhs[n % R], 0));
-    if (solve_unit(D.npad) == 64)
-        hipLaunchKernelGGL(k_crit_solve<64>, dim3(1), dim3(solve_threads()), solve_smem_bytes(D.npad), s1, D, g);
+    if (!h->p.nan_guard || h->steps_done == 0) return 0;
+    int* flag = h->D.err + 1;
+    if (fill_i32(flag, 0, 1)) return LBMIB_E_HIP;
+    hipStream_t st = nullptr;  // every engine call has synchronised its own streams
+    if (h->D.prec == LBMIB_PREC_FP64)
+        hipLaunchKernelGGL(k_nan_check<double>, dim3(1024), dim3(256), 0, st,
+                           (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
     else
-        hipLaunchKernelGGL(k_crit_solve<32>, dim3(1), dim3(solve_threads()), solve_smem_bytes(D.npad), s1, D, g);
-    if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
-    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3((cap + 255) / 256), dim3(256), 0, s1, D, g, cur, macro);
-    if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
-    HIP_TRY(hipEventRecord(h->ev_ib[n % R], s1));
-    HIP_TRY(hipStreamWaitEvent(s0, h->ev_ib[(n - 1) % R], 0));
-    if ((n - 1) % B == 0) HIP_TRY(hipStreamWaitEvent(s0, built, 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    hipLaunchKernelGGL(f64 ? k_lattice<double> : k_lattice<float>, dim3(lattice_blocks(nx, ny)), dim3(256), 0, s0, D, g, cur, h->cs, macro,
-                       lattice_ring_blocks(nx, ny));
-    if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
-    HIP_TRY(hipEventRecord(h->ev_step[n % R], s0));
-    HIP_TRY(hipGetLastError());
-    if (flag) h->last_flag_step[cur] = n;
+        hipLaunchKernelGGL(k_nan_check<float>, dim3(1024), dim3(256), 0, st,
+                           (const float*)h->D.g[h->steps_done & 1], 9 * h->D.N, flag);
+    int f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (f) return fail(LBMIB_E_NAN, "NaN in the populations after step %d", h->steps_done);
     return 0;
 }
+
+}  // namespace lbmib
+
+extern "C" {
 
 int lbmib_step(lbmib_handle* h, int32_t n_steps)
 {
     if (!h) return fail(LBMIB_E_ARG, "null handle");
     if (!h->have_motion) return fail(LBMIB_E_STATE, "lbmib_set_motion_table must be called before lbmib_step");
+    if (h->batch) return fail(LBMIB_E_STATE, "the case belongs to a batch: step it with lbmib_batch_step");
     if (n_steps < 0) return fail(LBMIB_E_ARG, "n_steps must be >= 0");
     if (h->steps_done + n_steps > h->max_steps)
         return fail(LBMIB_E_ARG, "step %d beyond the motion table (last step %d)", h->steps_done + n_steps, h->max_steps);
-    HIP_TRY(hipSetDevice(h->dev));
-    const int E = 5, every = h->prof_every;
-    const int nsamp = every > 0 ? (n_steps + every - 1) / every : 0;
-    if (every > 0) {
-        while (h->evpool.size() < (size_t)E * nsamp + 2) {
-            hipEvent_t e;
-            HIP_TRY(hipEventCreate(&e));
-            h->evpool.push_back(e);
-        }
-        h->gev_used = 0;
-    }
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (every > 0) {
-        t0 = h->evpool[(size_t)E * nsamp];
-        t1 = h->evpool[(size_t)E * nsamp + 1];
-        HIP_TRY(hipEventRecord(t0, h->s[0]));
-    }
-    for (int s = 0; s < n_steps; ++s) {
-        int n = h->steps_done + 1;
-        bool flag = s >= n_steps - 2;
-        hipEvent_t* ev = every > 0 && s % every == 0 ? &h->evpool[(size_t)E * (s / every)] : nullptr;
-        int rc = launch_step(h, n, flag, ev);
-        if (rc) return rc;
-        h->steps_done = n;
-    }
-    if (every > 0) HIP_TRY(hipEventRecord(t1, h->s[0]));
-    // forces of the steps taken so far (the history is complete on return)
-    if (h->forces_upto < h->steps_done) {
-        int rc = enqueue_forces(h, h->forces_upto + 1, h->steps_done);
-        if (rc) return rc;
-    }
-    for (hipStream_t st : h->s) HIP_TRY(hipStreamSynchronize(st));
-    int err = 0;
-    HIP_TRY(hipMemcpy(&err, h->D.err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) {
-        if (err & ERR_GEOMETRY) return fail(LBMIB_E_GEOMETRY, "an immersed-boundary node left the interior margin (4..nx-3, 4..ny-3)");
-        if (err & ERR_CAPACITY) return fail(LBMIB_E_CAPACITY, "an immersed-boundary list overflowed");
-        if (err & ERR_SOLVE) return fail(LBMIB_E_SOLVE, "immersed-boundary system not positive definite");
-    }
-    if (every > 0 && n_steps > 0) {
-        float ms;
-        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-        h->tms[0] += ms;
-        h->tms[1] += n_steps;
-        for (int q = 0; q < nsamp; ++q) {
-            hipEvent_t* ev = &h->evpool[(size_t)E * q];
-            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); h->tms[3] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); h->tms[4] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4])); h->tms[5] += ms;
-        }
-        h->tms[2] += nsamp;
-        for (int q = 0; q < h->gev_used; ++q) {
-            hipEvent_t* ev = &h->gevpool[(size_t)3 * q];
-            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); h->tms[6] += ms;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); h->tms[7] += ms;
-        }
-        h->tms[8] += h->gev_used;
-    }
-    if (h->p.nan_guard) {
-        int* flag = h->D.err + 1;
-        if (fill_i32(flag, 0, 1, h->s[0])) return LBMIB_E_HIP;
-        if (h->D.prec == LBMIB_PREC_FP64)
-            hipLaunchKernelGGL(k_nan_check<double>, dim3(1024), dim3(256), 0, h->s[0],
-                               (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
-        else
-            hipLaunchKernelGGL(k_nan_check<float>, dim3(1024), dim3(256), 0, h->s[0],
-                               (const float*)h->D.g[h->steps_done & 1], 9 * h->D.N, flag);
-        int f = 0;
-        HIP_TRY(hipMemcpy(&f, flag, sizeof(int), hipMemcpyDeviceToHost));
-        if (f) return fail(LBMIB_E_NAN, "NaN in the populations after step %d", h->steps_done);
-    }
-    return 0;
+    int rc = engine_run(h->own, n_steps);
+    if (rc) return rc;
+    return nan_guard_check(h);
 }
 
 int lbmib_steps_done(const lbmib_handle* h, int32_t* n)
@@ -958,11 +698,12 @@ int lbmib_residual(lbmib_handle* h, double* err)
         return fail(LBMIB_E_STATE, "velocity fields of steps %d and %d were not recorded", n - 1, n);
     HIP_TRY(hipSetDevice(h->dev));
     const int blocks = 1024;
-    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, h->s[0], (const double*)h->D.macro[sc],
+    hipStream_t st = nullptr;  // every engine call has synchronised its own streams
+    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, st, (const double*)h->D.macro[sc],
                        (const double*)h->D.macro[sp], h->D.nx, h->D.ny, h->D.N, h->D.red);
     std::vector<double> part(2 * blocks);
-    HIP_TRY(hipMemcpyAsync(part.data(), h->D.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->s[0]));
-    HIP_TRY(hipStreamSynchronize(h->s[0]));
+    HIP_TRY(hipMemcpyAsync(part.data(), h->D.red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     double t1 = 0.0, t2 = 0.0;
     for (int b = 0; b < blocks; ++b) { t1 += part[2 * b]; t2 += part[2 * b + 1]; }
     *err = std::sqrt(t1) / (std::sqrt(t2) + 1e-30);
@@ -999,15 +740,15 @@ int lbmib_set_profiling(lbmib_handle* h, int32_t every)
 {
     if (!h) return fail(LBMIB_E_ARG, "null handle");
     if (every < 0) return fail(LBMIB_E_ARG, "profiling interval must be >= 0");
-    h->prof_every = every;
-    for (double& t : h->tms) t = 0.0;
+    h->own->prof_every = every;
+    for (double& t : h->own->tms) t = 0.0;
     return 0;
 }
 
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n)
 {
     if (!h || !t) return fail(LBMIB_E_ARG, "null argument");
-    for (int i = 0; i < n && i < LBMIB_NTIMINGS; ++i) t[i] = h->tms[i];
+    for (int i = 0; i < n && i < LBMIB_NTIMINGS; ++i) t[i] = h->own->tms[i];
     for (int i = LBMIB_NTIMINGS; i < n; ++i) t[i] = 0.0;
     return 0;
 }

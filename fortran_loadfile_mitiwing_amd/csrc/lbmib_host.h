@@ -1,0 +1,111 @@
+// lbmib_host.h -- host-side state of liblbmib shared by its translation
+// units: the case handle, and the engine that drives the kernels of one or
+// more cases on one device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lbmib.h"
+#include "lbmib_kernels.h"
+
+namespace lbmib {
+
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::lbmib::fail(LBMIB_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Engine;
+
+}  // namespace lbmib
+
+struct lbmib_handle {
+    lbmib_params p;
+    lbmib::DevCase D;
+    lbmib::Engine* own = nullptr;    // engine that steps this case alone
+    lbmib_batch* batch = nullptr;    // batch the case belongs to, if any
+    std::vector<double> in_xb0, in_yb0;   // polygon as given to lbmib_create
+    std::vector<double> motion_in;        // motion table as given (rows x ncols)
+    std::vector<double> xb0, yb0, arc_l;  // polygon after the (xc-xp, yc-yp) shift
+    std::vector<int> perm, iperm;
+    // kinematics
+    bool have_motion = false;
+    std::vector<double> step_rows;  // per step: the motion row used (ncols)
+    int ncols = 0, max_steps = 0;
+    double motion_end_time = 0.0;
+    int steps_done = 0;
+    int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
+    std::vector<void*> allocs;        // device allocations to free
+    int dev = 0;
+};
+
+namespace lbmib {
+
+// How a window's kernels are launched; see lbmib_launch_shape in lbmib.h.
+struct Shape {
+    int npt = 1;
+    int order = 0;
+    int persist = 0;
+    int ib_cus = 8;
+    int graph = 0;
+};
+
+// Drives the kernels of a set of cases on one device.  Cases advance in
+// windows (one call); within a window every kernel covers all cases in one
+// launch, each case at its own step (CaseClock).  Streams, all at the
+// default priority:
+//   s0  lattice update (every node but the step's IB nodes, and the ring)
+//   s1  triangular solves + forced collide of the IB nodes (reserved CUs)
+//   s2  geometry of later batches, forces of earlier ones
+//   s3  right-hand sides of the IB systems
+struct Engine {
+    int dev = 0;
+    int ncu = 0;
+    int B = 8;                       // steps per geometry batch
+    Shape shape;
+    static constexpr int NSTREAM = 4;
+    hipStream_t s[NSTREAM] = {};
+    int streams_cus = -1;            // ib_cus the streams were made with
+    static constexpr int RING = 64;  // per-step events of engine step q live in slot q % RING
+    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {};
+    static constexpr int BRING = 8;  // per-boundary events
+    hipEvent_t ev_first[BRING] = {}, ev_next[BRING] = {}, ev_fork = nullptr, ev_join[NSTREAM] = {};
+    long long gstep = 0, gbound = 0;
+    std::vector<lbmib_handle*> members;
+    // device tables
+    DevCase* dcases = nullptr;
+    CaseClock* dclock = nullptr;
+    int2* dtiles = nullptr;
+    int* derr = nullptr;             // gathered error flags of the members
+    size_t case_cap = 0, tile_cap = 0, err_cap = 0;
+    int ntiles = 0;
+    bool tiles_valid = false;
+    std::vector<CaseClock> hclock;
+    // captured window (graph mode)
+    hipGraphExec_t gexec = nullptr;
+    int graph_len = 0;
+    bool graph_valid = false;
+    // profiling
+    int prof_every = 0;
+    std::vector<hipEvent_t> evpool, gevpool;
+    int gev_used = 0;
+    double tms[LBMIB_NTIMINGS] = {};
+};
+
+int engine_create(int dev, Engine** out);
+void engine_destroy(Engine* E);
+// every member advances min(n_steps, its remaining steps); synchronous
+int engine_run(Engine* E, int n_steps);
+int engine_set_shape(Engine* E, const Shape& s);
+int shape_from_env(Shape& s);
+// NaN guard of one case after a call (no-op unless params.nan_guard)
+int nan_guard_check(lbmib_handle* h);
+
+}  // namespace lbmib

@@ -48,7 +48,8 @@ struct SlotDoubles {
     long long total;
 };
 
-// All device pointers and scalars of one case, passed by value to kernels.
+// All device pointers and scalars of one case.  Kernels read it from a
+// device table of the cases they work on (one for a single handle).
 // The immersed-boundary geometry of step n (node set, kernel weights, factor
 // of D D^T) lives in slot (n-1) % nslot.  Geometry depends only on the
 // prescribed kinematics, so it is built in batches of steps ahead of time.
@@ -65,7 +66,9 @@ struct DevCase {
     int npad;               // nseg rounded up to a multiple of 64
     int steps_cap;          // rows of kin / hist
     int nslot;
+    int batch;              // steps per geometry batch (nslot = 2 * batch)
     int gi0, gj0, gw, gh;   // lattice window (0-based origin, size) holding every IB node
+    double cs;              // sound speed of the characteristic boundary, REAL(4) sqrt(3)/3
     // lattice (population-planar, x fastest): FP64 populations f, or in the
     // FP32 deviation form g = f - W
     int prec;               // LBMIB_PREC_FP64 / LBMIB_PREC_FP32DEV
@@ -96,6 +99,20 @@ struct DevCase {
     double* red;            // reduction scratch
 };
 
+// Steps of one case inside one launch window (one lbmib_step or batch call):
+// a kernel launched at window offset `off` works on step n = n0 + off of the
+// case, skips the case when n > n_end, and records rho/u/v when
+// n >= flag_from.  Cases of one window may stand at different steps, but all
+// at the same position inside a geometry batch.
+struct CaseClock {
+    int n0, n_end, flag_from, pad;
+};
+
+// Geometry launch modes: GEO_FIRST builds batch 0 of a case that takes its
+// first step at this offset; GEO_NEXT builds the batch after the one that
+// starts at this offset.
+enum GeoMode { GEO_FIRST = 0, GEO_NEXT = 1 };
+
 // One slot's arrays.
 struct Slot {
     int *claim, *nodeidx, *box, *draw_idx, *nraw_s, *adj, *nadj, *dcnt, *didx, *nmkcnt, *nmk;
@@ -117,5 +134,35 @@ __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
     S.dinv = db + D.dof.dinv; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
     return S;
 }
+
+// ---- kernels (lbmib_kernels.hip); C: case table, K: clocks of the window ----
+template <typename T, int NPT>
+__global__ void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                          const int2* __restrict__ tiles, int ntiles, int off);
+__host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
+int lattice_tiles(int nx, int ny, int npt);  // ring blocks + interior tiles of one case
+__global__ void k_geo_clear(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_markers(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_fill(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_phi(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_pairs(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_dsort(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_nsort(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_assemble(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_factor(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+template <typename T>
+__global__ void k_crit_rhs(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+template <int U>
+__global__ void k_crit_solve(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+int solve_unit(int npad);
+int solve_threads();
+template <typename T>
+__global__ void k_band(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+__global__ void k_forces(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+__global__ void k_clock_advance(CaseClock* __restrict__, int ncases, int steps);
+size_t factor_smem_bytes();
+size_t solve_smem_bytes(int npad);
+__global__ void k_residual(const double*, const double*, int, int, long long, double*);
+template <typename T> __global__ void k_nan_check(const T*, long long, int*);
 
 }  // namespace lbmib

@@ -21,21 +21,26 @@ namespace lbmib {
 // lattice
 // ---------------------------------------------------------------------------
 
+// Population planes are global memory: pointers read from the case table
+// are generic, so they are cast to the global address space to keep every
+// population access a global (not flat) memory instruction.
+#define GAS __attribute__((address_space(1)))
+
 // Stored population -> FP64 population: FP64 storage holds f itself, the
 // FP32 deviation form holds g = f - W.
 __device__ __forceinline__ double lift(double v, double) { return v; }
 __device__ __forceinline__ double lift(float v, double w) { return w + (double)v; }
-__device__ __forceinline__ void lower(double f, double, double* dst) { *dst = f; }
-__device__ __forceinline__ void lower(double f, double w, float* dst) { *dst = (float)(f - w); }
+__device__ __forceinline__ void lower(double f, double, GAS double* dst) { *dst = f; }
+__device__ __forceinline__ void lower(double f, double w, GAS float* dst) { *dst = (float)(f - w); }
 
-template <typename T> __device__ __forceinline__ T* pops(const DevCase& D, int i);
-template <> __device__ __forceinline__ double* pops<double>(const DevCase& D, int i) { return D.f[i]; }
-template <> __device__ __forceinline__ float* pops<float>(const DevCase& D, int i) { return D.g[i]; }
+template <typename T> __device__ __forceinline__ GAS T* pops(const DevCase& D, int i);
+template <> __device__ __forceinline__ GAS double* pops<double>(const DevCase& D, int i) { return (GAS double*)D.f[i]; }
+template <> __device__ __forceinline__ GAS float* pops<float>(const DevCase& D, int i) { return (GAS float*)D.g[i]; }
 
 // Pull the nine populations of interior node (i,j) from prev
 // (streaming, solution.f90:438-454: f0(i,j,k) = f(i-ex,j-ey,k)), in FP64.
 template <typename T>
-__device__ __forceinline__ void pull9(const T* __restrict__ src, long long N, int nx, long long c,
+__device__ __forceinline__ void pull9(const GAS T* __restrict__ src, long long N, int nx, long long c,
                                       double* f, const double* W)
 {
     f[0] = lift(src[c], W[0]);
@@ -51,7 +56,7 @@ __device__ __forceinline__ void pull9(const T* __restrict__ src, long long N, in
 
 // Moments of a boundary-ring node from its stale f0 (never streamed,
 // solution.f90:516-520).
-__device__ __forceinline__ void ring_moments(const double* f0col, int ny, int j, double& r,
+__device__ __forceinline__ void ring_moments(const GAS double* f0col, int ny, int j, double& r,
                                              double& u, double& v)
 {
     double f[9];
@@ -97,13 +102,13 @@ __device__ __forceinline__ void right_char(double cs, double tr, double tu, doub
 // Edge state of column `col` (0: left, 1: right) at interior row j: the
 // characteristic-updated rho/u/v and the neighbour's f0 and moments.
 template <typename T>
-__device__ void edge_column_state(const DevCase& D, const T* src, double cs, int col, int j,
+__device__ __forceinline__ void edge_column_state(const DevCase& D, const GAS T* src, double cs, int col, int j,
                                   double& ro, double& uo, double& vo, double* f0n, double& rn,
                                   double& un, double& vn)
 {
     const int nx = D.nx;
     const long long N = D.N;
-    const double* f0c = col == 0 ? D.f0L : D.f0R;
+    const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
     double tr, tu, tv;
     ring_moments(f0c, D.ny, j, tr, tu, tv);
     int i2 = col == 0 ? 1 : nx - 2;
@@ -129,12 +134,12 @@ __device__ void edge_column_state(const DevCase& D, const T* src, double cs, int
 // recomputed here.  Reads only the previous populations and the stale ring
 // f0, so it runs beside the interior update.
 template <typename T>
-__device__ void ring_thread(const DevCase& D, int cur, double cs, double* __restrict__ macro, int t)
+__device__ __forceinline__ void ring_thread(const DevCase& D, int cur, double cs, double* __restrict__ macro, int t)
 {
     const int nx = D.nx, ny = D.ny;
     const long long N = D.N;
-    const T* src = pops<T>(D, cur ^ 1);
-    T* dst = pops<T>(D, cur);
+    const GAS T* src = pops<T>(D, cur ^ 1);
+    GAS T* dst = pops<T>(D, cur);
     if (t < ny - 2) {
         int j = t + 1;
         for (int col = 0; col < 2; ++col) {
@@ -161,7 +166,7 @@ __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __rest
             int col = x == 0 ? 0 : 1;
             double fdummy[9], r2, u2, v2;
             edge_column_state(D, src, cs, col, jn, rn, un, vn, fdummy, r2, u2, v2);
-            const double* f0c = col == 0 ? D.f0L : D.f0R;
+            const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
 #pragma unroll
             for (int k = 0; k < 9; ++k) f0n[k] = f0c[k * ny + jn];
         } else {
@@ -177,38 +182,27 @@ __device__ void ring_thread(const DevCase& D, int cur, double cs, double* __rest
     }
 }
 
-// One lattice update prev -> cur: fused streaming + physical_variables +
-// colission of every interior node (solution.f90:428-643) and the outer
-// boundary (boundary_condition1).  The first rblocks blocks take the ring,
-// the rest the interior rows in blocks that start at multiples of 256 nodes,
-// so every wave writes whole cache lines.  The IB nodes of the step (slot g)
-// are skipped: k_band collides them with their forcing, concurrently.
-// Everywhere else gforce is zero and the force moments vanish exactly.
-// Optionally records rho/u/v (download / residual support).
+// One interior node (i, j) of the lattice update prev -> cur: fused
+// streaming + physical_variables + colission (solution.f90:428-643).  The IB
+// nodes of the step (slot g) are skipped: k_band collides them with their
+// forcing, concurrently.  Everywhere else gforce is zero and the force
+// moments vanish exactly.  Optionally records rho/u/v (download / residual
+// support).
 template <typename T>
-__global__ __launch_bounds__(256) void k_lattice(DevCase D, int g, int cur, double cs,
-                                                 double* __restrict__ macro, int rblocks)
+__device__ __forceinline__ void lattice_node(const DevCase& D, int g, int cur, double* __restrict__ macro,
+                                             int i, int j)
 {
-    int b = blockIdx.x;
-    if (b < rblocks) {
-        ring_thread<T>(D, cur, cs, macro, b * blockDim.x + threadIdx.x);
-        return;
-    }
-    b -= rblocks;
     const int nx = D.nx;
     const long long N = D.N;
-    const int gx = (nx - 1 + 255) / 256;
-    const int i = (b % gx) * 256 + threadIdx.x;
-    const int j = b / gx + 1;
     if (i < 1 || i > nx - 2) return;
     const unsigned wi = (unsigned)(i - D.gi0), wj = (unsigned)(j - D.gj0);
     if (wi < (unsigned)D.gw && wj < (unsigned)D.gh) {
-        const int* nodeidx = D.ipool + (long long)g * D.io.total + D.io.nodeidx;
+        const GAS int* nodeidx = (const GAS int*)(D.ipool + (long long)g * D.io.total + D.io.nodeidx);
         if (nodeidx[(long long)wj * D.gw + wi] >= 0) return;
     }
     const long long c = (long long)j * nx + i;
-    const T* __restrict__ src = pops<T>(D, cur ^ 1);
-    T* __restrict__ dst = pops<T>(D, cur);
+    const GAS T* __restrict__ src = pops<T>(D, cur ^ 1);
+    GAS T* __restrict__ dst = pops<T>(D, cur);
     if constexpr (sizeof(T) == 8) {
         double f[9];
         pull9(src, N, nx, c, f, D.W);
@@ -246,11 +240,57 @@ __global__ __launch_bounds__(256) void k_lattice(DevCase D, int g, int cur, doub
         for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(g[k], dst + k * N + c);
     }
 }
-template __global__ void k_lattice<double>(DevCase, int, int, double, double*, int);
-template __global__ void k_lattice<float>(DevCase, int, int, double, double*, int);
 
-int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
-int lattice_blocks(int nx, int ny) { return lattice_ring_blocks(nx, ny) + (nx - 1 + 255) / 256 * (ny - 2); }
+// One lattice update of every case of the window: the interior nodes and
+// the outer boundary ring (boundary_condition1).  Work is cut into tiles;
+// tile (c, b) of case c is ring block b (256 ring threads) for
+// b < lattice_ring_blocks, otherwise interior row 1 + q / gx, columns
+// (q % gx) * 256 * NPT .. + 256 * NPT - 1 with q = b - ring blocks and
+// gx = ceil((nx - 1) / (256 * NPT)): tiles start at multiples of 256 nodes,
+// so every wave writes whole cache lines.  Thread t of a tile takes columns
+// t, t + 256, ..  A grid smaller than the tile count makes the kernel
+// persistent: block v takes tiles v, v + gridDim.x, ..  The order of the
+// tile table (case-major, or dealt to the accelerator's XCDs) is the
+// launch shape's choice; the result does not depend on it.
+template <typename T, int NPT>
+__global__ __launch_bounds__(256) void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                 const int2* __restrict__ tiles, int ntiles, int off)
+{
+    for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        // the tile's case and block are uniform over the workgroup: keep them
+        // (and so every field of the case) in scalar registers
+        const int2 tl = tiles[v];
+        const int cix = __builtin_amdgcn_readfirstlane(tl.x);
+        const int blk = __builtin_amdgcn_readfirstlane(tl.y);
+        const DevCase& D = C[cix];
+        const CaseClock kc = K[cix];
+        const int n = kc.n0 + off;
+        if (n > kc.n_end) continue;
+        const int cur = n & 1;
+        double* macro = n >= kc.flag_from ? D.macro[cur] : nullptr;
+        const int rb = lattice_ring_blocks(D.nx, D.ny);
+        if (blk < rb) {
+            ring_thread<T>(D, cur, D.cs, macro, blk * 256 + threadIdx.x);
+            continue;
+        }
+        const int q = blk - rb;
+        const int g = (n - 1) % D.nslot;
+        const int gx = (D.nx - 1 + 256 * NPT - 1) / (256 * NPT);
+        const int j = q / gx + 1;
+        const int i0 = (q % gx) * 256 * NPT + (int)threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) lattice_node<T>(D, g, cur, macro, i0 + r * 256, j);
+    }
+}
+template __global__ void k_lattice<double, 1>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_lattice<double, 2>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_lattice<float, 1>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_lattice<float, 2>(const DevCase*, const CaseClock*, const int2*, int, int);
+
+int lattice_tiles(int nx, int ny, int npt)
+{
+    return lattice_ring_blocks(nx, ny) + (nx - 1 + 256 * npt - 1) / (256 * npt) * (ny - 2);
+}
 
 // Box search of move_boundary (airfoil.f90:290-321) for one axis: the 1-based
 // index range of nodes with |(o + (i-1)*sc - X)/sc| <= 2.5.
@@ -273,9 +313,9 @@ __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, i
 // ---------------------------------------------------------------------------
 // immersed boundary
 //
-// Geometry of a batch of steps (one slot per step, blockIdx.y = step in the
-// batch), built ahead of time on its own stream because it depends only on
-// the prescribed kinematics:
+// Geometry of a batch of steps (one slot per step; grid z, or y for the
+// one-workgroup-per-slot kernels, is the case), built ahead of time on its
+// own stream because it depends only on the prescribed kinematics:
 //   k_geo_clear -> k_geo_markers -> k_geo_fill -> k_geo_phi -> k_geo_pairs ->
 //   k_geo_dsort -> k_geo_nsort -> k_geo_assemble -> k_geo_factor
 // Per-step solve, on the critical path beside the lattice kernel:
@@ -300,6 +340,28 @@ __device__ __forceinline__ bool box_cell(const int* b, int q, int& i, int& j)
     return true;
 }
 
+// The geometry batch that a launch at window offset `off` builds for a case:
+// GEO_FIRST builds batch 0 when the case takes step 1 at this offset,
+// GEO_NEXT the batch after the one starting at this offset.  Block s of the
+// batch works on step n0b + s (1-based); returns false when there is none.
+__device__ __forceinline__ bool geo_step(const DevCase& D, const CaseClock& k, int off, int mode, int s,
+                                         int& step0)
+{
+    const int n = k.n0 + off;
+    if (n > k.n_end || (n - 1) % D.batch != 0) return false;
+    int j = (n - 1) / D.batch;
+    if (mode == GEO_FIRST) {
+        if (n != 1) return false;
+    } else {
+        j += 1;
+    }
+    const int n0b = j * D.batch + 1;
+    const int nb = min(D.batch, D.steps_cap - n0b + 1);
+    if (s >= nb) return false;
+    step0 = n0b - 1 + s;  // 0-based step index
+    return true;
+}
+
 // window index of 1-based lattice node (i, j)
 __device__ __forceinline__ long long wcell(const DevCase& D, int i, int j)
 {
@@ -307,14 +369,19 @@ __device__ __forceinline__ long long wcell(const DevCase& D, int i, int j)
 }
 
 // Clear the slot's previous node set (airfoil.f90:276-283).
-__global__ __launch_bounds__(256) void k_geo_clear(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_clear(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                   int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *S.nadj) return;
-    long long c = (long long)(S.adj[3 * a + 1] - D.gj0) * D.gw + (S.adj[3 * a] - D.gi0);
-    S.claim[c] = INT_MAX;
-    S.nodeidx[c] = -1;
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
+    const int na = *S.nadj;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x) {
+        long long c = (long long)(S.adj[3 * a + 1] - D.gj0) * D.gw + (S.adj[3 * a] - D.gi0);
+        S.claim[c] = INT_MAX;
+        S.nodeidx[c] = -1;
+    }
 }
 
 // Markers, marker velocities and boxes (move_boundary airfoil.f90:256-321,
@@ -322,10 +389,13 @@ __global__ __launch_bounds__(256) void k_geo_clear(DevCase D, int g0)
 // the smallest segment index wins, which is the segment that claims it first
 // in the reference's serial loop.  sin/cos of the pitch angle come
 // precomputed from the host's libm.
-__global__ __launch_bounds__(256) void k_geo_markers(DevCase D, int g0, int step0)
+__global__ __launch_bounds__(256) void k_geo_markers(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                     int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
-    const int step = step0 + blockIdx.y;
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= D.nseg) return;
     int w = s / D.spw;
@@ -364,9 +434,13 @@ __global__ __launch_bounds__(256) void k_geo_markers(DevCase D, int g0, int step
 // Ordered compaction: the node list in the reference's order (segment-major,
 // then i, then j; airfoil.f90:322-331), the node -> list index grid, the node
 // count, and cleared marker-list counters.  One workgroup per slot.
-__global__ __launch_bounds__(1024) void k_geo_fill(DevCase D, int g0)
+__global__ __launch_bounds__(1024) void k_geo_fill(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                  int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
+    const DevCase& D = C[blockIdx.y];
+    int step;
+    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     __shared__ int part[1024];
     __shared__ int s_total;
     const int ns = D.nseg, T = blockDim.x, tid = threadIdx.x;
@@ -428,9 +502,13 @@ __global__ __launch_bounds__(1024) void k_geo_fill(DevCase D, int g0)
 
 // One-axis kernel values of every box column (q < 6) and row (q >= 6)
 // (boundary_condition.f90:1581-1600).
-__global__ __launch_bounds__(256) void k_geo_phi(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_phi(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                              int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int s = t / 12, q = t % 12;
     if (s >= D.nseg) return;
@@ -451,9 +529,13 @@ __global__ __launch_bounds__(256) void k_geo_phi(DevCase D, int g0)
 
 // Entry xx1(s, node) = kernel1*kernel2 (boundary_condition.f90:1602) of every
 // box node, in box order, and registration of s in the node's marker list.
-__global__ __launch_bounds__(256) void k_geo_pairs(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_pairs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                              int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int s = t / BOXMAX, q = t % BOXMAX;
     if (s >= D.nseg) return;
@@ -476,9 +558,13 @@ __global__ __launch_bounds__(256) void k_geo_pairs(DevCase D, int g0)
 }
 
 // A row of xx1 in ascending node order: each entry moves to its rank.
-__global__ __launch_bounds__(256) void k_geo_dsort(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_dsort(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                              int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int s = t / BOXMAX, q = t % BOXMAX;
     if (s >= D.nseg) return;
@@ -495,9 +581,13 @@ __global__ __launch_bounds__(256) void k_geo_dsort(DevCase D, int g0)
 
 // Each node's marker list in ascending segment order (the order of the
 // spread sum); grid-stride over (node, entry).
-__global__ __launch_bounds__(256) void k_geo_nsort(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_nsort(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                              int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     const long long total = (long long)(*S.nadj) * NCAP;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -514,9 +604,13 @@ __global__ __launch_bounds__(256) void k_geo_nsort(DevCase D, int g0)
 
 // Band of G = D D^T in zig-zag order: G(p, p-d) is the sum over the nodes
 // shared by the boxes of segments perm[p] and perm[p-d], ascending list index.
-__global__ __launch_bounds__(256) void k_geo_assemble(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_assemble(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                              int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.y) % D.nslot);
+    const DevCase& D = C[blockIdx.z];
+    int step;
+    if (!geo_step(D, K[blockIdx.z], off, mode, blockIdx.y, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     const int W = D.bw + 1;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)D.nseg * W) return;
@@ -560,9 +654,13 @@ constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
 //   forward  [h] = {l(k+e1,k), l(k+e2,k)}, e1 = h-q (h > q), e2 = ws+h-q (h <= q)
 //   backward [h] = {l(k,k-e1), l(k,k-e2)}, e1 = q-h (h < q), e2 = ws+q-h (h >= q)
 // with q = k mod ws, zero outside the band.  One workgroup per slot.
-__global__ __launch_bounds__(256) void k_geo_factor(DevCase D, int g0)
+__global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                    int off, int mode)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
+    const DevCase& D = C[blockIdx.y];
+    int step;
+    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
     extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
     const int n = D.nseg, bw = D.bw, W = bw + 1, ws = D.ws;
@@ -671,10 +769,15 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 // segment's products are then summed in ascending node order.
 constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
 template <typename T>
-__global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
+__global__ __launch_bounds__(256) void k_crit_rhs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                  int off)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
-    const Slot S = slot_of(D, g);
+    const DevCase& D = C[blockIdx.y];
+    const int nstep = K[blockIdx.y].n0 + off;
+    if (nstep > K[blockIdx.y].n_end || (int)blockIdx.x * RHS_SEG >= D.nseg) return;
+    const int cur = nstep & 1;
+    const Slot S = slot_of(D, (nstep - 1) % D.nslot);
     __shared__ double pu[RHS_SEG][BOXMAX], pv[RHS_SEG][BOXMAX];
     const int sl = threadIdx.x / BOXMAX, q = threadIdx.x % BOXMAX;
     const int s = blockIdx.x * RHS_SEG + sl;
@@ -699,8 +802,8 @@ __global__ __launch_bounds__(256) void k_crit_rhs(DevCase D, int g, int cur)
         D.rhs[D.nseg + p] = S.vb[s] - sv;
     }
 }
-template __global__ void k_crit_rhs<double>(DevCase, int, int);
-template __global__ void k_crit_rhs<float>(DevCase, int, int);
+template __global__ void k_crit_rhs<double>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_rhs<float>(const DevCase*, const CaseClock*, int);
 
 // Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
 // right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
@@ -752,13 +855,17 @@ __device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane
 }
 
 template <int U>
-__global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(DevCase D, int g)
+__global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCase* __restrict__ C,
+                                                                     const CaseClock* __restrict__ K, int off)
 {
     constexpr int UPC = 64 / U;              // units per lane chunk
     constexpr int SPW = U / SV_NPROD;        // steps per producer and unit
     __shared__ double2 ring[2 * U * 64];     // two units
     extern __shared__ double zb[];           // [2][npad] values, then [npad] 1/d
-    const Slot S = slot_of(D, g);
+    const DevCase& D = C[blockIdx.x];
+    const int nstep = K[blockIdx.x].n0 + off;
+    if (nstep > K[blockIdx.x].n_end) return;
+    const Slot S = slot_of(D, (nstep - 1) % D.nslot);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = D.nseg, np = D.npad, nch = np / 64, nun = 2 * nch * UPC;
     double* dv = zb + 2 * np;
@@ -860,8 +967,8 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(DevCase D, i
         D.y[n + s] = zb[np + p];
     }
 }
-template __global__ void k_crit_solve<64>(DevCase, int);
-template __global__ void k_crit_solve<32>(DevCase, int);
+template __global__ void k_crit_solve<64>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve<32>(const DevCase*, const CaseClock*, int);
 
 // The 64-step staging unit when its two LDS buffers (128 KB) fit beside the
 // right-hand sides, the 32-step one otherwise.
@@ -877,12 +984,8 @@ size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
 // (solution.f90:598-638) of the IB nodes, which k_lattice leaves out;
 // the two run concurrently.
 template <typename T>
-__global__ __launch_bounds__(256) void k_band(DevCase D, int g, int cur, double* __restrict__ macro)
+__device__ __forceinline__ void band_node(const DevCase& D, const Slot& S, int a, int cur, double* __restrict__ macro)
 {
-    __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
-    const Slot S = slot_of(D, g);
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= *S.nadj) return;
     const int cnt = min(S.nmkcnt[a], NCAP);
     const int* km = S.nmk + (long long)a * NCAP;
     const double* wm = S.nmkv + (long long)a * NCAP;
@@ -908,25 +1011,44 @@ __global__ __launch_bounds__(256) void k_band(DevCase D, int g, int cur, double*
     double gfy = 2.0 * r * gy;
     bool forced = (gfx != 0.0) || (gfy != 0.0);
     collide_node(f, r, u, v, D.R, forced, gfx, gfy);
-    T* dst = pops<T>(D, cur);
+    GAS T* dst = pops<T>(D, cur);
 #pragma unroll
     for (int k = 0; k < 9; ++k) lower(f[k], D.W[k], dst + k * D.N + c);
     if (macro) { macro[c] = r; macro[D.N + c] = u; macro[2 * D.N + c] = v; }
 }
-template __global__ void k_band<double>(DevCase, int, int, double*);
-template __global__ void k_band<float>(DevCase, int, int, double*);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_band(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off)
+{
+    __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
+    const DevCase& D = C[blockIdx.y];
+    const CaseClock kc = K[blockIdx.y];
+    const int n = kc.n0 + off;
+    if (n > kc.n_end) return;
+    const int cur = n & 1;
+    double* macro = n >= kc.flag_from ? D.macro[cur] : nullptr;
+    const Slot S = slot_of(D, (n - 1) % D.nslot);
+    const int na = *S.nadj;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x)
+        band_node<T>(D, S, a, cur, macro);
+}
+template __global__ void k_band<double>(const DevCase*, const CaseClock*, int);
+template __global__ void k_band<float>(const DevCase*, const CaseClock*, int);
 
 // Per-wing force and moment, accumulated in node order
 // (boundary_condition.f90:1662-1675), and their normalisation
 // (solution.f90:327-330).  One workgroup per step of a batch: 256 threads
 // form the per-node terms of a chunk into LDS, one thread advances the
 // running sums node by node.  The nodes of a wing are contiguous in the list
-// (segment-major order).
+// (segment-major order).  Block (x, c) takes step n0 + off + x of case c.
 constexpr int FCH = 1024;
-__global__ __launch_bounds__(256) void k_forces(DevCase D, int g0, int step0)
+__global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off)
 {
-    const Slot S = slot_of(D, (g0 + blockIdx.x) % D.nslot);
-    const int step = step0 + blockIdx.x;
+    const DevCase& D = C[blockIdx.y];
+    const int n = K[blockIdx.y].n0 + off + (int)blockIdx.x;
+    if (n > K[blockIdx.y].n_end) return;
+    const int step = n - 1;  // 0-based
+    const Slot S = slot_of(D, step % D.nslot);
     __shared__ double terms[FCH][4];
     __shared__ int wing[FCH];
     const int tid = threadIdx.x;
@@ -981,6 +1103,15 @@ __global__ __launch_bounds__(256) void k_forces(DevCase D, int g0, int step0)
         flush(curw);
         for (int x = curw + 1; x < D.nw; ++x) { F0 = F1 = M = 0.0; flush(x); }
     }
+}
+
+// Moves every case's window forward by `steps` (the last node of a captured
+// window graph, so that its next replay takes the following steps); n_end
+// and flag_from are absolute steps and stay.
+__global__ void k_clock_advance(CaseClock* __restrict__ K, int ncases, int steps)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncases) K[c].n0 += steps;
 }
 
 // erranalysis partial sums over the interior (solution.f90:102-109): per

@@ -17,6 +17,9 @@ output / saveflowfield ``fields()`` -> ``lbmib_download``
 readflowfield          ``upload(f)`` -> ``lbmib_upload_state``
 =====================  ==========================================================
 
+:class:`Batch` groups cases that then advance together (``lbmib_batch_*``;
+the reference has no counterpart: it runs one case per process).
+
 Errors raise :class:`LBMIBError` carrying the library's code and message
 (the reference stops the program instead).  The library is loaded from this
 package directory; if it is missing the import of this module raises, there is
@@ -53,6 +56,21 @@ class Params(C.Structure):
                 ("nan_guard", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
+                ("ib_cus", C.c_int32), ("graph", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph")}
+
+
+def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0) -> LaunchShape:
+    s = LaunchShape()
+    s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
+    s.ib_cus, s.graph = ib_cus, graph
+    return s
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lib = None
@@ -60,7 +78,11 @@ _lib = None
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
            "lbmib_upload_state", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
-           "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version"]
+           "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
+           "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
+           "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
+           "lbmib_batch_get_shape", "lbmib_batch_set_profiling", "lbmib_batch_get_timings",
+           "lbmib_batch_autotune", "lbmib_batch_destroy"]
 
 
 def load(path: str = LIB_PATH):
@@ -86,6 +108,20 @@ def load(path: str = LIB_PATH):
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
     L.lbmib_get_timings.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_destroy.argtypes = [vp]
+    sp = C.POINTER(LaunchShape)
+    L.lbmib_set_shape.argtypes = [vp, sp]
+    L.lbmib_get_shape.argtypes = [vp, sp]
+    L.lbmib_batch_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    L.lbmib_batch_add.argtypes = [vp, vp]
+    L.lbmib_batch_remove.argtypes = [vp, vp]
+    L.lbmib_batch_size.argtypes = [vp, _ip]
+    L.lbmib_batch_step.argtypes = [vp, C.c_int32]
+    L.lbmib_batch_set_shape.argtypes = [vp, sp]
+    L.lbmib_batch_get_shape.argtypes = [vp, sp]
+    L.lbmib_batch_set_profiling.argtypes = [vp, C.c_int32]
+    L.lbmib_batch_get_timings.argtypes = [vp, _dp, C.c_int32]
+    L.lbmib_batch_autotune.argtypes = [vp, C.c_int32, sp, C.c_char_p, C.c_int32]
+    L.lbmib_batch_destroy.argtypes = [vp]
     L.lbmib_last_error.restype = C.c_char_p
     L.lbmib_version.restype = C.c_char_p
     _lib = L
@@ -222,10 +258,97 @@ class LBMIB:
         """Averages of the recorded phases (milliseconds) and their counts."""
         t = np.zeros(NTIMINGS)
         _check(load().lbmib_get_timings(self.h, _d(t), NTIMINGS))
-        ns, nb = max(t[2], 1.0), max(t[8], 1.0)
-        return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / ns, ib_solve_ms=t[4] / ns,
-                    ib_collide_ms=t[5] / ns, geometry_batch_ms=t[6] / nb, factor_batch_ms=t[7] / nb,
-                    steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]))
+        return _timings(t)
+
+    def set_shape(self, **kw):
+        """Launch shape of this handle's steps (see :func:`make_shape`)."""
+        _check(load().lbmib_set_shape(self.h, C.byref(make_shape(**kw))))
+
+    def shape(self) -> dict:
+        s = LaunchShape()
+        _check(load().lbmib_get_shape(self.h, C.byref(s)))
+        return s.as_dict()
+
+
+def _timings(t) -> dict:
+    ns, nb = max(t[2], 1.0), max(t[8], 1.0)
+    return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / ns, ib_solve_ms=t[4] / ns,
+                ib_collide_ms=t[5] / ns, geometry_batch_ms=t[6] / nb, factor_batch_ms=t[7] / nb,
+                steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]))
+
+
+class Batch:
+    """Cases advanced together on one GPU (``lbmib_batch_*``).
+
+    Members keep their :class:`LBMIB` handles for inputs and results; while
+    in the batch they step only through :meth:`step`.
+    """
+
+    def __init__(self, device: int = 0):
+        L = load()
+        b = C.c_void_p()
+        _check(L.lbmib_batch_create(int(device), C.byref(b)))
+        self.b = b
+        self.members = []
+
+    def add(self, sim: "LBMIB"):
+        _check(load().lbmib_batch_add(self.b, sim.h))
+        self.members.append(sim)
+
+    def remove(self, sim: "LBMIB"):
+        _check(load().lbmib_batch_remove(self.b, sim.h))
+        self.members.remove(sim)
+
+    def __len__(self):
+        n = C.c_int32()
+        _check(load().lbmib_batch_size(self.b, C.byref(n)))
+        return n.value
+
+    def step(self, n_steps: int):
+        _check(load().lbmib_batch_step(self.b, int(n_steps)))
+
+    def set_shape(self, **kw):
+        _check(load().lbmib_batch_set_shape(self.b, C.byref(make_shape(**kw))))
+
+    def shape(self) -> dict:
+        s = LaunchShape()
+        _check(load().lbmib_batch_get_shape(self.b, C.byref(s)))
+        return s.as_dict()
+
+    def set_profiling(self, every: int = 1):
+        _check(load().lbmib_batch_set_profiling(self.b, int(every)))
+
+    def timings(self) -> dict:
+        t = np.zeros(NTIMINGS)
+        _check(load().lbmib_batch_get_timings(self.b, _d(t), NTIMINGS))
+        return _timings(t)
+
+    def autotune(self, trial_steps: int = 32) -> dict:
+        """Pick the fastest launch shape on copies of the members; returns the
+        trials and the choice (applied to this batch)."""
+        import json
+        best = LaunchShape()
+        buf = C.create_string_buffer(1 << 16)
+        _check(load().lbmib_batch_autotune(self.b, int(trial_steps), C.byref(best), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "b", None):
+            load().lbmib_batch_destroy(self.b)
+            self.b = None
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version() -> str:

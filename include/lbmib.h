@@ -140,8 +140,75 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
 int lbmib_set_profiling(lbmib_handle* h, int32_t every);
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 
-/* Releases the handle and its device memory. */
+/* Releases the handle and its device memory (removing it from its batch). */
 int lbmib_destroy(lbmib_handle* h);
+
+/* ---- launch shape and the batched form ----
+ *
+ * The reference runs one case per process and has no counterpart of the
+ * entry points below; they serve parameter sweeps (many kinematics cases of
+ * uneven lattice sizes advanced together on one GPU) and a serving loop that
+ * admits and retires cases while others run.  A case keeps its own handle:
+ * its inputs, state, history, download and residual work as above.  While it
+ * belongs to a batch it is stepped only by lbmib_batch_step (lbmib_step
+ * returns LBMIB_E_STATE).  Every kernel of a batch step covers all member
+ * cases in one launch, each at its own step; the per-case arithmetic is the
+ * single-case path's, so a case's results are identical alone and in a
+ * batch. */
+
+/* How the steps are launched (never changes results). */
+typedef struct lbmib_launch_shape {
+    int32_t nodes_per_thread; /* lattice tiles of 256 x nodes_per_thread nodes of one row: 1 or 2 */
+    int32_t tile_order;       /* 0: case-major tile table; 1: tiles dealt to the 8 XCDs in row bands */
+    int32_t persistent;       /* 0: one workgroup per tile; k > 0: k workgroups per CU looping over tiles */
+    int32_t ib_cus;           /* CUs reserved for the triangular solves (multiple of 8; 0: none) */
+    int32_t graph;            /* 1: windows of 16 steps captured once as a HIP graph and replayed */
+    int32_t reserved[3];
+} lbmib_launch_shape;
+
+/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0; the
+ * environment variable LBMIB_IB_CUS overrides ib_cus). */
+int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
+int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);
+
+typedef struct lbmib_batch lbmib_batch;
+
+/* An empty batch on a device (its own HIP streams). */
+int lbmib_batch_create(int32_t device, lbmib_batch** out);
+
+/* Admits a case (motion table set, same device and precision as the other
+ * members, and at the same step modulo 8 -- a geometry batch -- as them,
+ * e.g. a fresh case beside members at multiples of 8 steps). */
+int lbmib_batch_add(lbmib_batch* b, lbmib_handle* h);
+
+/* Retires a case from the batch (it may then step alone again). */
+int lbmib_batch_remove(lbmib_batch* b, lbmib_handle* h);
+
+int lbmib_batch_size(const lbmib_batch* b, int32_t* n);
+
+/* Advances every member min(n_steps, its remaining motion-table steps)
+ * steps; synchronous.  Members that run out of steps idle. */
+int lbmib_batch_step(lbmib_batch* b, int32_t n_steps);
+
+int lbmib_batch_set_shape(lbmib_batch* b, const lbmib_launch_shape* s);
+int lbmib_batch_get_shape(const lbmib_batch* b, lbmib_launch_shape* s);
+
+/* Phase timings of the batch's steps, as lbmib_set_profiling /
+ * lbmib_get_timings for a handle (graph replays are not sampled). */
+int lbmib_batch_set_profiling(lbmib_batch* b, int32_t every);
+int lbmib_batch_get_timings(const lbmib_batch* b, double* t, int32_t n);
+
+/* Autotuner: times launch shapes on copies of the members (same lattices,
+ * markers and motion tables, own state; the members are not advanced), one
+ * coordinate at a time -- lattice tile shape, order and persistent grid, then
+ * the CUs reserved for the solves, then graph replay -- over trial_steps
+ * steps each, applies the fastest to the batch and returns it in *best.
+ * report (may be NULL) receives every trial as JSON. */
+int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape* best, char* report,
+                         int32_t report_len);
+
+/* Releases the batch; its members remain valid handles. */
+int lbmib_batch_destroy(lbmib_batch* b);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char* lbmib_last_error(void);

@@ -1,0 +1,206 @@
+"""The batched form: several cases of uneven lattice sizes advanced together,
+every kernel of a step covering all of them in one launch.
+
+The per-case arithmetic of a batch is the single-case path's, so the bar is
+bit-for-bit equality:
+  * FP64 members against the CPU oracle (as the single-case parity tests);
+  * every member against the same case run alone, FP64 and FP32;
+  * a case that joins a running batch, and one that leaves it, against the
+    same cases alone (continuous batching);
+  * every launch shape (tile width, tile order, persistent grid, reserved CUs,
+    graph replay) against the default one.
+"""
+import numpy as np
+import pytest
+
+from fortran_loadfile_mitiwing_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_cases(n_steps):
+    """Three cases of different lattice sizes, marker counts and kinematics."""
+    return [
+        cases.config(0, n_steps=n_steps, scale_down=2),                   # 200x100, 64 markers, plunge
+        cases.config(1, n_steps=n_steps, scale_down=16),                  # 256x128, 64 markers, pitch+plunge
+        cases.make_case(160, 80, 16.0, 48, 300.0, n_steps, heave=0.4, pitch_deg=20.0, phase_deg=60.0,
+                        strouhal=0.35, name="mini_160x80"),
+    ]
+
+
+def same(a, b):
+    fa, fb = a.fields(), b.fields()
+    for k in ("rho", "u", "v", "f"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert np.array_equal(a.history(), b.history())
+
+
+def alone(lbmib_mod, case, steps, prec, chunks=None):
+    g = lbmib_mod.LBMIB(case, precision=prec)
+    for c in chunks or [steps]:
+        g.evolution(c)
+    return g
+
+
+def test_fp64_batch_matches_the_oracle(oracle_mod, lbmib_mod):
+    cs = mixed_cases(48)
+    sims = [lbmib_mod.LBMIB(c) for c in cs]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    for chunk in (16, 13, 19):
+        print("  batch of %d: %d steps" % (len(b), chunk), flush=True)
+        b.step(chunk)
+    for c, s in zip(cs, sims):
+        o = oracle_mod.Oracle(c, n_threads=8)
+        o.step(48)
+        fo, fg = o.fields(), s.fields()
+        for k in ("rho", "u", "v", "f"):
+            assert np.array_equal(fg[k], fo[k]), (c.name, k)
+        assert np.array_equal(s.history(), o.history()), c.name
+        assert s.steps_done == 48
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_alone_equals_mixed(lbmib_mod, prec):
+    cs = mixed_cases(40)
+    sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    b.step(40)
+    for c, s in zip(cs, sims):
+        same(s, alone(lbmib_mod, c, 40, prec, chunks=[7, 33]))
+
+
+def test_join_and_leave(lbmib_mod):
+    """A joins an empty batch, B joins after 16 steps, A leaves after 32 and
+    steps on alone; B leaves after 24 of its own steps."""
+    ca, cb = mixed_cases(64)[:2]
+    prec = 1
+    a = lbmib_mod.LBMIB(ca, precision=prec)
+    bb = lbmib_mod.LBMIB(cb, precision=prec)
+    b = lbmib_mod.Batch()
+    b.add(a)
+    b.step(16)
+    b.add(bb)
+    b.step(16)
+    b.remove(a)
+    a.evolution(8)
+    b.step(8)
+    b.remove(bb)
+    assert a.steps_done == 40 and bb.steps_done == 24
+    same(a, alone(lbmib_mod, ca, 40, prec))
+    same(bb, alone(lbmib_mod, cb, 24, prec))
+
+
+def test_members_with_shorter_tables_idle(lbmib_mod):
+    cs = mixed_cases(60)
+    short = cases.make_case(160, 80, 16.0, 48, 300.0, 20, heave=0.4, name="short")
+    prec = 1
+    sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs[:2]] + [lbmib_mod.LBMIB(short, precision=prec)]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    b.step(32)
+    assert [s.steps_done for s in sims] == [32, 32, 20]
+    same(sims[2], alone(lbmib_mod, short, 20, prec))
+    same(sims[0], alone(lbmib_mod, cs[0], 32, prec))
+
+
+GRAPH_OFF = ("graph replay is switched off: its first GPU run (this three-case batch replaying captured "
+             "16-step windows) ended the host process with a segmentation fault of unknown cause")
+SHAPES = [
+    dict(nodes_per_thread=2),
+    dict(tile_order=1),
+    dict(persistent=4),
+    dict(nodes_per_thread=2, tile_order=1, persistent=2, ib_cus=0),
+    dict(ib_cus=16),
+    pytest.param(dict(graph=1), marks=pytest.mark.skip(reason=GRAPH_OFF)),
+    pytest.param(dict(graph=1, nodes_per_thread=2, tile_order=1, persistent=8), marks=pytest.mark.skip(reason=GRAPH_OFF)),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: ",".join("%s=%d" % kv for kv in s.items()))
+def test_launch_shapes_do_not_change_results(lbmib_mod, shape):
+    cs = mixed_cases(56)
+    prec = 0
+    ref = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    rb = lbmib_mod.Batch()
+    for s in ref:
+        rb.add(s)
+    rb.step(56)
+    sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib_mod.Batch()
+    b.set_shape(**shape)
+    for s in sims:
+        b.add(s)
+    print("  shape %s" % b.shape(), flush=True)
+    b.step(5)   # partial batch, then graph-sized windows
+    b.step(51)
+    for r, s in zip(ref, sims):
+        same(s, r)
+
+
+def test_graph_replay_is_refused_while_switched_off(lbmib_mod):
+    g = lbmib_mod.LBMIB(cases.config(0, n_steps=20, scale_down=4))
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        g.set_shape(graph=1)
+    assert e.value.code == -1
+
+
+@pytest.mark.skip(reason=GRAPH_OFF)
+def test_graph_replay_single_handle_matches_the_oracle(oracle_mod, lbmib_mod):
+    case = cases.config(0, n_steps=70, scale_down=2)
+    g = lbmib_mod.LBMIB(case)
+    g.set_shape(graph=1)
+    g.evolution(3)
+    g.evolution(67)
+    o = oracle_mod.Oracle(case, n_threads=8)
+    o.step(70)
+    fo, fg = o.fields(), g.fields()
+    for k in ("rho", "u", "v", "f"):
+        assert np.array_equal(fg[k], fo[k]), k
+    assert np.array_equal(g.history(), o.history())
+
+
+def test_autotune_picks_a_shape_and_keeps_results(lbmib_mod):
+    cs = mixed_cases(1200)  # 19 trials of 16 + 32 steps on the copies
+    prec = 1
+    sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    rep = b.autotune(trial_steps=16)
+    print("  autotune best:", rep["best"], "trials:", len(rep["trials"]), flush=True)
+    assert len(rep["trials"]) >= 16 + 3
+    assert rep["best"]["ms_per_step"] > 0
+    assert all(s.steps_done == 0 for s in sims)  # the tuner ran on copies
+    b.step(40)
+    for c, s in zip(cs, sims):
+        same(s, alone(lbmib_mod, c, 40, prec))
+
+
+def test_batch_errors(lbmib_mod):
+    cs = mixed_cases(40)
+    a = lbmib_mod.LBMIB(cs[0])
+    c = lbmib_mod.LBMIB(cs[1])
+    f32 = lbmib_mod.LBMIB(cs[2], precision=1)
+    b = lbmib_mod.Batch()
+    b.add(a)
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        b.add(a)
+    assert e.value.code == -6
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        a.evolution(1)
+    assert e.value.code == -6
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        b.add(f32)
+    assert e.value.code == -1
+    b.step(3)
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        b.add(c)  # stands at step 0, the batch at 3
+    assert e.value.code == -6
+    b.remove(a)
+    a.evolution(1)
+    assert a.steps_done == 4
