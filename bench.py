@@ -22,6 +22,14 @@ autotuned launch shape, against the same cases run one after another on the
 plain FP64 path.  value = cases per hour of the batch; the JSON also holds
 the sequential rate, and per-case C_L / C_D errors of the batch against the
 FP64 runs.
+
+--form serve: the serving loop.  Requests (cases drawn as for the sweep with
+nx up to 4096 and 2000..20000 steps, log-uniform) arrive as a Poisson process
+at the rate that offers 80% of the device's capacity (measured first on a
+probe batch of the same distribution); they are admitted by HBM footprint into
+one running FP32-deviation batch (continuous batching) and retire when done.
+value = aggregate MLUPS; the JSON holds p50/p99 time-to-result and a sample of
+results checked against FP64 runs of the same cases.
 """
 import argparse
 import json
@@ -43,7 +51,7 @@ WORKLOADS = {
        "(90deg lead), St 0.3, U 0.05",
     2: "8192x4096 lattice, NACA0012 chord 512 lu, 2048 markers, Re 2000, plunge 0.5c, St 0.3, U 0.05",
 }
-PROFILE_EVERY = 16
+PROFILE_EVERY = 17  # coprime with the 8-step geometry batch: samples every phase of it
 # HBM bytes per k_lattice_f64 launch on the 4096x2048 workload, from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
 # device copy measured the same way (the lattice traffic summary in profiles/)
@@ -72,15 +80,22 @@ def cpu_baseline(case, seconds_budget=20.0, threads=None):
 
 def coefficient_errors(h32, h64, skip_frac=0.5):
     """Errors of an FP32 force history against the FP64 one over the run's
-    last (1 - skip_frac): mean C_D relative, mean C_L relative to the C_L
-    r.m.s. (a symmetric section's mean C_L is near zero), peak |C_L|."""
+    last (1 - skip_frac).  Mean C_L is taken relative to the C_L r.m.s. (a
+    symmetric section's mean C_L is near zero); mean C_D both relative to
+    |mean C_D| and relative to the C_D r.m.s. (a flapping section that makes
+    thrust has a mean C_D near zero, where the first is ill-conditioned);
+    peak |C_L| relative."""
     k = int(len(h64) * skip_frac)
     cd32, cd64 = h32[k:, 1], h64[k:, 1]
     cl32, cl64 = h32[k:, 2], h64[k:, 2]
-    rms = float(np.sqrt(np.mean(cl64 ** 2)))
-    return {"mean_CD": float(abs(cd32.mean() - cd64.mean()) / abs(cd64.mean())),
-            "mean_CL": float(abs(cl32.mean() - cl64.mean()) / rms),
-            "peak_CL": float(abs(np.abs(cl32).max() - np.abs(cl64).max()) / np.abs(cl64).max())}
+    rms_l = float(np.sqrt(np.mean(cl64 ** 2)))
+    rms_d = float(np.sqrt(np.mean(cd64 ** 2)))
+    dcd = abs(cd32.mean() - cd64.mean())
+    return {"mean_CD_rel": float(dcd / abs(cd64.mean())),
+            "mean_CD": float(dcd / rms_d),
+            "mean_CL": float(abs(cl32.mean() - cl64.mean()) / rms_l),
+            "peak_CL": float(abs(np.abs(cl32).max() - np.abs(cl64).max()) / np.abs(cl64).max()),
+            "_mean_CD64": float(cd64.mean()), "_rms_CD64": rms_d}
 
 
 def run_batch(cs, prec, steps, tune, chunk=1000, label=""):
@@ -116,26 +131,46 @@ def bench_batch(args):
     nodes = sum(c.params.nx * c.params.ny for c in cs)
     interior = sum((c.params.nx - 2) * (c.params.ny - 2) for c in cs)
     print("[batch] %d cases, %d lattice nodes, %d steps each" % (len(cs), nodes, steps), file=sys.stderr, flush=True)
+    if args.only_batched:
+        t_batch, h32, tm, shape = run_batch(cs, lbmib.PREC_FP32DEV, steps, not args.no_tune, label="fp32")
+        print(json.dumps({"form": "batch (batched FP32 only)", "ms_per_step": round(t_batch / steps * 1e3, 4),
+                          "phases_ms": tm, "launch_shape": shape}))
+        return
+    # sequential plain FP64 runs, each case alone on one reused engine (a
+    # batch of one: the plain path's launches): setup + 20 untimed + 200 timed
+    # steps per case, the run time taken as setup + steps x (ms per timed step)
+    t_seq, seq_w, seq_k = 0.0, 20, 200
+    seq = lbmib.Batch()
+    for i, c in enumerate(cs):
+        t0 = time.perf_counter()
+        g = lbmib.LBMIB(c)
+        seq.add(g)
+        t_setup = time.perf_counter() - t0
+        seq.step(seq_w)
+        t0 = time.perf_counter()
+        seq.step(seq_k)
+        t_seq += t_setup + (time.perf_counter() - t0) / seq_k * steps
+        seq.remove(g)
+        g.close()
+        if i % 8 == 7:
+            print("[batch] sequential FP64 timing: %d cases, %.1f s projected" % (i + 1, t_seq),
+                  file=sys.stderr, flush=True)
+    seq.close()
     t_batch, h32, tm, shape = run_batch(cs, lbmib.PREC_FP32DEV, steps, not args.no_tune, label="fp32")
     # FP64 reference histories of the same cases (the batch's per-case
     # arithmetic is the single-case path's, bit for bit)
     t_b64, h64, _, _ = run_batch(cs, lbmib.PREC_FP64, steps, False, label="fp64")
-    errs = [coefficient_errors(a, b_) for a, b_ in zip(h32, h64)]
-    # sequential plain FP64 runs: setup + 20 untimed + 200 timed steps per
-    # case, the run time taken as setup + steps x (ms per timed step)
-    t_seq, seq_w, seq_k = 0.0, 20, 200
-    for i, c in enumerate(cs):
-        t0 = time.perf_counter()
-        g = lbmib.LBMIB(c)
-        t_setup = time.perf_counter() - t0
-        g.evolution(seq_w)
-        t0 = time.perf_counter()
-        g.evolution(seq_k)
-        t_seq += t_setup + (time.perf_counter() - t0) / seq_k * steps
-        g.close()
-        if i % 16 == 15:
-            print("[batch] sequential FP64 timing: %d cases" % (i + 1), file=sys.stderr, flush=True)
-    worst = {k: max(e[k] for e in errs) for k in errs[0]}
+    finite = [(bool(np.all(np.isfinite(a[:, 1:3]))), bool(np.all(np.isfinite(b_[:, 1:3])))) for a, b_ in zip(h32, h64)]
+    diverged = [{"case": c.name, "fp32_finite": f[0], "fp64_finite": f[1]} for c, f in zip(cs, finite) if not all(f)]
+    keep = [i for i, f in enumerate(finite) if all(f)]
+    errs = [coefficient_errors(h32[i], h64[i]) for i in keep]
+    cs_ok = [cs[i] for i in keep]
+    keys = ("mean_CD", "mean_CL", "peak_CL")
+    worst = {k: max(e[k] for e in errs) for k in keys + ("mean_CD_rel",)}
+    order = sorted(range(len(errs)), key=lambda i: -errs[i]["mean_CD_rel"])[:5]
+    worst_cases = [{"case": cs_ok[i].name, "mean_CD_fp64": round(errs[i]["_mean_CD64"], 5),
+                    "rms_CD_fp64": round(errs[i]["_rms_CD64"], 5),
+                    **{k: float("%.3g" % errs[i][k]) for k in keys + ("mean_CD_rel",)}} for i in order]
     lat_s = tm["lattice_ms"] * 1e-3
     achieved = interior * BYTES_PER_NODE_F32 / lat_s / 1e9
     out = {
@@ -156,19 +191,109 @@ def bench_batch(args):
                                "St 0.1..0.5, %d steps each" % (len(cs), steps),
                    "lattice_nodes": nodes},
         "sequential_plain_fp64_cases_per_hr": round(len(cs) / t_seq * 3600.0, 1),
-        "sequential_plain_timing": "per case: setup + %d untimed + %d timed steps, scaled to %d steps"
-                                   % (seq_w, seq_k, steps),
+        "sequential_plain_timing": "per case alone (FP64, one reused engine): setup + %d untimed + %d timed "
+                                   "steps, scaled to %d steps" % (seq_w, seq_k, steps),
         "speedup_vs_sequential_plain": round(t_seq / t_batch, 3),
         "batched_fp64_cases_per_hr": round(len(cs) / t_b64 * 3600.0, 1),
         "aggregate_MLUPS": round(nodes * steps / t_batch / 1e6, 1),
         "accuracy_vs_fp64_worst_case": {k: float("%.3g" % v) for k, v in worst.items()},
-        "accuracy_window": "last half of each run",
-        "cases_within_0.5pct": int(sum(all(v <= 0.005 for v in e.values()) for e in errs)),
+        "accuracy_window": "last half of each run; mean C_D and mean C_L relative to their r.m.s., peak |C_L| "
+                           "relative; mean_CD_rel relative to |mean C_D|",
+        "cases_compared": len(errs),
+        "cases_within_0.5pct": int(sum(all(e[k] <= 0.005 for k in keys) for e in errs)),
+        "cases_within_0.5pct_incl_mean_CD_rel": int(sum(all(e[k] <= 0.005 for k in keys + ("mean_CD_rel",))
+                                                        for e in errs)),
+        "largest_mean_CD_rel_cases": worst_cases,
+        "diverged_cases": diverged,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_lattice<float,%d> over all cases" % shape["nodes_per_thread"],
                      "bytes_per_launch": interior * BYTES_PER_NODE_F32, "mean_launch_ms": round(lat_s * 1e3, 5)},
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},
+        "launch_shape": shape,
+    }
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    from fortran_loadfile_mitiwing_amd.serve import GPUBackend, Server
+    n_req = args.requests
+    # capacity: a saturated batch of 16 cases of the serving distribution
+    probe = [r.case for r in cases.serving_requests(16, rate_per_s=1.0, seed=cases.SWEEP_SEED + 2)]
+    sims = [lbmib.LBMIB(c, precision=lbmib.PREC_FP32DEV) for c in probe]
+    b = lbmib.Batch()
+    for s_ in sims:
+        b.add(s_)
+    shape = b.autotune(trial_steps=32)["best"] if not args.no_tune else b.shape()
+    b.step(16)
+    t0 = time.perf_counter()
+    b.step(96)
+    dt = time.perf_counter() - t0
+    cap_mlups = sum(c.params.nx * c.params.ny for c in probe) * 96 / dt / 1e6
+    b.close()
+    for s_ in sims:
+        s_.close()
+    print("[serve] capacity %.0f MLUPS (16-case probe), shape %s" % (cap_mlups, shape), file=sys.stderr, flush=True)
+    # requests: Poisson arrivals offering args.load of that capacity
+    reqs = cases.serving_requests(n_req, rate_per_s=1.0)
+    work = np.array([r.case.params.nx * r.case.params.ny * r.case.n_steps for r in reqs], dtype=np.float64)
+    rate = args.load * cap_mlups * 1e6 / work.mean()
+    for r in reqs:
+        r.arrival = r.arrival / rate
+    free, total = lbmib.mem_info(0)
+    budget = int(min(free * 0.85, args.budget_gb * 1e9)) if args.budget_gb else int(free * 0.85)
+    print("[serve] %d requests at %.3f /s (load %.2f), HBM budget %.1f GB" % (n_req, rate, args.load, budget / 1e9),
+          file=sys.stderr, flush=True)
+    srv = Server(GPUBackend(precision=lbmib.PREC_FP32DEV, shape=shape), hbm_budget=budget, window=args.window)
+    last = [time.perf_counter()]
+
+    def progress(s):
+        if time.perf_counter() - last[0] > 5.0:
+            last[0] = time.perf_counter()
+            print("[serve] t=%.0fs done %d running %d queued %d" % (s.now(), len(s.results), len(s.running),
+                                                                    len(s.queue)), file=sys.stderr, flush=True)
+
+    res = srv.run(reqs, progress=progress)
+    st = srv.stats()
+    srv.close()
+    # accuracy: a seeded sample of requests rerun in FP64 (one batch)
+    rng = np.random.default_rng(7)
+    pick = sorted(rng.choice(n_req, size=min(args.check, n_req), replace=False).tolist())
+    byidx = {r.tag: r for r in res}
+    sims = [lbmib.LBMIB(reqs[i].case) for i in pick]
+    b = lbmib.Batch()
+    for s_ in sims:
+        b.add(s_)
+    b.step(max(reqs[i].case.n_steps for i in pick))
+    errs = [coefficient_errors(byidx[i].history, s_.history()) for i, s_ in zip(pick, sims)]
+    b.close()
+    for s_ in sims:
+        s_.close()
+    keys = ("mean_CD", "mean_CL", "peak_CL")
+    out = {
+        "metric": "serving aggregate MLUPS (continuous batching, FP32-deviation form, %d requests)" % n_req,
+        "value": round(st["aggregate_MLUPS"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": int(sum(r.case.n_steps for r in reqs)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 lattice, f64 immersed boundary",
+        "data": "synthetic (seeded requests: NACA0012 cases, nx 512..4096, 2000..20000 steps log-uniform)",
+        "config": {"workload": "serving: %d requests, Poisson arrivals at %.3f /s = %.0f%% of the %.0f MLUPS "
+                               "capacity, admission by HBM footprint (budget %.1f GB), windows of %d steps"
+                               % (n_req, rate, 100 * args.load, cap_mlups, budget / 1e9, args.window)},
+        "p50_time_to_result_s": round(st["p50_time_to_result_s"], 3),
+        "p99_time_to_result_s": round(st["p99_time_to_result_s"], 3),
+        "stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+        "capacity_MLUPS_probe": round(cap_mlups, 1),
+        "accuracy_vs_fp64_sample": {"requests": pick,
+                                    "worst": {k: float("%.3g" % max(e[k] for e in errs)) for k in keys},
+                                    "within_0.5pct": int(sum(all(e[k] <= 0.005 for k in keys) for e in errs))},
         "launch_shape": shape,
     }
     print(json.dumps(out))
@@ -183,11 +308,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
-    ap.add_argument("--form", choices=["step", "batch"], default="step",
-                    help="step: one case's timestep (default); batch: the 96-case parameter sweep")
+    ap.add_argument("--form", choices=["step", "batch", "serve"], default="step",
+                    help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
+                         "serve: the continuous-batching service")
+    ap.add_argument("--requests", type=int, default=500, help="--form serve: requests")
+    ap.add_argument("--load", type=float, default=0.8, help="--form serve: offered load / capacity")
+    ap.add_argument("--window", type=int, default=32, help="--form serve: steps per scheduling window")
+    ap.add_argument("--budget-gb", type=float, default=0.0, help="--form serve: HBM budget (0: 85%% of free)")
+    ap.add_argument("--check", type=int, default=8, help="--form serve: requests rerun in FP64")
     ap.add_argument("--cases", type=int, default=96, help="--form batch: cases in the sweep")
     ap.add_argument("--sweep-steps", type=int, default=10000, help="--form batch: steps per case")
     ap.add_argument("--no-tune", action="store_true", help="--form batch: keep the default launch shape")
+    ap.add_argument("--only-batched", action="store_true", help="--form batch: only the batched FP32 run (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (the known-size kernel that scales PMC byte counters)")
     args = ap.parse_args()
@@ -206,6 +338,8 @@ def main():
         sys.exit(2)
     if args.form == "batch":
         return bench_batch(args)
+    if args.form == "serve":
+        return bench_serve(args)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
     fp32 = args.precision == "fp32"

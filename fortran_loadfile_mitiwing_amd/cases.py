@@ -22,6 +22,7 @@ and ``s8 = s9 = 1/tau``.
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 from typing import List, Optional, Sequence, Tuple
 
@@ -46,6 +47,14 @@ def naca_thickness(x: np.ndarray, t: float = 0.12, closed_te: bool = True) -> np
 
 def naca4_polygon(n_markers: int, t: float = 0.12, pivot: float = 0.25,
                   fine: int = 400_000) -> Tuple[np.ndarray, np.ndarray]:
+    """Closed NACA 00xx polygon with ``n_markers`` segments of equal arclength
+    (see :func:`_naca4_polygon`; computed once per argument set)."""
+    px, py = _naca4_polygon(n_markers, t, pivot, fine)
+    return np.array(px), np.array(py)
+
+
+@functools.lru_cache(maxsize=64)
+def _naca4_polygon(n_markers: int, t: float, pivot: float, fine: int):
     """Closed NACA 00xx polygon with ``n_markers`` segments of equal arclength.
 
     Returns ``n_markers + 1`` points (first == last) in chord units with the
@@ -68,7 +77,7 @@ def naca4_polygon(n_markers: int, t: float = 0.12, pivot: float = 0.25,
     # LE -> upper -> TE, then the mirrored lower surface back to the LE
     px = np.concatenate([ux, ux[::-1][1:]])
     py = np.concatenate([uy, -uy[::-1][1:]])
-    return px, py
+    return tuple(px.tolist()), tuple(py.tolist())
 
 
 def quantise_f14_8(a: np.ndarray) -> np.ndarray:

@@ -157,6 +157,7 @@ int dalloc(lbmib_handle* h, T** ptr, size_t count)
     if (e != hipSuccess)
         return fail(LBMIB_E_HIP, "hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
     h->allocs.push_back(p);
+    h->device_bytes += (long long)(sizeof(T) * std::max<size_t>(count, 1));
     *ptr = (T*)p;
     return 0;
 }
@@ -733,6 +734,24 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
     }
     if (glx) HIP_TRY(hipMemcpy(glx, S.glx, sizeof(double) * na, hipMemcpyDeviceToHost));
     if (gly) HIP_TRY(hipMemcpy(gly, S.gly, sizeof(double) * na, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbmib_device_bytes(const lbmib_handle* h, int64_t* bytes)
+{
+    if (!h || !bytes) return fail(LBMIB_E_ARG, "null argument");
+    *bytes = h->device_bytes;
+    return 0;
+}
+
+int lbmib_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return fail(LBMIB_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    size_t fr = 0, to = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &to));
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)to;
     return 0;
 }
 

@@ -271,10 +271,18 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     else
         hipLaunchKernelGGL(k_crit_solve<32>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
+    // the forced collide of the IB nodes follows the solves on their reserved
+    // CUs, or (shape.band) on the shared CUs of the right-hand-side stream
+    hipStream_t sb = s1;
+    if (E->shape.band) {
+        HIP_TRY(hipEventRecord(E->ev_solve[gs % R], s1));
+        HIP_TRY(hipStreamWaitEvent(s3, E->ev_solve[gs % R], 0));
+        sb = s3;
+    }
     const int band_x = std::max(1, (3 * d.maxns + 255) / 256);
-    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, s1, C, K, off);
-    if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
-    HIP_TRY(hipEventRecord(E->ev_ib[gs % R], s1));
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, sb, C, K, off);
+    if (ev) HIP_TRY(hipEventRecord(ev[4], sb));
+    HIP_TRY(hipEventRecord(E->ev_ib[gs % R], sb));
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
     if (wfirst) HIP_TRY(hipStreamWaitEvent(s0, wfirst, 0));
     if (wnext) HIP_TRY(hipStreamWaitEvent(s0, wnext, 0));
@@ -476,7 +484,7 @@ int engine_create(int dev, Engine** out)
     auto mk = [&](hipEvent_t* ev) { return hipEventCreateWithFlags(ev, hipEventDisableTiming); };
     bool ok = true;
     for (int i = 0; i < Engine::RING; ++i)
-        for (hipEvent_t* ev : {&E->ev_ib[i], &E->ev_step[i], &E->ev_rhs[i]}) ok = ok && mk(ev) == hipSuccess;
+        for (hipEvent_t* ev : {&E->ev_ib[i], &E->ev_step[i], &E->ev_rhs[i], &E->ev_solve[i]}) ok = ok && mk(ev) == hipSuccess;
     for (int i = 0; i < Engine::BRING; ++i)
         for (hipEvent_t* ev : {&E->ev_first[i], &E->ev_next[i]}) ok = ok && mk(ev) == hipSuccess;
     ok = ok && mk(&E->ev_fork) == hipSuccess;
@@ -502,7 +510,7 @@ void engine_destroy(Engine* E)
     for (hipEvent_t e : E->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : E->gevpool) (void)hipEventDestroy(e);
     for (int i = 0; i < Engine::RING; ++i)
-        for (hipEvent_t e : {E->ev_ib[i], E->ev_step[i], E->ev_rhs[i]})
+        for (hipEvent_t e : {E->ev_ib[i], E->ev_step[i], E->ev_rhs[i], E->ev_solve[i]})
             if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < Engine::BRING; ++i)
         for (hipEvent_t e : {E->ev_first[i], E->ev_next[i]})
@@ -520,6 +528,7 @@ int engine_set_shape(Engine* E, const Shape& s)
     if (s.persist < 0 || s.persist > 64) return fail(LBMIB_E_ARG, "persistent must be 0..64");
     if (s.ib_cus < 0 || s.ib_cus % 8) return fail(LBMIB_E_ARG, "ib_cus must be a non-negative multiple of 8");
     if (s.graph != 0 && s.graph != 1) return fail(LBMIB_E_ARG, "graph must be 0 or 1");
+    if (s.band != 0 && s.band != 1) return fail(LBMIB_E_ARG, "ib_collide_shared must be 0 or 1");
     // Graph replay is switched off: its first run on the GPU (a three-case
     // batch, replaying captured 16-step windows on CU-masked streams) ended
     // the host process with a segmentation fault whose cause is not known.
@@ -642,6 +651,7 @@ Shape shape_in(const lbmib_launch_shape* s)
     o.persist = s->persistent;
     o.ib_cus = s->ib_cus;
     o.graph = s->graph;
+    o.band = s->ib_collide_shared;
     return o;
 }
 
@@ -653,6 +663,7 @@ void shape_out(const Shape& s, lbmib_launch_shape* o)
     o->persistent = s.persist;
     o->ib_cus = s.ib_cus;
     o->graph = s.graph;
+    o->ib_collide_shared = s.band;
 }
 
 }  // namespace
@@ -830,8 +841,8 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
         ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / timed;
         char buf[256];
         snprintf(buf, sizeof buf, "%s{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-                 "\"graph\":%d,\"ms_per_step\":%.5f}", firstrec ? "" : ",", s.npt, s.order, s.persist, s.ib_cus,
-                 s.graph, ms);
+                 "\"ib_collide_shared\":%d,\"graph\":%d,\"ms_per_step\":%.5f}", firstrec ? "" : ",", s.npt, s.order,
+                 s.persist, s.ib_cus, s.band, s.graph, ms);
         rep += buf;
         firstrec = false;
         return 0;
@@ -860,12 +871,14 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
                 }
         if (!rc) {
             const Shape base = cur;
-            for (int cus : {0, 8, 16, 32}) {
-                if (cus == base.ib_cus) continue;
-                Shape s = base;
-                s.ib_cus = cus;
-                if ((rc = consider(s))) break;
-            }
+            for (int cus : {0, 8, 16, 32})
+                for (int band : {0, 1}) {
+                    if (cus == base.ib_cus && band == base.band) continue;
+                    Shape s = base;
+                    s.ib_cus = cus;
+                    s.band = band;
+                    if ((rc = consider(s))) break;
+                }
         }
         // graph replay stays out of the trials while it is switched off
     }
@@ -877,7 +890,8 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
     if (rc) return rc;
     char buf[256];
     snprintf(buf, sizeof buf, "],\"best\":{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-             "\"graph\":%d,\"ms_per_step\":%.5f}}", cur.npt, cur.order, cur.persist, cur.ib_cus, cur.graph, cur_ms);
+             "\"ib_collide_shared\":%d,\"graph\":%d,\"ms_per_step\":%.5f}}", cur.npt, cur.order, cur.persist,
+             cur.ib_cus, cur.band, cur.graph, cur_ms);
     rep += buf;
     if (best) shape_out(cur, best);
     if (report && report_len > 0) {

@@ -43,6 +43,7 @@ struct lbmib_handle {
     int steps_done = 0;
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
     std::vector<void*> allocs;        // device allocations to free
+    long long device_bytes = 0;       // their total size
     int dev = 0;
 };
 
@@ -55,6 +56,7 @@ struct Shape {
     int persist = 0;
     int ib_cus = 8;
     int graph = 0;
+    int band = 0;
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in
@@ -62,7 +64,8 @@ struct Shape {
 // launch, each case at its own step (CaseClock).  Streams, all at the
 // default priority:
 //   s0  lattice update (every node but the step's IB nodes, and the ring)
-//   s1  triangular solves + forced collide of the IB nodes (reserved CUs)
+//   s1  triangular solves (reserved CUs), then the forced collide of the IB
+//       nodes (or on s3, shape.band)
 //   s2  geometry of later batches, forces of earlier ones
 //   s3  right-hand sides of the IB systems
 struct Engine {
@@ -74,7 +77,7 @@ struct Engine {
     hipStream_t s[NSTREAM] = {};
     int streams_cus = -1;            // ib_cus the streams were made with
     static constexpr int RING = 64;  // per-step events of engine step q live in slot q % RING
-    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {};
+    hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {}, ev_solve[RING] = {};
     static constexpr int BRING = 8;  // per-boundary events
     hipEvent_t ev_first[BRING] = {}, ev_next[BRING] = {}, ev_fork = nullptr, ev_join[NSTREAM] = {};
     long long gstep = 0, gbound = 0;

@@ -58,16 +58,18 @@ class Params(C.Structure):
 
 class LaunchShape(C.Structure):
     _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
-                ("ib_cus", C.c_int32), ("graph", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("ib_cus", C.c_int32), ("graph", C.c_int32), ("ib_collide_shared", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
-        return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph")}
+        return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph",
+                                                    "ib_collide_shared")}
 
 
-def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0) -> LaunchShape:
+def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0, ib_collide_shared=0) -> LaunchShape:
     s = LaunchShape()
     s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
-    s.ib_cus, s.graph = ib_cus, graph
+    s.ib_cus, s.graph, s.ib_collide_shared = ib_cus, graph, ib_collide_shared
     return s
 
 
@@ -82,7 +84,7 @@ EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
            "lbmib_batch_get_shape", "lbmib_batch_set_profiling", "lbmib_batch_get_timings",
-           "lbmib_batch_autotune", "lbmib_batch_destroy"]
+           "lbmib_batch_autotune", "lbmib_batch_destroy", "lbmib_device_bytes", "lbmib_mem_info"]
 
 
 def load(path: str = LIB_PATH):
@@ -122,6 +124,8 @@ def load(path: str = LIB_PATH):
     L.lbmib_batch_get_timings.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_batch_autotune.argtypes = [vp, C.c_int32, sp, C.c_char_p, C.c_int32]
     L.lbmib_batch_destroy.argtypes = [vp]
+    L.lbmib_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.lbmib_mem_info.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.lbmib_last_error.restype = C.c_char_p
     L.lbmib_version.restype = C.c_char_p
     _lib = L
@@ -260,6 +264,12 @@ class LBMIB:
         _check(load().lbmib_get_timings(self.h, _d(t), NTIMINGS))
         return _timings(t)
 
+    def device_bytes(self) -> int:
+        """Device memory this case holds."""
+        b = C.c_int64()
+        _check(load().lbmib_device_bytes(self.h, C.byref(b)))
+        return b.value
+
     def set_shape(self, **kw):
         """Launch shape of this handle's steps (see :func:`make_shape`)."""
         _check(load().lbmib_set_shape(self.h, C.byref(make_shape(**kw))))
@@ -349,6 +359,13 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def mem_info(device: int = 0):
+    """(free, total) bytes of device memory."""
+    f, t = C.c_int64(), C.c_int64()
+    _check(load().lbmib_mem_info(int(device), C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def version() -> str:

@@ -19,8 +19,9 @@
  *     (j*nx + i); populations are planar, f[k*nx*ny + j*nx + i] is the
  *     reference's f(i+1,j+1,k+1).
  *   - A handle owns its device memory (allocated once in lbmib_create /
- *     lbmib_set_motion_table, nothing per step) and one HIP stream pair.  A
- *     handle is not thread-safe; distinct handles may be used concurrently.
+ *     lbmib_set_motion_table, nothing per step) and, from its first step, four
+ *     HIP streams.  A handle is not thread-safe; distinct handles may be used
+ *     concurrently.
  *   - Calls are synchronous: host arrays are only touched during the call.
  */
 #ifndef LBMIB_H
@@ -140,6 +141,12 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
 int lbmib_set_profiling(lbmib_handle* h, int32_t every);
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 
+/* Device memory the handle holds (all of it allocated by lbmib_create and
+ * lbmib_set_motion_table), for admission by HBM footprint; and the device's
+ * free and total memory.  No reference counterpart. */
+int lbmib_device_bytes(const lbmib_handle* h, int64_t* bytes);
+int lbmib_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
+
 /* Releases the handle and its device memory (removing it from its batch). */
 int lbmib_destroy(lbmib_handle* h);
 
@@ -163,10 +170,12 @@ typedef struct lbmib_launch_shape {
     int32_t persistent;       /* 0: one workgroup per tile; k > 0: k workgroups per CU looping over tiles */
     int32_t ib_cus;           /* CUs reserved for the triangular solves (multiple of 8; 0: none) */
     int32_t graph;            /* 1: windows of 16 steps captured once as a HIP graph and replayed */
-    int32_t reserved[3];
+    int32_t ib_collide_shared; /* 1: the forced collide of the IB nodes runs on the shared CUs, not the
+                                  solves' reserved ones */
+    int32_t reserved[2];
 } lbmib_launch_shape;
 
-/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0; the
+/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0, 0; the
  * environment variable LBMIB_IB_CUS overrides ib_cus). */
 int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
 int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);
@@ -201,7 +210,8 @@ int lbmib_batch_get_timings(const lbmib_batch* b, double* t, int32_t n);
 /* Autotuner: times launch shapes on copies of the members (same lattices,
  * markers and motion tables, own state; the members are not advanced), one
  * coordinate at a time -- lattice tile shape, order and persistent grid, then
- * the CUs reserved for the solves, then graph replay -- over trial_steps
+ * the CUs reserved for the solves with the placement of the IB-node collide
+ * -- over trial_steps
  * steps each, applies the fastest to the batch and returns it in *best.
  * report (may be NULL) receives every trial as JSON. */
 int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape* best, char* report,
