@@ -308,6 +308,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
+    ap.add_argument("--shape", default="",
+                    help="launch shape of the step form, e.g. nodes_per_thread=2,tile_order=1,persistent=4,"
+                         "ib_cus=8,ib_collide_shared=0 (default: the library's)")
     ap.add_argument("--form", choices=["step", "batch", "serve"], default="step",
                     help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
                          "serve: the continuous-batching service")
@@ -346,6 +349,8 @@ def main():
     case = cases.config(args.config, n_steps=K + W + 8)
     p = case.params
     g = lbmib.LBMIB(case, device=0, precision=lbmib.PREC_FP32DEV if fp32 else lbmib.PREC_FP64)
+    if args.shape:
+        g.set_shape(**{k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))})
     if W:
         g.evolution(W)
     # phase events on every PROFILE_EVERY-th timed step: they time the lattice
@@ -384,6 +389,7 @@ def main():
                      "bytes_per_launch": interior * bpn, "mean_launch_ms": round(lat_s * 1e3, 5)},
         "whole_step_hbm_frac": round(value * 1e6 * bpn / (HBM_PEAK_GBS * 1e9), 4),
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},
+        "launch_shape": g.shape(),
     }
     g.close()
     if not args.no_cpu:

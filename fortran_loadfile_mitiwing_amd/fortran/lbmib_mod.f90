@@ -35,10 +35,21 @@ module lbmib
         integer(c_int32_t) :: reserved(6) = 0
     end type
 
+    ! struct lbmib_launch_shape (include/lbmib.h): how steps are launched
+    type, bind(C), public :: lbmib_launch_shape
+        integer(c_int32_t) :: nodes_per_thread = 1, tile_order = 0, persistent = 0
+        integer(c_int32_t) :: ib_cus = 8, graph = 0, ib_collide_shared = 0
+        integer(c_int32_t) :: reserved(2) = 0
+    end type
+
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
     public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings
     public :: lbmib_destroy, lbmib_error_message, lbmib_version_string, lbmib_check
+    ! batched form (many cases advanced together) and footprints
+    public :: lbmib_set_shape, lbmib_get_shape, lbmib_device_bytes, lbmib_mem_info
+    public :: lbmib_batch_create, lbmib_batch_add, lbmib_batch_remove, lbmib_batch_size, lbmib_batch_step
+    public :: lbmib_batch_set_shape, lbmib_batch_get_shape, lbmib_batch_autotune, lbmib_batch_destroy
 
     interface
         integer(c_int) function lbmib_create(p, xb0, yb0, h) bind(C, name="lbmib_create")
@@ -127,6 +138,85 @@ module lbmib
         integer(c_int) function lbmib_destroy(h) bind(C, name="lbmib_destroy")
             import :: c_int, c_ptr
             type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbmib_set_shape(h, s) bind(C, name="lbmib_set_shape")
+            import :: c_int, c_ptr, lbmib_launch_shape
+            type(c_ptr), value :: h
+            type(lbmib_launch_shape), intent(in) :: s
+        end function
+
+        integer(c_int) function lbmib_get_shape(h, s) bind(C, name="lbmib_get_shape")
+            import :: c_int, c_ptr, lbmib_launch_shape
+            type(c_ptr), value :: h
+            type(lbmib_launch_shape), intent(out) :: s
+        end function
+
+        integer(c_int) function lbmib_device_bytes(h, nbytes) bind(C, name="lbmib_device_bytes")
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: nbytes
+        end function
+
+        integer(c_int) function lbmib_mem_info(device, free_bytes, total_bytes) bind(C, name="lbmib_mem_info")
+            import :: c_int, c_int32_t, c_int64_t
+            integer(c_int32_t), value :: device
+            integer(c_int64_t), intent(out) :: free_bytes, total_bytes
+        end function
+
+        integer(c_int) function lbmib_batch_create(device, b) bind(C, name="lbmib_batch_create")
+            import :: c_int, c_int32_t, c_ptr
+            integer(c_int32_t), value :: device
+            type(c_ptr), intent(out) :: b
+        end function
+
+        integer(c_int) function lbmib_batch_add(b, h) bind(C, name="lbmib_batch_add")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, h
+        end function
+
+        integer(c_int) function lbmib_batch_remove(b, h) bind(C, name="lbmib_batch_remove")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, h
+        end function
+
+        integer(c_int) function lbmib_batch_size(b, n) bind(C, name="lbmib_batch_size")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int32_t), intent(out) :: n
+        end function
+
+        integer(c_int) function lbmib_batch_step(b, n_steps) bind(C, name="lbmib_batch_step")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: n_steps
+        end function
+
+        integer(c_int) function lbmib_batch_set_shape(b, s) bind(C, name="lbmib_batch_set_shape")
+            import :: c_int, c_ptr, lbmib_launch_shape
+            type(c_ptr), value :: b
+            type(lbmib_launch_shape), intent(in) :: s
+        end function
+
+        integer(c_int) function lbmib_batch_get_shape(b, s) bind(C, name="lbmib_batch_get_shape")
+            import :: c_int, c_ptr, lbmib_launch_shape
+            type(c_ptr), value :: b
+            type(lbmib_launch_shape), intent(out) :: s
+        end function
+
+        integer(c_int) function lbmib_batch_autotune(b, trial_steps, best, report, report_len) &
+                bind(C, name="lbmib_batch_autotune")
+            import :: c_int, c_int32_t, c_ptr, lbmib_launch_shape
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: trial_steps
+            type(lbmib_launch_shape), intent(out) :: best
+            type(c_ptr), value :: report
+            integer(c_int32_t), value :: report_len
+        end function
+
+        integer(c_int) function lbmib_batch_destroy(b) bind(C, name="lbmib_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
         end function
 
         type(c_ptr) function c_lbmib_last_error() bind(C, name="lbmib_last_error")
