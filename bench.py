@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--shape", default="",
                     help="launch shape of the step form, e.g. nodes_per_thread=2,tile_order=1,persistent=4,"
                          "ib_cus=8,ib_collide_shared=0 (default: the library's)")
+    ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
+                    help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
     ap.add_argument("--form", choices=["step", "batch", "serve"], default="step",
                     help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
                          "serve: the continuous-batching service")
@@ -348,7 +350,8 @@ def main():
     fp32 = args.precision == "fp32"
     case = cases.config(args.config, n_steps=K + W + 8)
     p = case.params
-    g = lbmib.LBMIB(case, device=0, precision=lbmib.PREC_FP32DEV if fp32 else lbmib.PREC_FP64)
+    solver = {"auto": lbmib.SOLVER_AUTO, "chain": lbmib.SOLVER_CHAIN, "separator": lbmib.SOLVER_SEPARATOR}[args.solver]
+    g = lbmib.LBMIB(case, device=0, precision=lbmib.PREC_FP32DEV if fp32 else lbmib.PREC_FP64, ib_solver=solver)
     if args.shape:
         g.set_shape(**{k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))})
     if W:

@@ -225,6 +225,8 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     if (p->precision != LBMIB_PREC_FP64 && p->precision != LBMIB_PREC_FP32DEV)
         return fail(LBMIB_E_ARG, "precision %d unknown (LBMIB_PREC_FP64 or LBMIB_PREC_FP32DEV)", p->precision);
     if (p->smooth != 0 && p->smooth != 1) return fail(LBMIB_E_ARG, "smooth must be 0 or 1");
+    if (p->ib_solver < LBMIB_SOLVER_AUTO || p->ib_solver > LBMIB_SOLVER_SEPARATOR)
+        return fail(LBMIB_E_ARG, "ib_solver %d unknown", p->ib_solver);
     if ((long long)p->n_wings * (p->pts_per_wing - 1) > MAXSEG)
         return fail(LBMIB_E_ARG, "at most %d marker segments supported", MAXSEG);
 
@@ -486,6 +488,25 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
                     "(wings too close together, or markers much denser than the lattice)", D.bw, MAXBW);
     D.ws = 64;
     D.npad = (D.nseg + 63) / 64 * 64;
+    // separator form of the IB solve: rows mT .. mT+bw-1 split the band into
+    // two halves solved as concurrent chains (a different round-off order
+    // than the oracle's single chain, so the FP64 drop-in uses it only when
+    // asked for)
+    {
+        const int sol = h->p.ib_solver;
+        const bool want = sol == LBMIB_SOLVER_SEPARATOR || (sol == LBMIB_SOLVER_AUTO && D.prec == LBMIB_PREC_FP32DEV);
+        const int b = D.bw, mT = (D.nseg - b) / 2, mB = D.nseg - b - mT;
+        D.dissect = want && b >= 1 && b <= SEPMAXBW && sep_len1(b) <= SEP_L1 && sep_len2(b) <= SEP_L2 && mT >= b &&
+                    mB >= b && mT >= 64 ? 1 : 0;
+        D.mT = D.dissect ? mT : 0;
+        D.mB = D.dissect ? mB : 0;
+        D.np2 = D.dissect ? (std::max(mT, mB) + 63) / 64 * 64 : 0;
+        if (sol == LBMIB_SOLVER_SEPARATOR && !D.dissect)
+            return fail(LBMIB_E_ARG, "the separator solve needs at least 2*64 + bandwidth segments and bandwidth "
+                        "<= %d (have %d segments, bandwidth %d)", SEPMAXBW, D.nseg, b);
+        if (D.dissect && solve2_unit(D.np2) == 0)
+            return fail(LBMIB_E_ARG, "too many segments for the separator solve's LDS");
+    }
     double* dkin;
     if (dalloc(h, &dkin, kin.size())) return LBMIB_E_HIP;
     HIP_TRY(hipMemcpy(dkin, kin.data(), sizeof(double) * kin.size(), hipMemcpyHostToDevice));
@@ -524,6 +545,13 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         dd.G = o; o += al(ns * W);
         dd.coef = o; o += al((long long)(2 * D.npad + 64) * 2 * D.ws);
         dd.dinv = o; o += al(D.npad);
+        {
+            const long long b = D.dissect ? D.bw : 0, np2 = D.np2;
+            dd.coef2 = o; o += al(D.dissect ? 2 * (2 * np2 + 64) * 2 * D.ws : 0);
+            dd.dinv2 = o; o += al(2 * np2);
+            dd.ltail2 = o; o += al(2 * b * (b + 1));
+            dd.zsep = o; o += al(5 * b * b);
+        }
         dd.rhoadj = o; o += al(cap);
         dd.glx = o; o += al(cap);
         dd.gly = o; o += al(cap);

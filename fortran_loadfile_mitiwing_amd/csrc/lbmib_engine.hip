@@ -133,7 +133,17 @@ void build_tiles(const Engine* E, std::vector<int2>& t)
 
 struct Dims {
     int nc = 0, maxns = 0, maxcap = 0, maxW = 0, maxnpad = 0, prec = 0;
+    int chain = 0, sep = 0, maxnp2 = 0;  // members solved as one chain / in separator form
 };
+
+const void* solve2_kernel(int np2)
+{
+    switch (solve2_unit(np2)) {
+    case 32: return (const void*)k_crit_solve2<32>;
+    case 16: return (const void*)k_crit_solve2<16>;
+    default: return (const void*)k_crit_solve2<8>;
+    }
+}
 
 Dims dims_of(const Engine* E)
 {
@@ -144,6 +154,12 @@ Dims dims_of(const Engine* E)
         d.maxcap = std::max(d.maxcap, h->D.cap);
         d.maxW = std::max(d.maxW, h->D.bw + 1);
         d.maxnpad = std::max(d.maxnpad, h->D.npad);
+        if (h->D.dissect) {
+            d.sep++;
+            d.maxnp2 = std::max(d.maxnp2, h->D.np2);
+        } else {
+            d.chain++;
+        }
     }
     d.prec = E->members.empty() ? 0 : E->members[0]->D.prec;
     return d;
@@ -193,6 +209,12 @@ int engine_prepare(Engine* E)
                                 (int)factor_smem_bytes()));
     HIP_TRY(hipFuncSetAttribute(solve_unit(d.maxnpad) == 64 ? (const void*)k_crit_solve<64> : (const void*)k_crit_solve<32>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_smem_bytes(d.maxnpad)));
+    if (d.sep) {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)factor_smem_bytes()));
+        HIP_TRY(hipFuncSetAttribute(solve2_kernel(d.maxnp2), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)solve2_smem_bytes(d.maxnp2)));
+    }
     return 0;
 }
 
@@ -229,9 +251,14 @@ int launch_geometry(Engine* E, const Dims& d, int off, int mode)
 int launch_factor(Engine* E, const Dims& d, int off, int mode, hipEvent_t* gev)
 {
     hipStream_t st = E->s[2];
+    const DevCase* C = E->dcases;
+    const CaseClock* K = E->dclock;
     if (gev) HIP_TRY(hipEventRecord(gev[1], st));
-    hipLaunchKernelGGL(k_geo_factor, dim3(E->B, d.nc), dim3(256), factor_smem_bytes(), st,
-                       (const DevCase*)E->dcases, (const CaseClock*)E->dclock, off, mode);
+    if (d.chain) hipLaunchKernelGGL(k_geo_factor, dim3(E->B, d.nc), dim3(256), factor_smem_bytes(), st, C, K, off, mode);
+    if (d.sep) {
+        hipLaunchKernelGGL(k_geo_factor2, dim3(E->B, d.nc, 2), dim3(256), factor_smem_bytes(), st, C, K, off, mode);
+        hipLaunchKernelGGL(k_geo_schur, dim3(E->B, d.nc), dim3(64), 0, st, C, K, off, mode);
+    }
     if (gev) HIP_TRY(hipEventRecord(gev[2], st));
     return 0;
 }
@@ -266,10 +293,20 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3(rhs_x, d.nc), dim3(256), 0, s3, C, K, off);
     HIP_TRY(hipEventRecord(E->ev_rhs[gs % R], s3));
     HIP_TRY(hipStreamWaitEvent(s1, E->ev_rhs[gs % R], 0));
-    if (solve_unit(d.maxnpad) == 64)
-        hipLaunchKernelGGL(k_crit_solve<64>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
-    else
-        hipLaunchKernelGGL(k_crit_solve<32>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
+    if (d.chain) {
+        if (solve_unit(d.maxnpad) == 64)
+            hipLaunchKernelGGL(k_crit_solve<64>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
+        else
+            hipLaunchKernelGGL(k_crit_solve<32>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
+    }
+    if (d.sep) {
+        const size_t sm = solve2_smem_bytes(d.maxnp2);
+        switch (solve2_unit(d.maxnp2)) {
+        case 32: hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
+        case 16: hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
+        default: hipLaunchKernelGGL(k_crit_solve2<8>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
+        }
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
     // the forced collide of the IB nodes follows the solves on their reserved
     // CUs, or (shape.band) on the shared CUs of the right-hand-side stream

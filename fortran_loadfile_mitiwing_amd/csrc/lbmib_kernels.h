@@ -13,6 +13,16 @@ constexpr int NCAP = 64;    // markers whose boxes may contain one IB node
 constexpr int KIN = 8;      // per wing per step: sin, cos, omega, sa, sau, sb, sbv, pad
 constexpr int MAXBW = 62;   // largest bandwidth of the banded IB solver
 constexpr int MAXSEG = 4096;
+// separator form of the IB solve: largest bandwidth, and the slices of the
+// separator matrices M (bw x 3bw) and N (2bw x bw) that each of the 128
+// chain threads holds in registers: row pieces of SEP_L1 and SEP_L2 terms
+constexpr int SEPMAXBW = 32;
+constexpr int SEP_L1 = 24;
+constexpr int SEP_L2 = 16;
+__host__ __device__ inline int sep_seg1(int bw) { return 128 / bw; }
+__host__ __device__ inline int sep_len1(int bw) { return (3 * bw + sep_seg1(bw) - 1) / sep_seg1(bw); }
+__host__ __device__ inline int sep_seg2(int bw) { return 128 / (2 * bw); }
+__host__ __device__ inline int sep_len2(int bw) { return (bw + sep_seg2(bw) - 1) / sep_seg2(bw); }
 
 enum ErrBits { ERR_GEOMETRY = 1, ERR_CAPACITY = 2, ERR_SOLVE = 4 };
 
@@ -43,6 +53,12 @@ struct SlotDoubles {
     long long G;        // [nseg][bw+1] band of D D^T, G[p][d] = A(p, p-d)
     long long coef;     // [2*npad+64][ws][2] per-lane coefficients of the solve timeline
     long long dinv;     // [npad] 1/d_k of G = L D L^T (zero in the padding)
+    // separator form of the solve (DevCase::dissect): two half systems, each
+    // factored as above, and the dense Schur complement of the separator
+    long long coef2;    // [2][2*np2+64][ws][2] solve timelines of the two halves
+    long long dinv2;    // [2][np2]
+    long long ltail2;   // [2][bw][bw+1] last bw rows of each half's factor, l(i, i-d)
+    long long zsep;     // separator step: M [bw][3bw] (x_S = M v), then N [2bw][bw] (tail corrections = N x_S)
     long long rhoadj;   // [cap] rho at the nodes
     long long glx, gly; // [cap] velocity corrections
     long long total;
@@ -64,6 +80,9 @@ struct DevCase {
     int bw;                 // lower bandwidth of the zig-zag ordered IB matrix
     int ws;                 // rows per lane group of the triangular solves (32 or 64)
     int npad;               // nseg rounded up to a multiple of 64
+    int dissect;            // 1: solve through the separator form (two concurrent half chains)
+    int mT, mB, np2;        // separator form: rows of the top half, of the bottom half (taken in
+                            // reverse), and the halves' padded length
     int steps_cap;          // rows of kin / hist
     int nslot;
     int batch;              // steps per geometry batch (nslot = 2 * batch)
@@ -117,6 +136,7 @@ enum GeoMode { GEO_FIRST = 0, GEO_NEXT = 1 };
 struct Slot {
     int *claim, *nodeidx, *box, *draw_idx, *nraw_s, *adj, *nadj, *dcnt, *didx, *nmkcnt, *nmk;
     double *xb2, *yb2, *ub, *vb, *phi, *draw_val, *nraw_w, *dval, *nmkv, *G, *coef, *dinv, *rhoadj, *glx, *gly;
+    double *coef2, *dinv2, *ltail2, *zsep;
 };
 
 __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
@@ -132,6 +152,8 @@ __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
     S.phi = db + D.dof.phi; S.draw_val = db + D.dof.draw_val; S.nraw_w = db + D.dof.nraw_w;
     S.dval = db + D.dof.dval; S.nmkv = db + D.dof.nmkv; S.G = db + D.dof.G; S.coef = db + D.dof.coef;
     S.dinv = db + D.dof.dinv; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
+    S.coef2 = db + D.dof.coef2; S.dinv2 = db + D.dof.dinv2; S.ltail2 = db + D.dof.ltail2;
+    S.zsep = db + D.dof.zsep;
     return S;
 }
 
@@ -150,12 +172,19 @@ __global__ void k_geo_dsort(const DevCase* __restrict__, const CaseClock* __rest
 __global__ void k_geo_nsort(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_assemble(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_factor(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_factor2(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
+__global__ void k_geo_schur(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 template <typename T>
 __global__ void k_crit_rhs(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 template <int U>
 __global__ void k_crit_solve(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 int solve_unit(int npad);
 int solve_threads();
+template <int U>
+__global__ void k_crit_solve2(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+int solve2_unit(int np2);  // 0 when the values leave no room for the rings
+int solve2_threads();
+size_t solve2_smem_bytes(int np2);
 template <typename T>
 __global__ void k_band(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 __global__ void k_forces(const DevCase* __restrict__, const CaseClock* __restrict__, int);

@@ -641,8 +641,9 @@ constexpr int WIN = 64;  // LDS window slots of the banded factorisation (rows k
 constexpr int CH = 32;   // rows / columns staged per chunk
 constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
 
-// Banded L D L^T of G, right-looking over a sliding LDS window of rows
-// k..k+bw.  Step k takes the pivot d_k = a(k,k), forms the column
+// Banded L D L^T of an n x n SPD band of lower bandwidth bw, right-looking
+// over a sliding LDS window of rows k..k+bw; ga(r, d) reads entry
+// a(r, r-d).  Step k takes the pivot d_k = a(k,k), forms the column
 // l(i,k) = t_i * (1/d_k) from its unscaled entries t_i and updates the
 // trailing band a(i,j) -= l(i,k) * t_j.  Each entry receives its updates in
 // increasing k, so the result equals the dense right-looking algorithm bit
@@ -650,33 +651,29 @@ constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
 // results leave through plain stores that are never waited on inside the
 // loop.  Besides 1/d_k the kernel writes the coefficients of step k of both
 // triangular solves in the per-lane order of k_crit_solve (see there), the
-// forward step at timeline position k, the backward one at 2*npad-1-k:
+// forward step at timeline position k, the backward one at 2*np-1-k:
 //   forward  [h] = {l(k+e1,k), l(k+e2,k)}, e1 = h-q (h > q), e2 = ws+h-q (h <= q)
 //   backward [h] = {l(k,k-e1), l(k,k-e2)}, e1 = q-h (h < q), e2 = ws+q-h (h >= q)
-// with q = k mod ws, zero outside the band.  One workgroup per slot.
-__global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                    int off, int mode)
+// with q = k mod ws, zero outside the band.  With ltail, row k's final band
+// (d_k, l(k,k-1), .., l(k,k-bw)) of the last bw rows is kept as well.
+// One workgroup of 256 threads.
+template <typename GA>
+__device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws, int np, double* __restrict__ coef,
+                                            double* __restrict__ dinv, double* __restrict__ ltail, int* err)
 {
-    const DevCase& D = C[blockIdx.y];
-    int step;
-    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
-    const Slot S = slot_of(D, step % D.nslot);
     extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
-    const int n = D.nseg, bw = D.bw, W = bw + 1, ws = D.ws;
+    const int W = bw + 1;
     double* win = smem;
     double* colv = win + WIN * WIN;
     double* colt = colv + WIN;
     double* stage = colt + WIN;  // [2][CH][WIN]
     int* pairs = (int*)(stage + 2 * CH * WIN);
-    double* coef = S.coef;
-    const int np = D.npad;
-    const double* G = S.G;
     for (int a = tid; a < bw; a += T)
         for (int b = 0; b <= a; ++b) pairs[a * (a + 1) / 2 + b] = (a << 8) | b;
     for (int e = tid; e < min(W, n) * W; e += T) {
         int r = e / W, d = e % W;
-        win[(r % WIN) * WIN + d] = G[(long long)r * W + d];
+        win[(r % WIN) * WIN + d] = ga(r, d);
     }
     // chunk c: rows W + c*CH .. W + c*CH + CH - 1, entering at steps c*CH .. c*CH + CH - 1
     constexpr int PER = (CH * WIN + 255) / 256;
@@ -686,7 +683,7 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
         for (int q = 0; q < PER; ++q) {
             int e = tid + q * T;
             int r = W + c * CH + e / W, d = e % W;
-            pre[q] = (e < CH * W && r < n) ? G[(long long)r * W + d] : 0.0;
+            pre[q] = (e < CH * W && r < n) ? ga(r, d) : 0.0;
         }
     };
     auto store_chunk = [&](int buf) {
@@ -720,7 +717,7 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
             colt[tid] = t;
         }
         lds_barrier();
-        if (tid == 0 && !(akk > 0.0)) atomicOr(D.err, ERR_SOLVE);
+        if (tid == 0 && !(akk > 0.0)) atomicOr(err, ERR_SOLVE);
         const int npairs = m * (m + 1) / 2;
         for (int t = tid; t < npairs; t += T) {
             int ab = pairs[t];
@@ -743,7 +740,10 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
                 const bool ok = (j == 0 ? hh < q : hh >= q) && e <= min(bw, k);
                 coef[(long long)(2 * np - 1 - k) * 2 * ws + tid - 2 * ws] = ok ? win[sk + e] : 0.0;
             }
-            if (tid == 0) S.dinv[k] = rk;
+            if (tid == 0) dinv[k] = rk;
+            if (ltail && k >= n - bw)
+                for (int d = tid; d < W; d += T)
+                    ltail[(long long)(k - (n - bw)) * W + d] = d == 0 ? akk : (d <= k ? win[sk + d] : 0.0);
         }
         // row k+W enters the window (its slot differs from every row in use)
         const int rn = k + W;
@@ -751,6 +751,150 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
             for (int d = tid; d < W; d += T)
                 win[(rn % WIN) * WIN + d] = stage[(c & 1) * CH * WIN + (k - c * CH) * WIN + d];
         lds_barrier();
+    }
+}
+
+// L D L^T of the whole zig-zag band G (cases solved as one chain).
+__global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                    int off, int mode)
+{
+    const DevCase& D = C[blockIdx.y];
+    if (D.dissect) return;
+    int step;
+    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
+    const double* G = S.G;
+    const int W = D.bw + 1;
+    auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
+    band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err);
+}
+
+// Separator form of the solve.  With bw the band's lower bandwidth, rows
+// mT .. mT+bw-1 (the separator S) split G into a top half T (rows 0..mT-1)
+// and a bottom half B (rows mT+bw .. nseg-1) that share no entry.  B is
+// taken in reverse row order, so that both halves meet S with their last bw
+// rows.  Each half is factored here (blockIdx.z: 0 = T, 1 = B reversed) as
+// its own band; k_geo_schur then forms the separator's Schur complement.
+__global__ __launch_bounds__(256) void k_geo_factor2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                     int off, int mode)
+{
+    const DevCase& D = C[blockIdx.y];
+    if (!D.dissect) return;
+    int step;
+    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
+    const double* G = S.G;
+    const int W = D.bw + 1, ns = D.nseg, half = blockIdx.z;
+    double* coef = S.coef2 + (long long)half * (2 * D.np2 + 64) * 2 * D.ws;
+    double* dinv = S.dinv2 + (long long)half * D.np2;
+    double* lt = S.ltail2 + (long long)half * D.bw * W;
+    if (half == 0) {
+        auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
+        band_factor(ga, D.mT, D.bw, D.ws, D.np2, coef, dinv, lt, D.err);
+    } else {
+        // reversed row r is row ns-1-r; its entry a'(r, r-d) = a(ns-1-r+d, ns-1-r)
+        auto ga = [G, W, ns](int r, int d) { return d <= r ? G[(long long)(ns - 1 - r + d) * W + d] : 0.0; };
+        band_factor(ga, D.mB, D.bw, D.ws, D.np2, coef, dinv, lt, D.err);
+    }
+}
+
+// Schur complement of the separator, S' = A_S - sum over the halves of
+// Y^T D^-1 Y with Y = L^-1 C (L, D the half's factor, C its coupling to S,
+// nonzero only in the half's last bw rows, so Y is the bw x bw block
+// Lt^-1 Ct of the factor's last bw rows), S'^-1 by Gauss-Jordan (S' is SPD),
+// and the separator matrices of the solve: with v = [f_S; w_T; w_B] (w the
+// halves' forward results D^-1 L^-1 f over their last bw rows),
+//   x_S = M v,                 M = [S'^-1, -S'^-1 Y_T^T, -S'^-1 Y_B^T]
+//   [w_T; w_B] -= N x_S,       N = [D_T Y_T; D_B Y_B]  (D_* the tail's 1/d)
+// One wavefront per slot; lane s owns column (or row) s.
+__global__ __launch_bounds__(64) void k_geo_schur(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                  int off, int mode)
+{
+    const DevCase& D = C[blockIdx.y];
+    if (!D.dissect) return;
+    int step;
+    if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
+    const Slot S = slot_of(D, step % D.nslot);
+    constexpr int MB = SEPMAXBW;
+    __shared__ double lt[2][MB][MB + 1], Y[2][MB][MB + 1], Sm[MB][MB + 1], Iv[MB][MB + 1];
+    __shared__ double fcol[MB];
+    const int b = D.bw, W = b + 1, mT = D.mT, mB = D.mB, np2 = D.np2, b3 = 3 * b;
+    const int s = threadIdx.x;
+    const double* G = S.G;
+    for (int e = s; e < 2 * b * W; e += 64) {
+        const int h = e / (b * W), r = e % (b * W);
+        lt[h][r / W][r % W] = S.ltail2[e];
+    }
+    __syncthreads();
+    if (s < b) {
+        for (int h = 0; h < 2; ++h) {
+            for (int i = 0; i < b; ++i) {
+                // coupling of the half's row (last bw rows, i) to separator row s
+                double cv;
+                if (h == 0) {
+                    const int d = b + s - i;  // a(mT+s, mT-b+i)
+                    cv = d <= b ? G[(long long)(mT + s) * W + d] : 0.0;
+                } else {
+                    const int d = 2 * b - 1 - i - s;  // a(mT+2b-1-i, mT+s)
+                    cv = d <= b ? G[(long long)(mT + 2 * b - 1 - i) * W + d] : 0.0;
+                }
+                for (int j = 0; j < i; ++j) cv = cv - lt[h][i][i - j] * Y[h][j][s];
+                Y[h][i][s] = cv;
+            }
+        }
+    }
+    __syncthreads();
+    const double* dT = S.dinv2 + (mT - b);
+    const double* dB = S.dinv2 + np2 + (mB - b);
+    if (s < b) {
+        for (int t = 0; t < b; ++t) {
+            const int hi = max(s, t), lo = min(s, t);
+            double v = G[(long long)(mT + hi) * W + (hi - lo)];
+            for (int i = 0; i < b; ++i) v = v - Y[0][i][s] * dT[i] * Y[0][i][t];
+            for (int i = 0; i < b; ++i) v = v - Y[1][i][s] * dB[i] * Y[1][i][t];
+            Sm[s][t] = v;
+            Iv[s][t] = s == t ? 1.0 : 0.0;
+        }
+    }
+    for (int k = 0; k < b; ++k) {
+        __syncthreads();
+        const double piv = Sm[k][k];
+        if (s == 0 && !(piv > 0.0)) atomicOr(D.err, ERR_SOLVE);
+        const double p = 1.0 / piv;
+        __syncthreads();
+        if (s < b) {
+            Sm[k][s] = Sm[k][s] * p;
+            Iv[k][s] = Iv[k][s] * p;
+        }
+        __syncthreads();
+        if (s < b) fcol[s] = Sm[s][k];
+        __syncthreads();
+        if (s < b)
+            for (int i = 0; i < b; ++i)
+                if (i != k) {
+                    Sm[i][s] = Sm[i][s] - fcol[i] * Sm[k][s];
+                    Iv[i][s] = Iv[i][s] - fcol[i] * Iv[k][s];
+                }
+    }
+    __syncthreads();
+    double* Z = S.zsep;
+    if (s < b) {
+        for (int c = 0; c < b3; ++c) {
+            double v;
+            if (c < b) {
+                v = Iv[s][c];
+            } else {
+                const int h = c < 2 * b ? 0 : 1, i = c - (h + 1) * b;
+                v = 0.0;
+                for (int t = 0; t < b; ++t) v = v + Iv[s][t] * Y[h][i][t];
+                v = -v;
+            }
+            Z[(long long)s * b3 + c] = v;
+        }
+        for (int h = 0; h < 2; ++h) {
+            const double di = h == 0 ? dT[s] : dB[s];
+            for (int t = 0; t < b; ++t) Z[(long long)b * b3 + (long long)(h * b + s) * b + t] = di * Y[h][s][t];
+        }
     }
 }
 
@@ -969,6 +1113,238 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
 }
 template __global__ void k_crit_solve<64>(const DevCase*, const CaseClock*, int);
 template __global__ void k_crit_solve<32>(const DevCase*, const CaseClock*, int);
+
+// Separator form of the solve (DevCase::dissect; factors from k_geo_factor2
+// and k_geo_schur).  Wavefront 0 runs the top half's chain, wavefront 1 the
+// reversed bottom half's, each exactly as in k_crit_solve (lane-dealt rows,
+// readlane broadcast, coefficients streamed through its own LDS ring by the
+// producer wavefronts), so the critical path is one half's length.  Between
+// the forward and backward sweeps the separator step applies Z (k_geo_schur)
+// to v = [f_S; both halves' forward results over their last bw rows]: x_S
+// and the two tail corrections.  The producer threads, idle there, hold
+// their slice of Z in registers from the start, form partial sums into the
+// free half of a coefficient ring, and one pass adds them in fixed order;
+// the backward sweeps then give x_T and x_B.
+template <int U>
+__global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCase* __restrict__ C,
+                                                                      const CaseClock* __restrict__ K, int off)
+{
+    constexpr int UPC = 64 / U;              // units per lane chunk
+    constexpr int SPW = U / SV_NPROD;        // steps per producer, half and unit
+    __shared__ double2 ring[2][2 * U * 64];  // per half, two units
+    extern __shared__ double zb[];           // [2 halves][2 rhs][np2], [2][np2] 1/d, then separator scratch
+    const DevCase& D = C[blockIdx.x];
+    if (!D.dissect) return;
+    const int nstep = K[blockIdx.x].n0 + off;
+    if (nstep > K[blockIdx.x].n_end) return;
+    const Slot S = slot_of(D, (nstep - 1) % D.nslot);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ns = D.nseg, b = D.bw, mT = D.mT, mB = D.mB, np = D.np2, nch = np / 64;
+    const int nun = 2 * nch * UPC, F = nch * UPC;
+    double* dv = zb + 4 * np;     // [2][np]
+    double* fs = zb + 6 * np;     // [2 rhs][64] separator right-hand sides
+    double* xs = fs + 128;        // [2 rhs][64] x_S
+    for (int e = tid; e < 4 * np; e += blockDim.x) {
+        const int h = e / (2 * np), r = (e / np) & 1, p = e % np;
+        double v = 0.0;
+        if (h == 0 && p < mT) v = D.rhs[r * ns + p];
+        if (h == 1 && p < mB) v = D.rhs[r * ns + ns - 1 - p];
+        zb[e] = v;
+    }
+    for (int e = tid; e < 2 * np; e += blockDim.x) dv[e] = S.dinv2[e];
+    for (int e = tid; e < 2 * b; e += blockDim.x) fs[(e / b) * 64 + e % b] = D.rhs[(e / b) * ns + mT + e % b];
+    // partial sums of the separator step go to the half of ring 0 that
+    // neither the last forward unit nor the first backward unit uses
+    double* part = (double*)(ring[0] + (((nch * UPC) + 1) & 1) * U * 64);
+    const long long cstride = (long long)(2 * np + 64) * 2 * D.ws;
+    if (wave >= 2) {
+        // ---- producer: unit u, steps u*U + w*SPW + s of both halves ----
+        const int w = wave - 2;
+        const sv_d2* cf0 = (const sv_d2*)S.coef2 + (long long)w * SPW * 64 + lane;
+        const sv_d2* cf1 = (const sv_d2*)(S.coef2 + cstride) + (long long)w * SPW * 64 + lane;
+        sv_d2* rg0 = (sv_d2*)ring[0];
+        sv_d2* rg1 = (sv_d2*)ring[1];
+        sv_d2 A0[SPW], A1[SPW], B0[SPW], B1[SPW];
+        auto load = [&](sv_d2 (&R0)[SPW], sv_d2 (&R1)[SPW], int u) {
+            const long long u_ = min(u, nun - 1);  // units past the end reload the last one, never read
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) {
+                R0[q] = cf0[(u_ * U + q) * 64];
+                R1[q] = cf1[(u_ * U + q) * 64];
+            }
+        };
+        auto store = [&](const sv_d2 (&R0)[SPW], const sv_d2 (&R1)[SPW], int u) {
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) {
+                rg0[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = R0[q];
+                rg1[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = R1[q];
+            }
+        };
+        load(A0, A1, 0);
+        load(B0, B1, 1);
+        store(A0, A1, 0);
+        load(A0, A1, 2);
+        __syncthreads();
+        for (int u = 0; u < nun; u += 2) {
+            store(B0, B1, u + 1);
+            load(B0, B1, u + 3);
+            __syncthreads();
+            if (u == F - 1) { __syncthreads(); __syncthreads(); __syncthreads(); __syncthreads(); }
+            store(A0, A1, u + 2);
+            load(A0, A1, u + 4);
+            __syncthreads();
+            if (u + 1 == F - 1) { __syncthreads(); __syncthreads(); __syncthreads(); __syncthreads(); }
+        }
+    } else {
+        // ---- the two chains; wave h runs half h ----
+        const int h = wave;
+        double* z = zb + 2 * h * np;
+        const double* di = dv + h * np;
+        const double2* rb = ring[h];
+        // this thread's pieces of M (row o1, terms q1*L1 ..) and N (row o2,
+        // terms q2*L2 ..), loaded now and first used after the forward sweeps
+        const int b3 = 3 * b, n1 = sep_seg1(b), L1 = sep_len1(b), n2 = sep_seg2(b), L2 = sep_len2(b);
+        const int o1 = tid / n1, q1 = tid % n1, o2 = tid / n2, q2 = tid % n2;
+        const bool a1 = o1 < b, a2 = o2 < 2 * b;
+        double m1[SEP_L1], m2[SEP_L2];
+#pragma unroll
+        for (int j = 0; j < SEP_L1; ++j) {
+            const int c = q1 * L1 + j;
+            m1[j] = (a1 && j < L1 && c < b3) ? S.zsep[(long long)o1 * b3 + c] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < SEP_L2; ++j) {
+            const int t = q2 * L2 + j;
+            m2[j] = (a2 && j < L2 && t < b) ? S.zsep[(long long)b * b3 + (long long)o2 * b + t] : 0.0;
+        }
+        __syncthreads();
+        double cur[2], nxt[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            cur[r] = z[r * np + lane];
+            nxt[r] = nch > 1 ? z[r * np + 64 + lane] : 0.0;
+        }
+        for (int c = 0; c < nch; ++c) {
+#pragma unroll
+            for (int j = 0; j < UPC; ++j) {
+                const int u = c * UPC + j;
+                sv_unit<true, U>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                if (j == UPC - 1) {
+                    const int row = c * 64 + lane;
+                    const double dd = di[row];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        z[r * np + row] = cur[r] * dd;
+                        cur[r] = nxt[r];
+                        nxt[r] = c + 2 < nch ? z[r * np + row + 128] : 0.0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // ---- separator step (four barriers shared with the producers) ----
+        if (a1) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < SEP_L1; ++j) {
+                    const int c = q1 * L1 + j;
+                    if (j < L1 && c < b3) {
+                        const double v = c < b ? fs[r * 64 + c]
+                                       : c < 2 * b ? zb[r * np + mT - b + (c - b)]
+                                                   : zb[2 * np + r * np + mB - b + (c - 2 * b)];
+                        acc = acc + m1[j] * v;
+                    }
+                }
+                part[(r * b + o1) * n1 + q1] = acc;
+            }
+        }
+        __syncthreads();
+        if (tid < 2 * b) {
+            const int r = tid / b, o = tid % b;
+            double sum = 0.0;
+            for (int q = 0; q < n1; ++q) sum = sum + part[(r * b + o) * n1 + q];
+            xs[r * 64 + o] = sum;
+        }
+        __syncthreads();
+        if (a2) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < SEP_L2; ++j) {
+                    const int t = q2 * L2 + j;
+                    if (j < L2 && t < b) acc = acc + m2[j] * xs[r * 64 + t];
+                }
+                part[512 + (r * 2 * b + o2) * n2 + q2] = acc;
+            }
+        }
+        __syncthreads();
+        if (tid < 4 * b) {
+            const int r = tid / (2 * b), o = tid % (2 * b);
+            double sum = 0.0;
+            for (int q = 0; q < n2; ++q) sum = sum + part[512 + (r * 2 * b + o) * n2 + q];
+            if (o < b) zb[r * np + mT - b + o] -= sum;
+            else zb[2 * np + r * np + mB - b + (o - b)] -= sum;
+        }
+        __syncthreads();
+        // backward; lane l starts on rows (nch-1)*64 + l and (nch-2)*64 + l
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            cur[r] = z[r * np + (nch - 1) * 64 + lane];
+            nxt[r] = nch > 1 ? z[r * np + (nch - 2) * 64 + lane] : 0.0;
+        }
+        for (int c = nch - 1; c >= 0; --c) {
+#pragma unroll
+            for (int j = 0; j < UPC; ++j) {
+                const int u = (2 * nch - 1 - c) * UPC + j;
+                sv_unit<false, U>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                if (j == UPC - 1) {
+                    const int row = c * 64 + lane;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        z[r * np + row] = cur[r];
+                        cur[r] = nxt[r];
+                        nxt[r] = c >= 2 ? z[r * np + row - 128] : 0.0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < ns; p += blockDim.x) {
+        double v0, v1;
+        if (p < mT) {
+            v0 = zb[p];
+            v1 = zb[np + p];
+        } else if (p < mT + b) {
+            v0 = xs[p - mT];
+            v1 = xs[64 + p - mT];
+        } else {
+            const int q = ns - 1 - p;
+            v0 = zb[2 * np + q];
+            v1 = zb[3 * np + q];
+        }
+        const int sg = D.perm[p];
+        D.y[sg] = v0;
+        D.y[ns + sg] = v1;
+    }
+}
+template __global__ void k_crit_solve2<32>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve2<16>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve2<8>(const DevCase*, const CaseClock*, int);
+
+size_t solve2_smem_bytes(int np2) { return sizeof(double) * (6 * (size_t)np2 + 256); }  // values, 1/d, f_S, x_S
+// the largest staging unit whose two rings fit beside the values
+int solve2_unit(int np2)
+{
+    for (int U : {32, 16, 8})
+        if (solve2_smem_bytes(np2) + 2 * 2 * (size_t)U * 64 * 16 <= 160 * 1024) return U;
+    return 0;
+}
+int solve2_threads() { return 64 * (2 + SV_NPROD); }
 
 // The 64-step staging unit when its two LDS buffers (128 KB) fit beside the
 // right-hand sides, the 32-step one otherwise.

@@ -23,6 +23,9 @@ module lbmib
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP64 = 0
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP32DEV = 1
     integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 9
+    integer(c_int32_t), parameter, public :: LBMIB_SOLVER_AUTO = 0
+    integer(c_int32_t), parameter, public :: LBMIB_SOLVER_CHAIN = 1
+    integer(c_int32_t), parameter, public :: LBMIB_SOLVER_SEPARATOR = 2
 
     ! struct lbmib_params (include/lbmib.h)
     type, bind(C), public :: lbmib_params
@@ -31,8 +34,8 @@ module lbmib
         real(c_double) :: xc = 0.0d0, yc = 0.0d0, xp = 0.0d0, yp = 0.0d0
         real(c_double) :: s(9) = 0.0d0
         integer(c_int32_t) :: smooth = 1, precision = 0, n_wings = 1, pts_per_wing = 0
-        integer(c_int32_t) :: device = 0, nan_guard = 0
-        integer(c_int32_t) :: reserved(6) = 0
+        integer(c_int32_t) :: device = 0, nan_guard = 0, ib_solver = 0
+        integer(c_int32_t) :: reserved(5) = 0
     end type
 
     ! struct lbmib_launch_shape (include/lbmib.h): how steps are launched

@@ -38,6 +38,9 @@ LIB_PATH = os.path.join(_HERE, "liblbmib.so")
 
 PREC_FP64 = 0
 PREC_FP32DEV = 1
+SOLVER_AUTO = 0       # LBMIB_SOLVER_*: chain for FP64, separator form for FP32
+SOLVER_CHAIN = 1
+SOLVER_SEPARATOR = 2
 NTIMINGS = 9  # LBMIB_NTIMINGS
 
 
@@ -53,7 +56,7 @@ class Params(C.Structure):
                 ("xc", C.c_double), ("yc", C.c_double), ("xp", C.c_double), ("yp", C.c_double),
                 ("s", C.c_double * 9), ("smooth", C.c_int32), ("precision", C.c_int32),
                 ("n_wings", C.c_int32), ("pts_per_wing", C.c_int32), ("device", C.c_int32),
-                ("nan_guard", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("nan_guard", C.c_int32), ("ib_solver", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class LaunchShape(C.Structure):
@@ -141,7 +144,8 @@ def _check(rc: int):
         raise LBMIBError(rc, load().lbmib_last_error().decode())
 
 
-def make_params(p, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0) -> Params:
+def make_params(p, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0,
+                ib_solver: int = SOLVER_AUTO) -> Params:
     prm = Params()
     prm.nx, prm.ny = p.nx, p.ny
     prm.ox, prm.oy, prm.scale, prm.U = p.ox, p.oy, p.scale, p.U
@@ -150,18 +154,19 @@ def make_params(p, precision: int = PREC_FP64, device: int = 0, nan_guard: int =
         prm.s[k] = p.s[k]
     prm.smooth, prm.precision = p.smooth, precision
     prm.n_wings, prm.pts_per_wing = p.n_wings, p.pts_per_wing
-    prm.device, prm.nan_guard = device, nan_guard
+    prm.device, prm.nan_guard, prm.ib_solver = device, nan_guard, ib_solver
     return prm
 
 
 class LBMIB:
     """One solver case resident on one GPU."""
 
-    def __init__(self, case, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0):
+    def __init__(self, case, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0,
+                 ib_solver: int = SOLVER_AUTO):
         L = load()
         self.case = case
         self.p = case.params
-        prm = make_params(self.p, precision, device, nan_guard)
+        prm = make_params(self.p, precision, device, nan_guard, ib_solver)
         xb = np.ascontiguousarray(case.xb0, dtype=np.float64)
         yb = np.ascontiguousarray(case.yb0, dtype=np.float64)
         h = C.c_void_p()

@@ -45,6 +45,18 @@ extern "C" {
 #define LBMIB_PREC_FP64 0    /* plain FP64 drop-in path */
 #define LBMIB_PREC_FP32DEV 1 /* FP32 populations stored as deviations from w_i*rho0 */
 
+/* The immersed-boundary system G y = b (G = D D^T, banded in zig-zag marker
+ * order) is solved with a banded L D L^T.  CHAIN runs its triangular sweeps
+ * as one sequential chain, in the oracle's operation order (bit-exact);
+ * SEPARATOR splits the band at a separator of bandwidth rows into two halves
+ * whose chains run concurrently, with the separator's small Schur complement
+ * in between (half the chain length, a different round-off order).  AUTO
+ * takes CHAIN for LBMIB_PREC_FP64 and SEPARATOR for LBMIB_PREC_FP32DEV (when
+ * the system has at least 2*64 + bandwidth rows). */
+#define LBMIB_SOLVER_AUTO 0
+#define LBMIB_SOLVER_CHAIN 1
+#define LBMIB_SOLVER_SEPARATOR 2
+
 typedef struct lbmib_params {
     int32_t nx, ny;        /* lattice nodes, mesh%Nx/mesh%Ny (ini_mesh.f90:22-23) */
     double ox, oy;         /* physical position of node (1,1) (ini_mesh.f90:36-37) */
@@ -60,7 +72,8 @@ typedef struct lbmib_params {
     int32_t pts_per_wing;  /* closed polygon points per wing (main.f90:21) */
     int32_t device;        /* HIP device ordinal */
     int32_t nan_guard;     /* 1: check the populations for NaN after every lbmib_step call */
-    int32_t reserved[6];
+    int32_t ib_solver;     /* LBMIB_SOLVER_* */
+    int32_t reserved[5];
 } lbmib_params;
 
 typedef struct lbmib_handle lbmib_handle;

@@ -44,7 +44,7 @@ def test_binding_constants_match_the_header(lbmib_mod):
 
 
 def test_params_struct_layout(lbmib_mod):
-    # 2 ints, 8 doubles, 9 doubles, 6 ints, 6 reserved ints
+    # 2 ints, 8 doubles, 9 doubles, 7 ints, 5 reserved ints
     P = lbmib_mod.Params
     assert P.ox.offset == 8
     assert P.s.offset == 8 + 8 * 8

@@ -73,6 +73,20 @@ def test_alone_equals_mixed(lbmib_mod, prec):
         same(s, alone(lbmib_mod, c, 40, prec, chunks=[7, 33]))
 
 
+def test_separator_and_chain_cases_in_one_batch(lbmib_mod):
+    """FP32 members solved in the separator form (256 markers) and as one
+    chain (64 markers) side by side; each equals itself alone."""
+    cs = [cases.config(1, n_steps=24, scale_down=4), mixed_cases(24)[0]]
+    prec = 1
+    sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    b.step(24)
+    for c, s in zip(cs, sims):
+        same(s, alone(lbmib_mod, c, 24, prec))
+
+
 def test_join_and_leave(lbmib_mod):
     """A joins an empty batch, B joins after 16 steps, A leaves after 32 and
     steps on alone; B leaves after 24 of its own steps."""

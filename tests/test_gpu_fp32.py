@@ -95,3 +95,48 @@ def test_nan_guard_reports_nan(lbmib_mod):
         with pytest.raises(lbmib_mod.LBMIBError) as e:
             g.evolution(1)
         assert e.value.code == -7
+
+
+def test_separator_solve_fp64_matches_the_oracle(oracle_mod, lbmib_mod):
+    """The separator form in FP64: not the oracle's operation order, so
+    checked at tolerances instead of bit for bit: rho and u relative L2
+    1e-12 and the force history 1e-10 (the stated FP64 bars), v relative L2
+    1e-10.  v is near zero away from the body, so its relative norm measures
+    the IB corrections, where the solve's round-off is amplified by the
+    conditioning of D D^T: on this case the oracle's own dense LU and its
+    L D L^T differ by 3.8e-11 in v (1.6e-12 in u, 1.7e-13 in rho)."""
+    case = cases.config(1, n_steps=48, scale_down=4)  # 1024x512, 256 markers
+    g = lbmib_mod.LBMIB(case, ib_solver=lbmib_mod.SOLVER_SEPARATOR)
+    o = oracle_mod.Oracle(case, n_threads=8)
+    print("  separator solve fp64 vs oracle: %s" % case.name, flush=True)
+    g.evolution(48)
+    o.step(48)
+    fg, fo = g.fields(), o.fields()
+    for k, tol in (("rho", 1e-12), ("u", 1e-12), ("v", 1e-10)):
+        assert rel_l2(fg[k], fo[k]) < tol, k
+    hg, ho = g.history(), o.history()
+    cols = [1, 2, 3]
+    assert np.max(np.abs(hg[:, cols] - ho[:, cols])) < 1e-10 * np.max(np.abs(ho[:, cols]))
+    assert not np.array_equal(hg, ho)  # a different operation order did run
+
+
+def test_separator_equals_chain_in_fp32(lbmib_mod):
+    case = cases.config(1, n_steps=40, scale_down=4)
+    a = lbmib_mod.LBMIB(case, precision=lbmib_mod.PREC_FP32DEV, ib_solver=lbmib_mod.SOLVER_CHAIN)
+    b = lbmib_mod.LBMIB(case, precision=lbmib_mod.PREC_FP32DEV)  # AUTO: separator form
+    a.evolution(40)
+    b.evolution(40)
+    ha, hb = a.history(), b.history()
+    cols = [1, 2, 3]
+    assert np.max(np.abs(ha[:, cols] - hb[:, cols])) < 1e-6 * np.max(np.abs(ha[:, cols]))
+    fa, fb = a.fields(), b.fields()
+    for k in ("rho", "u", "v"):
+        assert rel_l2(fb[k], fa[k]) < 1e-6, k
+    assert not np.array_equal(ha, hb)
+
+
+def test_separator_solver_argument_checks(lbmib_mod):
+    small = cases.config(0, n_steps=10, scale_down=4)  # 32 markers: too few rows for the separator
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        lbmib_mod.LBMIB(small, ib_solver=lbmib_mod.SOLVER_SEPARATOR)
+    assert e.value.code == -1
