@@ -39,7 +39,7 @@ using namespace lbmib;
 
 namespace {
 
-constexpr int GEO_BATCH = 8;  // steps per geometry batch (slots = 2 * batch)
+constexpr int GEO_BATCH = LBMIB_GEOMETRY_BATCH;  // slots = 2 * batch
 
 // Gauss-Jordan inverse of the MRT transform as nizhen computes it
 // (solution.f90:1462-1517), row-major.
@@ -496,11 +496,12 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         const int sol = h->p.ib_solver;
         const bool want = sol == LBMIB_SOLVER_SEPARATOR || (sol == LBMIB_SOLVER_AUTO && D.prec == LBMIB_PREC_FP32DEV);
         const int b = D.bw, mT = (D.nseg - b) / 2, mB = D.nseg - b - mT;
-        D.dissect = want && b >= 1 && b <= SEPMAXBW && sep_len1(b) <= SEP_L1 && sep_len2(b) <= SEP_L2 && mT >= b &&
-                    mB >= b && mT >= 64 ? 1 : 0;
+        D.compact = b < CW ? 1 : 0;
+        D.dissect = want && D.compact && b >= 1 && b <= SEPMAXBW && sep_len1(b) <= SEP_L1 && sep_len2(b) <= SEP_L2 &&
+                    mT >= b && mB >= b && mT >= 64 ? 1 : 0;
         D.mT = D.dissect ? mT : 0;
         D.mB = D.dissect ? mB : 0;
-        D.np2 = D.dissect ? (std::max(mT, mB) + 63) / 64 * 64 : 0;
+        D.np2 = D.dissect ? (std::max(mT, mB) + 127) / 128 * 128 : 0;
         if (sol == LBMIB_SOLVER_SEPARATOR && !D.dissect)
             return fail(LBMIB_E_ARG, "the separator solve needs at least 2*64 + bandwidth segments and bandwidth "
                         "<= %d (have %d segments, bandwidth %d)", SEPMAXBW, D.nseg, b);
@@ -543,11 +544,11 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         dd.dval = o; o += al(ns * BOXMAX);
         dd.nmkv = o; o += al(cap * NCAP);
         dd.G = o; o += al(ns * W);
-        dd.coef = o; o += al((long long)(2 * D.npad + 64) * 2 * D.ws);
+        dd.coef = o; o += al(D.compact ? (long long)(2 * D.npad + 256) * CW : (long long)(2 * D.npad + 64) * 2 * D.ws);
         dd.dinv = o; o += al(D.npad);
         {
             const long long b = D.dissect ? D.bw : 0, np2 = D.np2;
-            dd.coef2 = o; o += al(D.dissect ? 2 * (2 * np2 + 64) * 2 * D.ws : 0);
+            dd.coef2 = o; o += al(D.dissect ? 2 * (2 * np2 + 256) * CW : 0);
             dd.dinv2 = o; o += al(2 * np2);
             dd.ltail2 = o; o += al(2 * b * (b + 1));
             dd.zsep = o; o += al(5 * b * b);

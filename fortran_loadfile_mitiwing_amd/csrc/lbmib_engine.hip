@@ -134,15 +134,18 @@ void build_tiles(const Engine* E, std::vector<int2>& t)
 struct Dims {
     int nc = 0, maxns = 0, maxcap = 0, maxW = 0, maxnpad = 0, prec = 0;
     int chain = 0, sep = 0, maxnp2 = 0;  // members solved as one chain / in separator form
+    int plain = 0;                       // chain members with per-lane (not compact) coefficients
 };
 
 const void* solve2_kernel(int np2)
 {
-    switch (solve2_unit(np2)) {
-    case 32: return (const void*)k_crit_solve2<32>;
-    case 16: return (const void*)k_crit_solve2<16>;
-    default: return (const void*)k_crit_solve2<8>;
-    }
+    return solve2_unit(np2) == 32 ? (const void*)k_crit_solve2<32> : (const void*)k_crit_solve2<16>;
+}
+
+const void* solve_kernel(int npad, bool cmp)
+{
+    if (solve_unit(npad) == 64) return cmp ? (const void*)k_crit_solve<64, true> : (const void*)k_crit_solve<64, false>;
+    return cmp ? (const void*)k_crit_solve<32, true> : (const void*)k_crit_solve<32, false>;
 }
 
 Dims dims_of(const Engine* E)
@@ -159,6 +162,7 @@ Dims dims_of(const Engine* E)
             d.maxnp2 = std::max(d.maxnp2, h->D.np2);
         } else {
             d.chain++;
+            if (!h->D.compact) d.plain++;
         }
     }
     d.prec = E->members.empty() ? 0 : E->members[0]->D.prec;
@@ -207,8 +211,9 @@ int engine_prepare(Engine* E)
     const Dims d = dims_of(E);
     HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)factor_smem_bytes()));
-    HIP_TRY(hipFuncSetAttribute(solve_unit(d.maxnpad) == 64 ? (const void*)k_crit_solve<64> : (const void*)k_crit_solve<32>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_smem_bytes(d.maxnpad)));
+    for (bool cmp : {false, true})
+        HIP_TRY(hipFuncSetAttribute(solve_kernel(d.maxnpad, cmp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)solve_smem_bytes(d.maxnpad)));
     if (d.sep) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor2, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)factor_smem_bytes()));
@@ -257,7 +262,7 @@ int launch_factor(Engine* E, const Dims& d, int off, int mode, hipEvent_t* gev)
     if (d.chain) hipLaunchKernelGGL(k_geo_factor, dim3(E->B, d.nc), dim3(256), factor_smem_bytes(), st, C, K, off, mode);
     if (d.sep) {
         hipLaunchKernelGGL(k_geo_factor2, dim3(E->B, d.nc, 2), dim3(256), factor_smem_bytes(), st, C, K, off, mode);
-        hipLaunchKernelGGL(k_geo_schur, dim3(E->B, d.nc), dim3(64), 0, st, C, K, off, mode);
+        hipLaunchKernelGGL(k_geo_schur, dim3(E->B, d.nc), dim3(256), 0, st, C, K, off, mode);
     }
     if (gev) HIP_TRY(hipEventRecord(gev[2], st));
     return 0;
@@ -281,31 +286,42 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     const DevCase* C = E->dcases;
     const CaseClock* K = E->dclock;
     hipStream_t s0 = E->s[0], s1 = E->s[1], s3 = E->s[3];
+    // right-hand sides on the shared CUs (s3), or (shape.rhs) in the solves'
+    // stream, which saves the hand-over between the two streams
+    const bool rr = E->shape.rhs != 0;
+    hipStream_t sr = rr ? s1 : s3;
     if (after_prev) {
-        HIP_TRY(hipStreamWaitEvent(s3, E->ev_step[(gs - 1) % R], 0));
-        HIP_TRY(hipStreamWaitEvent(s3, E->ev_ib[(gs - 1) % R], 0));
+        HIP_TRY(hipStreamWaitEvent(sr, E->ev_step[(gs - 1) % R], 0));
+        if (!rr || E->shape.band) HIP_TRY(hipStreamWaitEvent(sr, E->ev_ib[(gs - 1) % R], 0));
     }
-    if (wfirst) HIP_TRY(hipStreamWaitEvent(s3, wfirst, 0));
-    if (wnext) HIP_TRY(hipStreamWaitEvent(s3, wnext, 0));
-    if (ev) HIP_TRY(hipEventRecord(ev[2], s3));
+    if (wfirst) HIP_TRY(hipStreamWaitEvent(sr, wfirst, 0));
+    if (wnext) HIP_TRY(hipStreamWaitEvent(sr, wnext, 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], sr));
     const bool f64 = d.prec == LBMIB_PREC_FP64;
     const int rhs_x = (d.maxns + 6) / 7;
-    hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3(rhs_x, d.nc), dim3(256), 0, s3, C, K, off);
-    HIP_TRY(hipEventRecord(E->ev_rhs[gs % R], s3));
-    HIP_TRY(hipStreamWaitEvent(s1, E->ev_rhs[gs % R], 0));
-    if (d.chain) {
-        if (solve_unit(d.maxnpad) == 64)
-            hipLaunchKernelGGL(k_crit_solve<64>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
-        else
-            hipLaunchKernelGGL(k_crit_solve<32>, dim3(d.nc), dim3(solve_threads()), solve_smem_bytes(d.maxnpad), s1, C, K, off);
+    hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3(rhs_x, d.nc), dim3(256), 0, sr, C, K, off);
+    if (!rr) {
+        HIP_TRY(hipEventRecord(E->ev_rhs[gs % R], s3));
+        HIP_TRY(hipStreamWaitEvent(s1, E->ev_rhs[gs % R], 0));
+    }
+    // chain members of each coefficient layout (a kernel returns at once for
+    // the other layout's and the separator form's members)
+    const size_t smc = solve_smem_bytes(d.maxnpad);
+    const bool u64 = solve_unit(d.maxnpad) == 64;
+    if (d.chain > d.plain) {
+        if (u64) hipLaunchKernelGGL((k_crit_solve<64, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
+        else hipLaunchKernelGGL((k_crit_solve<32, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
+    }
+    if (d.plain) {
+        if (u64) hipLaunchKernelGGL((k_crit_solve<64, false>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
+        else hipLaunchKernelGGL((k_crit_solve<32, false>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
     }
     if (d.sep) {
         const size_t sm = solve2_smem_bytes(d.maxnp2);
-        switch (solve2_unit(d.maxnp2)) {
-        case 32: hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
-        case 16: hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
-        default: hipLaunchKernelGGL(k_crit_solve2<8>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off); break;
-        }
+        if (solve2_unit(d.maxnp2) == 32)
+            hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
+        else
+            hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
     // the forced collide of the IB nodes follows the solves on their reserved
@@ -566,6 +582,7 @@ int engine_set_shape(Engine* E, const Shape& s)
     if (s.ib_cus < 0 || s.ib_cus % 8) return fail(LBMIB_E_ARG, "ib_cus must be a non-negative multiple of 8");
     if (s.graph != 0 && s.graph != 1) return fail(LBMIB_E_ARG, "graph must be 0 or 1");
     if (s.band != 0 && s.band != 1) return fail(LBMIB_E_ARG, "ib_collide_shared must be 0 or 1");
+    if (s.rhs != 0 && s.rhs != 1) return fail(LBMIB_E_ARG, "ib_rhs_reserved must be 0 or 1");
     // Graph replay is switched off: its first run on the GPU (a three-case
     // batch, replaying captured 16-step windows on CU-masked streams) ended
     // the host process with a segmentation fault whose cause is not known.
@@ -689,6 +706,7 @@ Shape shape_in(const lbmib_launch_shape* s)
     o.ib_cus = s->ib_cus;
     o.graph = s->graph;
     o.band = s->ib_collide_shared;
+    o.rhs = s->ib_rhs_reserved;
     return o;
 }
 
@@ -701,6 +719,7 @@ void shape_out(const Shape& s, lbmib_launch_shape* o)
     o->ib_cus = s.ib_cus;
     o->graph = s.graph;
     o->ib_collide_shared = s.band;
+    o->ib_rhs_reserved = s.rhs;
 }
 
 }  // namespace
@@ -865,21 +884,21 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
     bool firstrec = true;
     // the first geometry batch of the copies, outside every trial
     rc = engine_run(T, T->B);
-    const int timed = (trial_steps + 2 * T->B - 1) / (2 * T->B) * (2 * T->B);
+    const int timed = (trial_steps + T->B - 1) / T->B * T->B;
     auto trial = [&](const Shape& s, double& ms) -> int {
-        // warm-up of one window (captures the graph in graph mode), then timed windows
-        if (clones[0]->steps_done + 2 * T->B + timed > room) return 1;  // out of motion table
+        // warm-up of one geometry batch, then the timed batches
+        if (clones[0]->steps_done + T->B + timed > room) return 1;  // out of motion table
         int r = engine_set_shape(T, s);
         if (r) return r;
-        if ((r = engine_run(T, 2 * T->B))) return r;
+        if ((r = engine_run(T, T->B))) return r;
         auto t0 = std::chrono::steady_clock::now();
         if ((r = engine_run(T, timed))) return r;
         auto t1 = std::chrono::steady_clock::now();
         ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / timed;
         char buf[256];
         snprintf(buf, sizeof buf, "%s{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-                 "\"ib_collide_shared\":%d,\"graph\":%d,\"ms_per_step\":%.5f}", firstrec ? "" : ",", s.npt, s.order,
-                 s.persist, s.ib_cus, s.band, s.graph, ms);
+                 "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"graph\":%d,\"ms_per_step\":%.5f}",
+                 firstrec ? "" : ",", s.npt, s.order, s.persist, s.ib_cus, s.band, s.rhs, s.graph, ms);
         rep += buf;
         firstrec = false;
         return 0;
@@ -909,13 +928,15 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
         if (!rc) {
             const Shape base = cur;
             for (int cus : {0, 8, 16, 32})
-                for (int band : {0, 1}) {
-                    if (cus == base.ib_cus && band == base.band) continue;
-                    Shape s = base;
-                    s.ib_cus = cus;
-                    s.band = band;
-                    if ((rc = consider(s))) break;
-                }
+                for (int band : {0, 1})
+                    for (int rhs : {0, 1}) {
+                        if (rc || (cus == base.ib_cus && band == base.band && rhs == base.rhs)) continue;
+                        Shape s = base;
+                        s.ib_cus = cus;
+                        s.band = band;
+                        s.rhs = rhs;
+                        rc = consider(s);
+                    }
         }
         // graph replay stays out of the trials while it is switched off
     }
@@ -927,8 +948,8 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
     if (rc) return rc;
     char buf[256];
     snprintf(buf, sizeof buf, "],\"best\":{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-             "\"ib_collide_shared\":%d,\"graph\":%d,\"ms_per_step\":%.5f}}", cur.npt, cur.order, cur.persist,
-             cur.ib_cus, cur.band, cur.graph, cur_ms);
+             "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"graph\":%d,\"ms_per_step\":%.5f}}", cur.npt,
+             cur.order, cur.persist, cur.ib_cus, cur.band, cur.rhs, cur.graph, cur_ms);
     rep += buf;
     if (best) shape_out(cur, best);
     if (report && report_len > 0) {

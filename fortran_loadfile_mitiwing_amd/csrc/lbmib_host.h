@@ -57,6 +57,7 @@ struct Shape {
     int ib_cus = 8;
     int graph = 0;
     int band = 0;
+    int rhs = 0;
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in
@@ -65,13 +66,13 @@ struct Shape {
 // default priority:
 //   s0  lattice update (every node but the step's IB nodes, and the ring)
 //   s1  triangular solves (reserved CUs), then the forced collide of the IB
-//       nodes (or on s3, shape.band)
+//       nodes (or on s3, shape.band); with shape.rhs the right-hand sides too
 //   s2  geometry of later batches, forces of earlier ones
 //   s3  right-hand sides of the IB systems
 struct Engine {
     int dev = 0;
     int ncu = 0;
-    int B = 8;                       // steps per geometry batch
+    int B = LBMIB_GEOMETRY_BATCH;    // steps per geometry batch
     Shape shape;
     static constexpr int NSTREAM = 4;
     hipStream_t s[NSTREAM] = {};

@@ -14,14 +14,16 @@ constexpr int KIN = 8;      // per wing per step: sin, cos, omega, sa, sau, sb, 
 constexpr int MAXBW = 62;   // largest bandwidth of the banded IB solver
 constexpr int MAXSEG = 4096;
 // separator form of the IB solve: largest bandwidth, and the slices of the
-// separator matrices M (bw x 3bw) and N (2bw x bw) that each of the 128
-// chain threads holds in registers: row pieces of SEP_L1 and SEP_L2 terms
-constexpr int SEPMAXBW = 32;
-constexpr int SEP_L1 = 24;
-constexpr int SEP_L2 = 16;
-__host__ __device__ inline int sep_seg1(int bw) { return 128 / bw; }
+// separator matrices M (bw x 3bw) and N (2bw x bw) that each of the 512
+// producer threads holds in registers: row pieces of SEP_L1 and SEP_L2 terms
+constexpr int SEPMAXBW = 31;
+// compact solve coefficients: CW values per timeline step (bandwidth < CW)
+constexpr int CW = 32;
+constexpr int SEP_L1 = 8;
+constexpr int SEP_L2 = 8;
+__host__ __device__ inline int sep_seg1(int bw) { return 512 / bw; }
 __host__ __device__ inline int sep_len1(int bw) { return (3 * bw + sep_seg1(bw) - 1) / sep_seg1(bw); }
-__host__ __device__ inline int sep_seg2(int bw) { return 128 / (2 * bw); }
+__host__ __device__ inline int sep_seg2(int bw) { return 512 / (2 * bw); }
 __host__ __device__ inline int sep_len2(int bw) { return (bw + sep_seg2(bw) - 1) / sep_seg2(bw); }
 
 enum ErrBits { ERR_GEOMETRY = 1, ERR_CAPACITY = 2, ERR_SOLVE = 4 };
@@ -51,11 +53,12 @@ struct SlotDoubles {
     long long dval;     // [nseg][BOXMAX] kernel weights phi_x*phi_y, ascending node order
     long long nmkv;     // [cap][NCAP]
     long long G;        // [nseg][bw+1] band of D D^T, G[p][d] = A(p, p-d)
-    long long coef;     // [2*npad+64][ws][2] per-lane coefficients of the solve timeline
+    long long coef;     // [2*npad+64][ws][2] per-lane coefficients of the solve timeline, or in the
+                        // compact layout [2*npad+256][CW]
     long long dinv;     // [npad] 1/d_k of G = L D L^T (zero in the padding)
     // separator form of the solve (DevCase::dissect): two half systems, each
     // factored as above, and the dense Schur complement of the separator
-    long long coef2;    // [2][2*np2+64][ws][2] solve timelines of the two halves
+    long long coef2;    // [2][2*np2+256][CW] solve timelines of the two halves (compact layout)
     long long dinv2;    // [2][np2]
     long long ltail2;   // [2][bw][bw+1] last bw rows of each half's factor, l(i, i-d)
     long long zsep;     // separator step: M [bw][3bw] (x_S = M v), then N [2bw][bw] (tail corrections = N x_S)
@@ -81,6 +84,7 @@ struct DevCase {
     int ws;                 // rows per lane group of the triangular solves (32 or 64)
     int npad;               // nseg rounded up to a multiple of 64
     int dissect;            // 1: solve through the separator form (two concurrent half chains)
+    int compact;            // 1: solve coefficients in the compact layout (bw < CW)
     int mT, mB, np2;        // separator form: rows of the top half, of the bottom half (taken in
                             // reverse), and the halves' padded length
     int steps_cap;          // rows of kin / hist
@@ -176,7 +180,7 @@ __global__ void k_geo_factor2(const DevCase* __restrict__, const CaseClock* __re
 __global__ void k_geo_schur(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 template <typename T>
 __global__ void k_crit_rhs(const DevCase* __restrict__, const CaseClock* __restrict__, int);
-template <int U>
+template <int U, bool CMP>
 __global__ void k_crit_solve(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 int solve_unit(int npad);
 int solve_threads();

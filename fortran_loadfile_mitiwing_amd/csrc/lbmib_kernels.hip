@@ -14,6 +14,7 @@
 #include "lbmib_kernels.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace lbmib {
 
@@ -654,12 +655,16 @@ constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
 // forward step at timeline position k, the backward one at 2*np-1-k:
 //   forward  [h] = {l(k+e1,k), l(k+e2,k)}, e1 = h-q (h > q), e2 = ws+h-q (h <= q)
 //   backward [h] = {l(k,k-e1), l(k,k-e2)}, e1 = q-h (h < q), e2 = ws+q-h (h >= q)
-// with q = k mod ws, zero outside the band.  With ltail, row k's final band
-// (d_k, l(k,k-1), .., l(k,k-bw)) of the last bw rows is kept as well.
-// One workgroup of 256 threads.
+// with q = k mod ws, zero outside the band.  In the compact layout
+// (compact, bw < CW) each timeline row holds CW values instead, entry e the
+// coefficient of the row e away from the pivot: forward l(k+e,k), backward
+// l(k,k-e), entry 0 and entries past the band zero.  With ltail, row k's
+// final band (d_k, l(k,k-1), .., l(k,k-bw)) of the last bw rows is kept as
+// well.  One workgroup of 256 threads.
 template <typename GA>
 __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws, int np, double* __restrict__ coef,
-                                            double* __restrict__ dinv, double* __restrict__ ltail, int* err)
+                                            double* __restrict__ dinv, double* __restrict__ ltail, int* err,
+                                            bool compact)
 {
     extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
@@ -728,7 +733,21 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
         }
         // solve coefficients of step k: column k (final now) and row k (final
         // since step k-1)
-        {
+        if (compact) {
+            // row k (forward) and row 2*np-1-k (backward) of CW values: entry
+            // e is the coefficient of the row e away from the pivot
+            if (tid < CW) {
+                const int e = tid;
+                coef[(long long)k * CW + e] = (e >= 1 && e <= m) ? colv[e - 1] : 0.0;
+            } else if (tid < 2 * CW) {
+                const int e = tid - CW;
+                coef[(long long)(2 * np - 1 - k) * CW + e] = (e >= 1 && e <= min(bw, k)) ? win[sk + e] : 0.0;
+            }
+            if (tid == 0) dinv[k] = rk;
+            if (ltail && k >= n - bw)
+                for (int d = tid; d < W; d += T)
+                    ltail[(long long)(k - (n - bw)) * W + d] = d == 0 ? akk : (d <= k ? win[sk + d] : 0.0);
+        } else {
             const int q = k & (ws - 1);
             const int hh = (tid >> 1) & (ws - 1), j = tid & 1;
             if (tid < 2 * ws) {
@@ -766,7 +785,7 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
     const double* G = S.G;
     const int W = D.bw + 1;
     auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
-    band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err);
+    band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err, D.compact != 0);
 }
 
 // Separator form of the solve.  With bw the band's lower bandwidth, rows
@@ -785,16 +804,16 @@ __global__ __launch_bounds__(256) void k_geo_factor2(const DevCase* __restrict__
     const Slot S = slot_of(D, step % D.nslot);
     const double* G = S.G;
     const int W = D.bw + 1, ns = D.nseg, half = blockIdx.z;
-    double* coef = S.coef2 + (long long)half * (2 * D.np2 + 64) * 2 * D.ws;
+    double* coef = S.coef2 + (long long)half * (2 * D.np2 + 256) * CW;
     double* dinv = S.dinv2 + (long long)half * D.np2;
     double* lt = S.ltail2 + (long long)half * D.bw * W;
     if (half == 0) {
         auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
-        band_factor(ga, D.mT, D.bw, D.ws, D.np2, coef, dinv, lt, D.err);
+        band_factor(ga, D.mT, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
     } else {
         // reversed row r is row ns-1-r; its entry a'(r, r-d) = a(ns-1-r+d, ns-1-r)
         auto ga = [G, W, ns](int r, int d) { return d <= r ? G[(long long)(ns - 1 - r + d) * W + d] : 0.0; };
-        band_factor(ga, D.mB, D.bw, D.ws, D.np2, coef, dinv, lt, D.err);
+        band_factor(ga, D.mB, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
     }
 }
 
@@ -806,95 +825,98 @@ __global__ __launch_bounds__(256) void k_geo_factor2(const DevCase* __restrict__
 // halves' forward results D^-1 L^-1 f over their last bw rows),
 //   x_S = M v,                 M = [S'^-1, -S'^-1 Y_T^T, -S'^-1 Y_B^T]
 //   [w_T; w_B] -= N x_S,       N = [D_T Y_T; D_B Y_B]  (D_* the tail's 1/d)
-// One wavefront per slot; lane s owns column (or row) s.
-__global__ __launch_bounds__(64) void k_geo_schur(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                  int off, int mode)
+// One workgroup of 256 threads per slot.  Every entry is formed by the
+// same operations in the same order whichever thread takes it.
+__global__ __launch_bounds__(256) void k_geo_schur(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                   int off, int mode)
 {
     const DevCase& D = C[blockIdx.y];
     if (!D.dissect) return;
     int step;
     if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
     const Slot S = slot_of(D, step % D.nslot);
-    constexpr int MB = SEPMAXBW;
+    constexpr int MB = SEPMAXBW, T = 256;
     __shared__ double lt[2][MB][MB + 1], Y[2][MB][MB + 1], Sm[MB][MB + 1], Iv[MB][MB + 1];
-    __shared__ double fcol[MB];
+    __shared__ double fcol[MB], dts[2][MB];
     const int b = D.bw, W = b + 1, mT = D.mT, mB = D.mB, np2 = D.np2, b3 = 3 * b;
-    const int s = threadIdx.x;
+    const int tid = threadIdx.x;
     const double* G = S.G;
-    for (int e = s; e < 2 * b * W; e += 64) {
+    for (int e = tid; e < 2 * b * W; e += T) {
         const int h = e / (b * W), r = e % (b * W);
         lt[h][r / W][r % W] = S.ltail2[e];
     }
+    // the tails' 1/d: half 0 rows mT-b.., half 1 rows mB-b.. of its own factor
+    for (int e = tid; e < 2 * b; e += T) {
+        const int h = e / b, i = e % b;
+        dts[h][i] = h == 0 ? S.dinv2[mT - b + i] : S.dinv2[np2 + mB - b + i];
+    }
     __syncthreads();
-    if (s < b) {
-        for (int h = 0; h < 2; ++h) {
-            for (int i = 0; i < b; ++i) {
-                // coupling of the half's row (last bw rows, i) to separator row s
-                double cv;
-                if (h == 0) {
-                    const int d = b + s - i;  // a(mT+s, mT-b+i)
-                    cv = d <= b ? G[(long long)(mT + s) * W + d] : 0.0;
-                } else {
-                    const int d = 2 * b - 1 - i - s;  // a(mT+2b-1-i, mT+s)
-                    cv = d <= b ? G[(long long)(mT + 2 * b - 1 - i) * W + d] : 0.0;
-                }
-                for (int j = 0; j < i; ++j) cv = cv - lt[h][i][i - j] * Y[h][j][s];
-                Y[h][i][s] = cv;
+    if (tid < 2 * b) {
+        // column s of half h: Y = Lt^-1 Ct by forward substitution
+        const int h = tid / b, s = tid % b;
+        for (int i = 0; i < b; ++i) {
+            // coupling of the half's row (last bw rows, i) to separator row s
+            double cv;
+            if (h == 0) {
+                const int d = b + s - i;  // a(mT+s, mT-b+i)
+                cv = d <= b ? G[(long long)(mT + s) * W + d] : 0.0;
+            } else {
+                const int d = 2 * b - 1 - i - s;  // a(mT+2b-1-i, mT+s)
+                cv = d <= b ? G[(long long)(mT + 2 * b - 1 - i) * W + d] : 0.0;
             }
+            for (int j = 0; j < i; ++j) cv = cv - lt[h][i][i - j] * Y[h][j][s];
+            Y[h][i][s] = cv;
         }
     }
     __syncthreads();
-    const double* dT = S.dinv2 + (mT - b);
-    const double* dB = S.dinv2 + np2 + (mB - b);
-    if (s < b) {
-        for (int t = 0; t < b; ++t) {
-            const int hi = max(s, t), lo = min(s, t);
-            double v = G[(long long)(mT + hi) * W + (hi - lo)];
-            for (int i = 0; i < b; ++i) v = v - Y[0][i][s] * dT[i] * Y[0][i][t];
-            for (int i = 0; i < b; ++i) v = v - Y[1][i][s] * dB[i] * Y[1][i][t];
-            Sm[s][t] = v;
-            Iv[s][t] = s == t ? 1.0 : 0.0;
-        }
+    for (int e = tid; e < b * b; e += T) {
+        const int s = e / b, t = e % b;
+        const int hi = max(s, t), lo = min(s, t);
+        double v = G[(long long)(mT + hi) * W + (hi - lo)];
+        for (int i = 0; i < b; ++i) v = v - Y[0][i][s] * dts[0][i] * Y[0][i][t];
+        for (int i = 0; i < b; ++i) v = v - Y[1][i][s] * dts[1][i] * Y[1][i][t];
+        Sm[s][t] = v;
+        Iv[s][t] = s == t ? 1.0 : 0.0;
     }
+    // S'^-1 by Gauss-Jordan without pivoting (S' is SPD)
     for (int k = 0; k < b; ++k) {
         __syncthreads();
         const double piv = Sm[k][k];
-        if (s == 0 && !(piv > 0.0)) atomicOr(D.err, ERR_SOLVE);
+        if (tid == 0 && !(piv > 0.0)) atomicOr(D.err, ERR_SOLVE);
         const double p = 1.0 / piv;
         __syncthreads();
-        if (s < b) {
-            Sm[k][s] = Sm[k][s] * p;
-            Iv[k][s] = Iv[k][s] * p;
+        if (tid < b) {
+            Sm[k][tid] = Sm[k][tid] * p;
+            Iv[k][tid] = Iv[k][tid] * p;
+        } else if (tid >= 64 && tid < 64 + b && tid - 64 != k) {
+            fcol[tid - 64] = Sm[tid - 64][k];  // column k below and above row k
         }
         __syncthreads();
-        if (s < b) fcol[s] = Sm[s][k];
-        __syncthreads();
-        if (s < b)
-            for (int i = 0; i < b; ++i)
-                if (i != k) {
-                    Sm[i][s] = Sm[i][s] - fcol[i] * Sm[k][s];
-                    Iv[i][s] = Iv[i][s] - fcol[i] * Iv[k][s];
-                }
+        for (int e = tid; e < 2 * b * b; e += T) {
+            const int m = e / (b * b), r = e % (b * b), i = r / b, s = r % b;
+            if (i == k) continue;
+            if (m == 0) Sm[i][s] = Sm[i][s] - fcol[i] * Sm[k][s];
+            else Iv[i][s] = Iv[i][s] - fcol[i] * Iv[k][s];
+        }
     }
     __syncthreads();
     double* Z = S.zsep;
-    if (s < b) {
-        for (int c = 0; c < b3; ++c) {
-            double v;
-            if (c < b) {
-                v = Iv[s][c];
-            } else {
-                const int h = c < 2 * b ? 0 : 1, i = c - (h + 1) * b;
-                v = 0.0;
-                for (int t = 0; t < b; ++t) v = v + Iv[s][t] * Y[h][i][t];
-                v = -v;
-            }
-            Z[(long long)s * b3 + c] = v;
+    for (int e = tid; e < b * b3; e += T) {
+        const int s = e / b3, c = e % b3;
+        double v;
+        if (c < b) {
+            v = Iv[s][c];
+        } else {
+            const int h = c < 2 * b ? 0 : 1, i = c - (h + 1) * b;
+            v = 0.0;
+            for (int t = 0; t < b; ++t) v = v + Iv[s][t] * Y[h][i][t];
+            v = -v;
         }
-        for (int h = 0; h < 2; ++h) {
-            const double di = h == 0 ? dT[s] : dB[s];
-            for (int t = 0; t < b; ++t) Z[(long long)b * b3 + (long long)(h * b + s) * b + t] = di * Y[h][s][t];
-        }
+        Z[e] = v;
+    }
+    for (int e = tid; e < 2 * b * b; e += T) {
+        const int h = e / (b * b), r = e % (b * b), s = r / b, t = r % b;
+        Z[(long long)b * b3 + e] = dts[h][s] * Y[h][s][t];
     }
 }
 
@@ -974,7 +996,10 @@ constexpr int SV_NPROD = 8;  // producer wavefronts
 typedef double sv_d2 __attribute__((ext_vector_type(2)));  // register-friendly pair
 constexpr int SV_P = 4;      // steps of coefficients read ahead from LDS, divides U
 
-template <bool FWD, int U>
+// FMA: each update is one fused multiply-add (one rounding) instead of a
+// product and a difference; the separator form, whose operation order differs
+// from the oracle's anyway, takes it for the shorter dependency chain.
+template <bool FWD, int U, bool FMA = false>
 __device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane, int i0, double (&cur)[2],
                                         double (&nxt)[2])
 {
@@ -988,17 +1013,50 @@ __device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane
         const int q = FWD ? i0 + i : 63 - i0 - i;
         const double z0 = readlane_f64(cur[0], q);
         const double z1 = readlane_f64(cur[1], q);
-        cur[0] = cur[0] - c.x * z0;
-        cur[1] = cur[1] - c.x * z1;
-        nxt[0] = nxt[0] - c.y * z0;
-        nxt[1] = nxt[1] - c.y * z1;
+        if constexpr (FMA) {
+            cur[0] = __builtin_fma(-c.x, z0, cur[0]);
+            cur[1] = __builtin_fma(-c.x, z1, cur[1]);
+            nxt[0] = __builtin_fma(-c.y, z0, nxt[0]);
+            nxt[1] = __builtin_fma(-c.y, z1, nxt[1]);
+        } else {
+            cur[0] = cur[0] - c.x * z0;
+            cur[1] = cur[1] - c.x * z1;
+            nxt[0] = nxt[0] - c.y * z0;
+            nxt[1] = nxt[1] - c.y * z1;
+        }
         // keep this step's updates in this step
         asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(nxt[0]), "+v"(nxt[1]));
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-template <int U>
+// Per-lane coefficient pair (c1, c2) of timeline step t for this lane, from
+// the compact row of step t (band_factor's compact layout, np the chain's
+// padded length): the inverse of that layout, so the pair equals the one the
+// per-lane layout stores.
+__device__ __forceinline__ int compact_entry(int t, int np, int lane)
+{
+    const bool fwd = t < np;
+    const int q = fwd ? (t & 63) : 63 - (t & 63);
+    const int e = (fwd ? lane - q : q - lane) & 63;
+    return e < CW ? e : 0;  // entry 0 is always zero
+}
+__device__ __forceinline__ sv_d2 compact_pair(double v, int t, int np, int lane)
+{
+    const bool fwd = t < np;
+    const int q = fwd ? (t & 63) : 63 - (t & 63);
+    const bool act = fwd ? lane > q : lane < q;  // the row lies in the active chunk
+    sv_d2 r;
+    r.x = act ? v : 0.0;
+    r.y = act ? 0.0 : v;
+    return r;
+}
+
+// CMP: the coefficients are in the compact layout (DevCase::compact); the
+// producers fetch those CW values per step (a quarter of the per-lane
+// layout's bytes) and expand them into the per-lane LDS ring, so the chain's
+// instruction stream is the same for both layouts.
+template <int U, bool CMP>
 __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCase* __restrict__ C,
                                                                      const CaseClock* __restrict__ K, int off)
 {
@@ -1007,6 +1065,7 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
     __shared__ double2 ring[2 * U * 64];     // two units
     extern __shared__ double zb[];           // [2][npad] values, then [npad] 1/d
     const DevCase& D = C[blockIdx.x];
+    if (D.dissect || (D.compact != 0) != CMP) return;
     const int nstep = K[blockIdx.x].n0 + off;
     if (nstep > K[blockIdx.x].n_end) return;
     const Slot S = slot_of(D, (nstep - 1) % D.nslot);
@@ -1021,36 +1080,44 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
     if (wave > 0) {
         // ---- producer: unit u, steps u*U + w*SPW + s ----
         const int w = wave - 1;
-        const sv_d2* coef = (const sv_d2*)S.coef + (long long)w * SPW * 64 + lane;
         sv_d2* rg = (sv_d2*)ring;
-        sv_d2 A[SPW], B[SPW];
+        typedef typename std::conditional<CMP, double, sv_d2>::type Val;
+        Val A[SPW], B[SPW];
         // units past the end reload the last one and are never read
-#define SV_LOAD(R, U_)                                                              \
-    {                                                                               \
-        const long long u_ = min((U_), nun - 1);                                    \
-        _Pragma("unroll") for (int s = 0; s < SPW; ++s)                             \
-            R[s] = coef[(u_ * U + s) * 64];                                         \
-    }
-#define SV_STORE(R, U_)                                                             \
-    {                                                                               \
-        _Pragma("unroll") for (int s = 0; s < SPW; ++s)                             \
-            rg[((U_) & 1) * U * 64 + (w * SPW + s) * 64 + lane] = R[s];             \
-    }
-        SV_LOAD(A, 0)
-        SV_LOAD(B, 1)
-        SV_STORE(A, 0)
-        SV_LOAD(A, 2)
+        auto load = [&](Val (&R)[SPW], int u) {
+            const long long u_ = min(u, nun - 1);
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) {
+                const int t = (int)(u_ * U) + w * SPW + q;
+                if constexpr (CMP)
+                    R[q] = S.coef[(long long)t * CW + compact_entry(t, np, lane)];
+                else
+                    R[q] = ((const sv_d2*)S.coef)[(long long)t * 64 + lane];
+            }
+        };
+        auto store = [&](const Val (&R)[SPW], int u) {
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) {
+                const int t = u * U + w * SPW + q;
+                sv_d2 v;
+                if constexpr (CMP) v = compact_pair(R[q], t, np, lane);
+                else v = R[q];
+                rg[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = v;
+            }
+        };
+        load(A, 0);
+        load(B, 1);
+        store(A, 0);
+        load(A, 2);
         __syncthreads();
         for (int u = 0; u < nun; u += 2) {
-            SV_STORE(B, u + 1)
-            SV_LOAD(B, u + 3)
+            store(B, u + 1);
+            load(B, u + 3);
             __syncthreads();
-            SV_STORE(A, u + 2)
-            SV_LOAD(A, u + 4)
+            store(A, u + 2);
+            load(A, u + 4);
             __syncthreads();
         }
-#undef SV_LOAD
-#undef SV_STORE
     } else {
         // ---- the chain ----
         __syncthreads();
@@ -1111,20 +1178,23 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
         D.y[n + s] = zb[np + p];
     }
 }
-template __global__ void k_crit_solve<64>(const DevCase*, const CaseClock*, int);
-template __global__ void k_crit_solve<32>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve<64, false>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve<32, false>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve<64, true>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_solve<32, true>(const DevCase*, const CaseClock*, int);
 
 // Separator form of the solve (DevCase::dissect; factors from k_geo_factor2
-// and k_geo_schur).  Wavefront 0 runs the top half's chain, wavefront 1 the
-// reversed bottom half's, each exactly as in k_crit_solve (lane-dealt rows,
-// readlane broadcast, coefficients streamed through its own LDS ring by the
-// producer wavefronts), so the critical path is one half's length.  Between
-// the forward and backward sweeps the separator step applies Z (k_geo_schur)
-// to v = [f_S; both halves' forward results over their last bw rows]: x_S
-// and the two tail corrections.  The producer threads, idle there, hold
-// their slice of Z in registers from the start, form partial sums into the
-// free half of a coefficient ring, and one pass adds them in fixed order;
-// the backward sweeps then give x_T and x_B.
+// and k_geo_schur; compact coefficients).  Wavefront 0 runs the top half's
+// chain, wavefront 1 the reversed bottom half's, each as in k_crit_solve
+// (per-lane LDS rings that the producer wavefronts fill from the compact
+// rows) but with fused multiply-adds, so the critical path is one half's
+// length at one operation per update.  Between the forward and
+// backward sweeps the separator step (matrices from k_geo_schur) forms
+// x_S = M v from v = [f_S; both halves' forward results over their last bw
+// rows], then the halves' tail corrections N x_S.  The 512 producer threads,
+// idle there, hold their pieces of M and N in registers from the start and
+// write partial sums into the free half of a coefficient ring; the chain
+// threads add them in fixed order.  The backward sweeps then give x_T and x_B.
 template <int U>
 __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCase* __restrict__ C,
                                                                       const CaseClock* __restrict__ K, int off)
@@ -1132,7 +1202,7 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
     constexpr int UPC = 64 / U;              // units per lane chunk
     constexpr int SPW = U / SV_NPROD;        // steps per producer, half and unit
     __shared__ double2 ring[2][2 * U * 64];  // per half, two units
-    extern __shared__ double zb[];           // [2 halves][2 rhs][np2], [2][np2] 1/d, then separator scratch
+    extern __shared__ double zb[];           // [2 halves][2 rhs][np2], [2][np2] 1/d, f_S, x_S
     const DevCase& D = C[blockIdx.x];
     if (!D.dissect) return;
     const int nstep = K[blockIdx.x].n0 + off;
@@ -1155,56 +1225,13 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
     for (int e = tid; e < 2 * b; e += blockDim.x) fs[(e / b) * 64 + e % b] = D.rhs[(e / b) * ns + mT + e % b];
     // partial sums of the separator step go to the half of ring 0 that
     // neither the last forward unit nor the first backward unit uses
-    double* part = (double*)(ring[0] + (((nch * UPC) + 1) & 1) * U * 64);
-    const long long cstride = (long long)(2 * np + 64) * 2 * D.ws;
+    double* part = (double*)(ring[0] + ((F + 1) & 1) * U * 64);
+    const long long cstride = (long long)(2 * np + 256) * CW;
+    const int b3 = 3 * b, n1 = sep_seg1(b), L1 = sep_len1(b), n2 = sep_seg2(b), L2 = sep_len2(b);
     if (wave >= 2) {
-        // ---- producer: unit u, steps u*U + w*SPW + s of both halves ----
-        const int w = wave - 2;
-        const sv_d2* cf0 = (const sv_d2*)S.coef2 + (long long)w * SPW * 64 + lane;
-        const sv_d2* cf1 = (const sv_d2*)(S.coef2 + cstride) + (long long)w * SPW * 64 + lane;
-        sv_d2* rg0 = (sv_d2*)ring[0];
-        sv_d2* rg1 = (sv_d2*)ring[1];
-        sv_d2 A0[SPW], A1[SPW], B0[SPW], B1[SPW];
-        auto load = [&](sv_d2 (&R0)[SPW], sv_d2 (&R1)[SPW], int u) {
-            const long long u_ = min(u, nun - 1);  // units past the end reload the last one, never read
-#pragma unroll
-            for (int q = 0; q < SPW; ++q) {
-                R0[q] = cf0[(u_ * U + q) * 64];
-                R1[q] = cf1[(u_ * U + q) * 64];
-            }
-        };
-        auto store = [&](const sv_d2 (&R0)[SPW], const sv_d2 (&R1)[SPW], int u) {
-#pragma unroll
-            for (int q = 0; q < SPW; ++q) {
-                rg0[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = R0[q];
-                rg1[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = R1[q];
-            }
-        };
-        load(A0, A1, 0);
-        load(B0, B1, 1);
-        store(A0, A1, 0);
-        load(A0, A1, 2);
-        __syncthreads();
-        for (int u = 0; u < nun; u += 2) {
-            store(B0, B1, u + 1);
-            load(B0, B1, u + 3);
-            __syncthreads();
-            if (u == F - 1) { __syncthreads(); __syncthreads(); __syncthreads(); __syncthreads(); }
-            store(A0, A1, u + 2);
-            load(A0, A1, u + 4);
-            __syncthreads();
-            if (u + 1 == F - 1) { __syncthreads(); __syncthreads(); __syncthreads(); __syncthreads(); }
-        }
-    } else {
-        // ---- the two chains; wave h runs half h ----
-        const int h = wave;
-        double* z = zb + 2 * h * np;
-        const double* di = dv + h * np;
-        const double2* rb = ring[h];
-        // this thread's pieces of M (row o1, terms q1*L1 ..) and N (row o2,
-        // terms q2*L2 ..), loaded now and first used after the forward sweeps
-        const int b3 = 3 * b, n1 = sep_seg1(b), L1 = sep_len1(b), n2 = sep_seg2(b), L2 = sep_len2(b);
-        const int o1 = tid / n1, q1 = tid % n1, o2 = tid / n2, q2 = tid % n2;
+        const int w = wave - 2, pt = tid - 128;
+        // pieces of M (row o1, terms q1*L1 ..) and N (row o2, terms q2*L2 ..)
+        const int o1 = pt / n1, q1 = pt % n1, o2 = pt / n2, q2 = pt % n2;
         const bool a1 = o1 < b, a2 = o2 < 2 * b;
         double m1[SEP_L1], m2[SEP_L2];
 #pragma unroll
@@ -1217,6 +1244,93 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
             const int t = q2 * L2 + j;
             m2[j] = (a2 && j < L2 && t < b) ? S.zsep[(long long)b * b3 + (long long)o2 * b + t] : 0.0;
         }
+        auto mid = [&]() {
+            if (a1) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < SEP_L1; ++j) {
+                        const int c = q1 * L1 + j;
+                        if (j < L1 && c < b3) {
+                            const double v = c < b ? fs[r * 64 + c]
+                                           : c < 2 * b ? zb[r * np + mT - b + (c - b)]
+                                                       : zb[2 * np + r * np + mB - b + (c - 2 * b)];
+                            acc = acc + m1[j] * v;
+                        }
+                    }
+                    part[(r * b + o1) * n1 + q1] = acc;
+                }
+            }
+            __syncthreads();
+            __syncthreads();
+            if (a2) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < SEP_L2; ++j) {
+                        const int t = q2 * L2 + j;
+                        if (j < L2 && t < b) acc = acc + m2[j] * xs[r * 64 + t];
+                    }
+                    part[1024 + (r * 2 * b + o2) * n2 + q2] = acc;
+                }
+            }
+            __syncthreads();
+            __syncthreads();
+        };
+        // ---- producer: unit u, steps u*U + w*SPW + q of both halves ----
+        const double* cf[2] = {S.coef2, S.coef2 + cstride};
+        // unit v sits in register set v % NB from its load, NB units ahead
+        // of its use, until it is stored to the ring just before barrier v
+        constexpr int NB = 4;
+        double R[NB][2][SPW];
+        auto load = [&](double (&R)[2][SPW], int u) {
+            const long long u_ = min(u, nun - 1);  // units past the end reload the last one, never read
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < SPW; ++q) {
+                    const int t = (int)(u_ * U) + w * SPW + q;
+                    R[h][q] = cf[h][(long long)t * CW + compact_entry(t, np, lane)];
+                }
+        };
+        auto store = [&](const double (&R)[2][SPW], int u) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                sv_d2* rg = (sv_d2*)ring[h];
+#pragma unroll
+                for (int q = 0; q < SPW; ++q) {
+                    const int t = u * U + w * SPW + q;
+                    rg[(u & 1) * U * 64 + (w * SPW + q) * 64 + lane] = compact_pair(R[h][q], t, np, lane);
+                }
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < NB; ++k) load(R[k], k);
+        store(R[0], 0);
+        load(R[0], NB);
+        __syncthreads();
+        // barriers 1 .. nun, as the chains take them (unit nun is a copy of
+        // the last one and never read)
+        for (int u = 1; u <= nun; u += NB) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int v = u + k;
+                if (v <= nun) {
+                    store(R[(1 + k) % NB], v);
+                    load(R[(1 + k) % NB], v + NB);
+                    __syncthreads();
+                    if (v == F) mid();
+                }
+            }
+        }
+    } else {
+        // ---- the two chains; wave h runs half h ----
+        const int h = wave;
+        double* z = zb + 2 * h * np;
+        const double* di = dv + h * np;
+        const double2* rb = ring[h];
         __syncthreads();
         double cur[2], nxt[2];
 #pragma unroll
@@ -1228,7 +1342,7 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = c * UPC + j;
-                sv_unit<true, U>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<true, U, true>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
                     const double dd = di[row];
@@ -1242,24 +1356,7 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
                 __syncthreads();
             }
         }
-        // ---- separator step (four barriers shared with the producers) ----
-        if (a1) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                double acc = 0.0;
-#pragma unroll
-                for (int j = 0; j < SEP_L1; ++j) {
-                    const int c = q1 * L1 + j;
-                    if (j < L1 && c < b3) {
-                        const double v = c < b ? fs[r * 64 + c]
-                                       : c < 2 * b ? zb[r * np + mT - b + (c - b)]
-                                                   : zb[2 * np + r * np + mB - b + (c - 2 * b)];
-                        acc = acc + m1[j] * v;
-                    }
-                }
-                part[(r * b + o1) * n1 + q1] = acc;
-            }
-        }
+        // ---- separator step, chain side (four barriers shared with the producers) ----
         __syncthreads();
         if (tid < 2 * b) {
             const int r = tid / b, o = tid % b;
@@ -1268,23 +1365,11 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
             xs[r * 64 + o] = sum;
         }
         __syncthreads();
-        if (a2) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                double acc = 0.0;
-#pragma unroll
-                for (int j = 0; j < SEP_L2; ++j) {
-                    const int t = q2 * L2 + j;
-                    if (j < L2 && t < b) acc = acc + m2[j] * xs[r * 64 + t];
-                }
-                part[512 + (r * 2 * b + o2) * n2 + q2] = acc;
-            }
-        }
         __syncthreads();
         if (tid < 4 * b) {
             const int r = tid / (2 * b), o = tid % (2 * b);
             double sum = 0.0;
-            for (int q = 0; q < n2; ++q) sum = sum + part[512 + (r * 2 * b + o) * n2 + q];
+            for (int q = 0; q < n2; ++q) sum = sum + part[1024 + (r * 2 * b + o) * n2 + q];
             if (o < b) zb[r * np + mT - b + o] -= sum;
             else zb[2 * np + r * np + mB - b + (o - b)] -= sum;
         }
@@ -1299,7 +1384,7 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = (2 * nch - 1 - c) * UPC + j;
-                sv_unit<false, U>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<false, U, true>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
 #pragma unroll
@@ -1334,13 +1419,12 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
 }
 template __global__ void k_crit_solve2<32>(const DevCase*, const CaseClock*, int);
 template __global__ void k_crit_solve2<16>(const DevCase*, const CaseClock*, int);
-template __global__ void k_crit_solve2<8>(const DevCase*, const CaseClock*, int);
 
 size_t solve2_smem_bytes(int np2) { return sizeof(double) * (6 * (size_t)np2 + 256); }  // values, 1/d, f_S, x_S
-// the largest staging unit whose two rings fit beside the values
+// the largest staging unit whose two per-lane rings fit beside the values
 int solve2_unit(int np2)
 {
-    for (int U : {32, 16, 8})
+    for (int U : {32, 16})
         if (solve2_smem_bytes(np2) + 2 * 2 * (size_t)U * 64 * 16 <= 160 * 1024) return U;
     return 0;
 }
@@ -1460,17 +1544,34 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
         }
         __syncthreads();
         if (tid == 0) {
-            for (int q = 0; q < cnt; ++q) {
-                int wq = wing[q];
-                if (wq != curw) {
-                    flush(curw);
-                    for (int x = curw + 1; x < wq; ++x) { F0 = F1 = M = 0.0; flush(x); }
-                    curw = wq;
-                    F0 = F1 = M = 0.0;
+            // ascending node order; the terms of PF nodes are read ahead of
+            // their additions so that the LDS latency overlaps the sums
+            constexpr int PF = 8;
+            for (int q0 = 0; q0 < cnt; q0 += PF) {
+                double t0[PF], t1[PF], t2[PF], t3[PF];
+                int wq[PF];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    const int q = min(q0 + u, cnt - 1);
+                    t0[u] = terms[q][0];
+                    t1[u] = terms[q][1];
+                    t2[u] = terms[q][2];
+                    t3[u] = terms[q][3];
+                    wq[u] = wing[q];
                 }
-                F0 = F0 - terms[q][0];
-                F1 = F1 - terms[q][1];
-                M = M - terms[q][2] + terms[q][3];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    if (q0 + u >= cnt) break;
+                    if (wq[u] != curw) {
+                        flush(curw);
+                        for (int x = curw + 1; x < wq[u]; ++x) { F0 = F1 = M = 0.0; flush(x); }
+                        curw = wq[u];
+                        F0 = F1 = M = 0.0;
+                    }
+                    F0 = F0 - t0[u];
+                    F1 = F1 - t1[u];
+                    M = M - t2[u] + t3[u];
+                }
             }
         }
         __syncthreads();

@@ -23,6 +23,7 @@ module lbmib
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP64 = 0
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP32DEV = 1
     integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 9
+    integer(c_int32_t), parameter, public :: LBMIB_GEOMETRY_BATCH = 16
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_AUTO = 0
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_CHAIN = 1
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_SEPARATOR = 2
@@ -41,8 +42,8 @@ module lbmib
     ! struct lbmib_launch_shape (include/lbmib.h): how steps are launched
     type, bind(C), public :: lbmib_launch_shape
         integer(c_int32_t) :: nodes_per_thread = 1, tile_order = 0, persistent = 0
-        integer(c_int32_t) :: ib_cus = 8, graph = 0, ib_collide_shared = 0
-        integer(c_int32_t) :: reserved(2) = 0
+        integer(c_int32_t) :: ib_cus = 8, graph = 0, ib_collide_shared = 0, ib_rhs_reserved = 0
+        integer(c_int32_t) :: reserved(1) = 0
     end type
 
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps

@@ -38,6 +38,7 @@ LIB_PATH = os.path.join(_HERE, "liblbmib.so")
 
 PREC_FP64 = 0
 PREC_FP32DEV = 1
+GEOMETRY_BATCH = 16  # steps per geometry batch (LBMIB_GEOMETRY_BATCH)
 SOLVER_AUTO = 0       # LBMIB_SOLVER_*: chain for FP64, separator form for FP32
 SOLVER_CHAIN = 1
 SOLVER_SEPARATOR = 2
@@ -62,17 +63,19 @@ class Params(C.Structure):
 class LaunchShape(C.Structure):
     _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
                 ("ib_cus", C.c_int32), ("graph", C.c_int32), ("ib_collide_shared", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("ib_rhs_reserved", C.c_int32), ("reserved", C.c_int32 * 1)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph",
-                                                    "ib_collide_shared")}
+                                                    "ib_collide_shared", "ib_rhs_reserved")}
 
 
-def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0, ib_collide_shared=0) -> LaunchShape:
+def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0, ib_collide_shared=0,
+               ib_rhs_reserved=0) -> LaunchShape:
     s = LaunchShape()
     s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
     s.ib_cus, s.graph, s.ib_collide_shared = ib_cus, graph, ib_collide_shared
+    s.ib_rhs_reserved = ib_rhs_reserved
     return s
 
 

@@ -15,8 +15,9 @@ of them in one launch.
 * An admitted case is created on the device, joins the batch at its step 0
   and stays resident until its last step; finished cases leave the batch at
   the end of the window in which they finish, and their force history is
-  the result.  Cases of one batch join at multiples of 8 steps (the geometry
-  batch), which a window of a multiple of 8 steps guarantees.
+  the result.  Cases of one batch join at multiples of
+  ``lbmib.GEOMETRY_BATCH`` steps (the geometry batch), which a window of a
+  multiple of that many steps guarantees.
 * A case's result does not depend on what else runs: the per-case
   arithmetic is the single-case path's (checked bit for bit by the GPU tests).
 
@@ -116,8 +117,8 @@ class Server:
 
     def __init__(self, backend, hbm_budget: int, max_cases: int = 256, window: int = 32,
                  clock: Callable[[], float] = time.perf_counter):
-        if window <= 0 or window % 8:
-            raise ValueError("window must be a positive multiple of 8 steps")
+        if window <= 0 or window % lbmib.GEOMETRY_BATCH:
+            raise ValueError("window must be a positive multiple of %d steps" % lbmib.GEOMETRY_BATCH)
         self.be = backend
         self.budget = int(hbm_budget)
         self.max_cases = int(max_cases)

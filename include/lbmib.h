@@ -45,6 +45,11 @@ extern "C" {
 #define LBMIB_PREC_FP64 0    /* plain FP64 drop-in path */
 #define LBMIB_PREC_FP32DEV 1 /* FP32 populations stored as deviations from w_i*rho0 */
 
+/* Steps per geometry batch: the immersed-boundary geometry (markers, node
+ * set, interpolation weights, band of D D^T and its factors) of this many
+ * steps is built in one launch, a batch ahead of the steps that use it. */
+#define LBMIB_GEOMETRY_BATCH 16
+
 /* The immersed-boundary system G y = b (G = D D^T, banded in zig-zag marker
  * order) is solved with a banded L D L^T.  CHAIN runs its triangular sweeps
  * as one sequential chain, in the oracle's operation order (bit-exact);
@@ -185,10 +190,12 @@ typedef struct lbmib_launch_shape {
     int32_t graph;            /* 1: windows of 16 steps captured once as a HIP graph and replayed */
     int32_t ib_collide_shared; /* 1: the forced collide of the IB nodes runs on the shared CUs, not the
                                   solves' reserved ones */
-    int32_t reserved[2];
+    int32_t ib_rhs_reserved;  /* 1: the right-hand sides of the IB systems run in the solves' stream (on
+                                 their reserved CUs), not beside the lattice update */
+    int32_t reserved[1];
 } lbmib_launch_shape;
 
-/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0, 0; the
+/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0, 0, 0; the
  * environment variable LBMIB_IB_CUS overrides ib_cus). */
 int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
 int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);
@@ -199,8 +206,8 @@ typedef struct lbmib_batch lbmib_batch;
 int lbmib_batch_create(int32_t device, lbmib_batch** out);
 
 /* Admits a case (motion table set, same device and precision as the other
- * members, and at the same step modulo 8 -- a geometry batch -- as them,
- * e.g. a fresh case beside members at multiples of 8 steps). */
+ * members, and at the same step modulo LBMIB_GEOMETRY_BATCH as them, e.g.
+ * a fresh case beside members at multiples of LBMIB_GEOMETRY_BATCH steps). */
 int lbmib_batch_add(lbmib_batch* b, lbmib_handle* h);
 
 /* Retires a case from the batch (it may then step alone again). */

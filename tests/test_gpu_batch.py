@@ -132,6 +132,8 @@ SHAPES = [
     dict(ib_cus=16),
     dict(ib_collide_shared=1),
     dict(ib_collide_shared=1, ib_cus=0, tile_order=1),
+    dict(ib_rhs_reserved=1),
+    dict(ib_rhs_reserved=1, ib_collide_shared=1, ib_cus=16),
     pytest.param(dict(graph=1), marks=pytest.mark.skip(reason=GRAPH_OFF)),
     pytest.param(dict(graph=1, nodes_per_thread=2, tile_order=1, persistent=8), marks=pytest.mark.skip(reason=GRAPH_OFF)),
 ]
@@ -181,7 +183,7 @@ def test_graph_replay_single_handle_matches_the_oracle(oracle_mod, lbmib_mod):
 
 
 def test_autotune_picks_a_shape_and_keeps_results(lbmib_mod):
-    cs = mixed_cases(1200)  # 23 trials of 16 + 32 steps on the copies
+    cs = mixed_cases(1200)  # 31 trials of 16 + 16 steps on the copies
     prec = 1
     sims = [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
     b = lbmib_mod.Batch()
@@ -189,7 +191,7 @@ def test_autotune_picks_a_shape_and_keeps_results(lbmib_mod):
         b.add(s)
     rep = b.autotune(trial_steps=16)
     print("  autotune best:", rep["best"], "trials:", len(rep["trials"]), flush=True)
-    assert len(rep["trials"]) >= 16 + 7
+    assert len(rep["trials"]) >= 16 + 15
     assert rep["best"]["ms_per_step"] > 0
     assert all(s.steps_done == 0 for s in sims)  # the tuner ran on copies
     b.step(40)

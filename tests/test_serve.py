@@ -4,7 +4,7 @@ behaviour (results alone == in the mix) is in test_gpu_serve.py."""
 import numpy as np
 import pytest
 
-from fortran_loadfile_mitiwing_amd import cases
+from fortran_loadfile_mitiwing_amd import cases, lbmib
 from fortran_loadfile_mitiwing_amd.serve import Server
 
 
@@ -40,7 +40,8 @@ class FakeBackend:
 
     def add(self, sim):
         for m in self.members:
-            assert m.done % 8 == sim.done % 8, "joins must stay aligned to the geometry batch"
+            assert m.done % lbmib.GEOMETRY_BATCH == sim.done % lbmib.GEOMETRY_BATCH, \
+                "joins must stay aligned to the geometry batch"
         self.members.append(sim)
         self.log.append(("add", sim.case.name))
 
@@ -113,13 +114,13 @@ def test_cases_retire_when_their_steps_are_done_and_free_memory():
 
 def test_max_cases_and_window_checks():
     be = FakeBackend()
-    s = Server(be, hbm_budget=10 ** 9, max_cases=2, window=8, clock=Clock())
+    s = Server(be, hbm_budget=10 ** 9, max_cases=2, window=16, clock=Clock())
     for k in range(5):
         s.submit(FakeCase("c%d" % k, 512, 8, 1), tag=k)
     s.admit()
     assert len(s.running) == 2
     with pytest.raises(ValueError):
-        Server(be, hbm_budget=1, window=12)
+        Server(be, hbm_budget=1, window=8)
     with pytest.raises(ValueError):
         s.submit(FakeCase("huge", 512, 8, 10 ** 10))
 
