@@ -310,7 +310,7 @@ def main():
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
     ap.add_argument("--shape", default="",
                     help="launch shape of the step form, e.g. nodes_per_thread=2,tile_order=1,persistent=4,"
-                         "ib_cus=8,ib_collide_shared=0 (default: the library's)")
+                         "ib_cus=16,ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
     ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
                     help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
     ap.add_argument("--form", choices=["step", "batch", "serve"], default="step",

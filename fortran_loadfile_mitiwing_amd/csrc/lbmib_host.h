@@ -54,10 +54,10 @@ struct Shape {
     int npt = 1;
     int order = 0;
     int persist = 0;
-    int ib_cus = 8;
+    int ib_cus = 16;
     int graph = 0;
     int band = 0;
-    int rhs = 0;
+    int rhs = 1;
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in

@@ -70,8 +70,8 @@ class LaunchShape(C.Structure):
                                                     "ib_collide_shared", "ib_rhs_reserved")}
 
 
-def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=8, graph=0, ib_collide_shared=0,
-               ib_rhs_reserved=0) -> LaunchShape:
+def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=16, graph=0, ib_collide_shared=0,
+               ib_rhs_reserved=1) -> LaunchShape:
     s = LaunchShape()
     s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
     s.ib_cus, s.graph, s.ib_collide_shared = ib_cus, graph, ib_collide_shared

@@ -195,7 +195,7 @@ typedef struct lbmib_launch_shape {
     int32_t reserved[1];
 } lbmib_launch_shape;
 
-/* Launch shape of a single handle's steps (default 1, 0, 0, 8, 0, 0, 0; the
+/* Launch shape of a single handle's steps (default 1, 0, 0, 16, 0, 0, 1; the
  * environment variable LBMIB_IB_CUS overrides ib_cus). */
 int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
 int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);

@@ -129,10 +129,10 @@ SHAPES = [
     dict(tile_order=1),
     dict(persistent=4),
     dict(nodes_per_thread=2, tile_order=1, persistent=2, ib_cus=0),
-    dict(ib_cus=16),
+    dict(ib_cus=8, ib_rhs_reserved=0),
     dict(ib_collide_shared=1),
     dict(ib_collide_shared=1, ib_cus=0, tile_order=1),
-    dict(ib_rhs_reserved=1),
+    dict(ib_rhs_reserved=0),
     dict(ib_rhs_reserved=1, ib_collide_shared=1, ib_cus=16),
     pytest.param(dict(graph=1), marks=pytest.mark.skip(reason=GRAPH_OFF)),
     pytest.param(dict(graph=1, nodes_per_thread=2, tile_order=1, persistent=8), marks=pytest.mark.skip(reason=GRAPH_OFF)),
