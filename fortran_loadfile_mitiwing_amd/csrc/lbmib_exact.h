@@ -203,6 +203,7 @@ struct DevConst {
 // 1/6, 1/12, 1/6, 1/12, 1/4, 1/4).
 LBM_HD void mrt_inverse_f32(const float* m, float* f)
 {
+#pragma clang fp contract(fast)
     const float a0 = m[0] * (1.0f / 9.0f);
     const float r1 = m[1] * (1.0f / 36.0f), r2 = m[2] * (1.0f / 18.0f);
     const float p3 = m[3] * (1.0f / 6.0f), p4 = m[4] * (1.0f / 6.0f);
@@ -220,11 +221,29 @@ LBM_HD void mrt_inverse_f32(const float* m, float* f)
     f[8] = a0 + s1 + s2 + p3 + q4 - p5 - q6 - p8;
 }
 
-// Unforced collision of one node in deviation form; returns drho, jx, jy.
+// M f in FP32 (the rows of mrt_forward), products and sums fused where they
+// meet: the FP32 form's collision has no operation order to reproduce.
+LBM_HD void mrt_forward_f32(const float* f, float* m)
+{
+#pragma clang fp contract(fast)
+    m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    m[1] = -4 * f[0] - f[1] - f[2] - f[3] - f[4] + 2 * f[5] + 2 * f[6] + 2 * f[7] + 2 * f[8];
+    m[2] = 4 * f[0] - 2 * f[1] - 2 * f[2] - 2 * f[3] - 2 * f[4] + f[5] + f[6] + f[7] + f[8];
+    m[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    m[4] = -2 * f[1] + 2 * f[3] + f[5] - f[6] - f[7] + f[8];
+    m[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    m[6] = -2 * f[2] + 2 * f[4] + f[5] + f[6] - f[7] - f[8];
+    m[7] = f[1] - f[2] + f[3] - f[4];
+    m[8] = f[5] - f[6] + f[7] - f[8];
+}
+
+// Unforced collision of one node in deviation form, with fused multiply-adds;
+// returns drho, jx, jy.
 LBM_HD void collide_dev_f32(float* g, const DevConst& K, float& drho, float& jx, float& jy)
 {
+#pragma clang fp contract(fast)
     float m[9], t[9];
-    mrt_forward(g, m);
+    mrt_forward_f32(g, m);
     drho = m[0];
     jx = m[3];
     jy = m[5];

@@ -979,19 +979,19 @@ template __global__ void k_crit_rhs<float>(const DevCase*, const CaseClock*, int
 // Step k (q = k mod 64) takes the finished value of row k from lane q by
 // readlane and every lane applies
 //   cur -= c1 * z_k,   nxt -= c2 * z_k
-// with the two coefficients k_geo_factor laid out for this lane (zero outside
-// the band).  A zero coefficient leaves a finite value unchanged, so every
+// with this lane's two coefficients of step k (zero outside the band).  A zero coefficient leaves a finite value unchanged, so every
 // row receives exactly the oracle's updates in increasing (forward) /
 // decreasing (backward) k, and the step has no lane-dependent control flow.
 //
 // The 2*npad steps form one timeline (forward steps, then backward steps);
-// their coefficients lie in that order in coef, 1 KB per step.  Wavefront 0
+// their coefficients lie in that order in coef, 1 KB per step as per-lane
+// pairs or 256 B per step in the compact layout.  Wavefront 0
 // runs the steps and touches only LDS and registers: a global load in its
 // instruction stream would wait for the CU's vector-memory queue, which the
 // lattice kernel keeps full, and stall the whole dependency chain.  The
 // SV_NPROD producer wavefronts stream units of U steps from HBM into a
-// double-buffered LDS ring, loading each unit two units before it is used;
-// one workgroup barrier per unit hands the buffers over.
+// double-buffered LDS ring of per-lane pairs, loading each unit two units
+// before it is used; one workgroup barrier per unit hands the buffers over.
 constexpr int SV_NPROD = 8;  // producer wavefronts
 typedef double sv_d2 __attribute__((ext_vector_type(2)));  // register-friendly pair
 constexpr int SV_P = 4;      // steps of coefficients read ahead from LDS, divides U
@@ -1497,10 +1497,12 @@ template __global__ void k_band<float>(const DevCase*, const CaseClock*, int);
 
 // Per-wing force and moment, accumulated in node order
 // (boundary_condition.f90:1662-1675), and their normalisation
-// (solution.f90:327-330).  One workgroup per step of a batch: 256 threads
-// form the per-node terms of a chunk into LDS, one thread advances the
-// running sums node by node.  The nodes of a wing are contiguous in the list
-// (segment-major order).  Block (x, c) takes step n0 + off + x of case c.
+// (solution.f90:327-330).  One workgroup per step of a batch.  The nodes'
+// terms are formed in chunks of FCH nodes into one of two LDS buffers by
+// wavefronts 1..3 while thread 0 advances the running sums node by node over
+// the other buffer, so the serial sums overlap the loads.  The nodes of a
+// wing are contiguous in the list (segment-major order).  Block (x, c) takes
+// step n0 + off + x of case c.
 constexpr int FCH = 1024;
 __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off)
 {
@@ -1509,11 +1511,52 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     if (n > K[blockIdx.y].n_end) return;
     const int step = n - 1;  // 0-based
     const Slot S = slot_of(D, step % D.nslot);
-    __shared__ double terms[FCH][4];
-    __shared__ int wing[FCH];
+    __shared__ double terms[2][FCH][4];
+    __shared__ int wing[2][FCH];
+    __shared__ double sab[8][2];  // body offsets sa, sb of each wing at this step
     const int tid = threadIdx.x;
     const int na = *S.nadj;
     const double sc = D.scale;
+    if (tid < D.nw) {
+        const double* kn = D.kin + ((long long)step * D.nw + tid) * KIN;
+        sab[tid][0] = kn[3];
+        sab[tid][1] = kn[5];
+    }
+    __syncthreads();
+    // terms of chunk ch into buffer ch & 1, by threads t0 .. t0 + nt - 1
+    auto form = [&](int ch, int t0, int nt) {
+        const int base = ch * FCH, cnt = min(FCH, na - base), b = ch & 1;
+        constexpr int UN = 4;
+        for (int q0 = tid - t0; q0 < cnt; q0 += UN * nt) {
+            int ax[UN], ay[UN], aw[UN];
+            double rho[UN], gx[UN], gy[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const int q = min(q0 + u * nt, cnt - 1), a = base + q;
+                ax[u] = S.adj[3 * a];
+                ay[u] = S.adj[3 * a + 1];
+                aw[u] = S.adj[3 * a + 2];
+                rho[u] = S.rhoadj[a];
+                gx[u] = S.glx[a];
+                gy[u] = S.gly[a];
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const int q = q0 + u * nt;
+                if (q >= cnt) break;
+                const int w = aw[u];
+                const double t1 = 2.0 * rho[u] * gx[u];
+                const double t2 = 2.0 * rho[u] * gy[u];
+                const double x0 = (D.ox + sc * (double)ax[u] - (D.xc + sab[w][0])) / sc;
+                const double y0 = (D.oy + sc * (double)ay[u] - (D.yc + sab[w][1])) / sc;
+                terms[b][q][0] = t1;
+                terms[b][q][1] = t2;
+                terms[b][q][2] = x0 * t2;
+                terms[b][q][3] = y0 * t1;
+                wing[b][q] = w;
+            }
+        }
+    };
     double F0 = 0.0, F1 = 0.0, M = 0.0;
     int curw = 0;
     double* hist = D.hist + (long long)step * 3 * D.nw;
@@ -1523,29 +1566,16 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
         hist[3 * w + 1] = 2.0 * F1 / (U * U * (1.0 / sc));
         hist[3 * w + 2] = -2.0 * M / (U * U * (1.0 / (sc * sc)));
     };
-    for (int base = 0; base < na; base += FCH) {
-        const int cnt = min(FCH, na - base);
-        for (int q = tid; q < cnt; q += 256) {
-            int a = base + q;
-            const int* ad = S.adj + 3 * a;
-            int w = ad[2];
-            const double* kn = D.kin + ((long long)step * D.nw + w) * KIN;
-            double sa = kn[3], sb = kn[5];
-            double rho = S.rhoadj[a];
-            double t1 = 2.0 * rho * S.glx[a];
-            double t2 = 2.0 * rho * S.gly[a];
-            double x0 = (D.ox + sc * (double)ad[0] - (D.xc + sa)) / sc;
-            double y0 = (D.oy + sc * (double)ad[1] - (D.yc + sb)) / sc;
-            terms[q][0] = t1;
-            terms[q][1] = t2;
-            terms[q][2] = x0 * t2;
-            terms[q][3] = y0 * t1;
-            wing[q] = w;
-        }
-        __syncthreads();
-        if (tid == 0) {
+    const int nch = (na + FCH - 1) / FCH;
+    if (nch > 0) form(0, 0, 256);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        if (tid >= 64) {
+            if (ch + 1 < nch) form(ch + 1, 64, 192);
+        } else if (tid == 0) {
             // ascending node order; the terms of PF nodes are read ahead of
             // their additions so that the LDS latency overlaps the sums
+            const int cnt = min(FCH, na - ch * FCH), b = ch & 1;
             constexpr int PF = 8;
             for (int q0 = 0; q0 < cnt; q0 += PF) {
                 double t0[PF], t1[PF], t2[PF], t3[PF];
@@ -1553,11 +1583,11 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
 #pragma unroll
                 for (int u = 0; u < PF; ++u) {
                     const int q = min(q0 + u, cnt - 1);
-                    t0[u] = terms[q][0];
-                    t1[u] = terms[q][1];
-                    t2[u] = terms[q][2];
-                    t3[u] = terms[q][3];
-                    wq[u] = wing[q];
+                    t0[u] = terms[b][q][0];
+                    t1[u] = terms[b][q][1];
+                    t2[u] = terms[b][q][2];
+                    t3[u] = terms[b][q][3];
+                    wq[u] = wing[b][q];
                 }
 #pragma unroll
                 for (int u = 0; u < PF; ++u) {
