@@ -41,6 +41,31 @@ def test_binding_constants_match_the_header(lbmib_mod):
     assert d["LBMIB_PREC_FP32DEV"] == lbmib_mod.PREC_FP32DEV
     assert d["LBMIB_NTIMINGS"] == lbmib_mod.NTIMINGS
     assert d["LBMIB_E_ARG"] == -1 and d["LBMIB_E_HIP"] == -2
+    assert d["LBMIB_GEOMETRY_BATCH"] == lbmib_mod.GEOMETRY_BATCH
+    for k in ("AUTO", "CHAIN", "SEPARATOR"):
+        assert d["LBMIB_SOLVER_" + k] == getattr(lbmib_mod, "SOLVER_" + k)
+
+
+def test_fortran_module_constants_match_the_header():
+    d = header_defines()
+    src = open(os.path.join(ROOT, "fortran_loadfile_mitiwing_amd", "fortran", "lbmib_mod.f90")).read()
+    found = dict((n, int(v)) for n, v in re.findall(r"parameter, public :: (LBMIB_\w+) = (-?\d+)", src))
+    assert len(found) >= 12
+    for name, val in found.items():
+        assert d[name] == val, name
+
+
+def test_launch_shape_struct_layout(lbmib_mod):
+    """The ctypes and Fortran mirrors of lbmib_launch_shape have the header's
+    field order and size (8 int32 fields)."""
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct lbmib_launch_shape"):text.index("} lbmib_launch_shape;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"int32_t\s+(\w+)(?:\[(\d+)\])?;", body)
+    names = [f for f, _ in fields]
+    count = sum(int(n) if n else 1 for _, n in fields)
+    assert [f[0] for f in lbmib_mod.LaunchShape._fields_] == names
+    assert C.sizeof(lbmib_mod.LaunchShape) == 4 * count
 
 
 def test_params_struct_layout(lbmib_mod):
