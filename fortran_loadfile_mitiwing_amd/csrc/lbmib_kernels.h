@@ -166,6 +166,8 @@ template <typename T, int NPT>
 __global__ void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                           const int2* __restrict__ tiles, int ntiles, int off);
 __host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
+// first node of the interior tiles: row 1 rounded down to a multiple of 64 nodes
+__host__ __device__ inline int lattice_base(int nx) { return nx / 64 * 64; }
 int lattice_tiles(int nx, int ny, int npt);  // ring blocks + interior tiles of one case
 __global__ void k_geo_clear(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_markers(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);

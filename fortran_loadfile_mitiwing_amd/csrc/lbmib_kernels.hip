@@ -245,12 +245,14 @@ __device__ __forceinline__ void lattice_node(const DevCase& D, int g, int cur, d
 // One lattice update of every case of the window: the interior nodes and
 // the outer boundary ring (boundary_condition1).  Work is cut into tiles;
 // tile (c, b) of case c is ring block b (256 ring threads) for
-// b < lattice_ring_blocks, otherwise interior row 1 + q / gx, columns
-// (q % gx) * 256 * NPT .. + 256 * NPT - 1 with q = b - ring blocks and
-// gx = ceil((nx - 1) / (256 * NPT)): tiles start at multiples of 256 nodes,
-// so every wave writes whole cache lines.  Thread t of a tile takes columns
-// t, t + 256, ..  A grid smaller than the tile count makes the kernel
-// persistent: block v takes tiles v, v + gridDim.x, ..  The order of the
+// b < lattice_ring_blocks, otherwise the 256 * NPT consecutive nodes
+// lattice_base(nx) + q * 256 * NPT .. of the node order c = j * nx + i, with
+// q = b - ring blocks.  Tiles run on over row ends, so no lane idles at a
+// row's short last tile, and they start at multiples of 64 nodes, so every
+// wave writes whole cache lines; lanes on the ring columns and rows skip.
+// Thread t of a tile takes nodes t, t + 256, ..  A grid smaller than the
+// tile count makes the kernel persistent: block v takes tiles v,
+// v + gridDim.x, ..  The order of the
 // tile table (case-major, or dealt to the accelerator's XCDs) is the
 // launch shape's choice; the result does not depend on it.
 template <typename T, int NPT>
@@ -276,11 +278,18 @@ __global__ __launch_bounds__(256) void k_lattice(const DevCase* __restrict__ C, 
         }
         const int q = blk - rb;
         const int g = (n - 1) % D.nslot;
-        const int gx = (D.nx - 1 + 256 * NPT - 1) / (256 * NPT);
-        const int j = q / gx + 1;
-        const int i0 = (q % gx) * 256 * NPT + (int)threadIdx.x;
+        const int nx = D.nx;
+        const int c0 = lattice_base(nx) + q * 256 * NPT;  // wave-uniform
+        const int j0 = c0 / nx;
 #pragma unroll
-        for (int r = 0; r < NPT; ++r) lattice_node<T>(D, g, cur, macro, i0 + r * 256, j);
+        for (int r = 0; r < NPT; ++r) {
+            int i = c0 - j0 * nx + r * 256 + (int)threadIdx.x, j = j0;
+            while (i >= nx) {  // at most (256 * NPT) / nx + 1 rows on
+                i -= nx;
+                ++j;
+            }
+            if (j >= 1 && j <= D.ny - 2) lattice_node<T>(D, g, cur, macro, i, j);
+        }
     }
 }
 template __global__ void k_lattice<double, 1>(const DevCase*, const CaseClock*, const int2*, int, int);
@@ -290,7 +299,8 @@ template __global__ void k_lattice<float, 2>(const DevCase*, const CaseClock*, c
 
 int lattice_tiles(int nx, int ny, int npt)
 {
-    return lattice_ring_blocks(nx, ny) + (nx - 1 + 256 * npt - 1) / (256 * npt) * (ny - 2);
+    const long long span = (long long)(ny - 1) * nx - lattice_base(nx);  // through the last interior row
+    return lattice_ring_blocks(nx, ny) + (int)((span + 256 * npt - 1) / (256 * npt));
 }
 
 // Box search of move_boundary (airfoil.f90:290-321) for one axis: the 1-based
