@@ -23,6 +23,12 @@ plain FP64 path.  value = cases per hour of the batch; the JSON also holds
 the sequential rate, and per-case C_L / C_D errors of the batch against the
 FP64 runs.
 
+--form precision: the precision form's workload (8192x4096, chord 512 lu,
+2048 markers, Re 2000, plunge 0.5c, St 0.3), run in the FP32 deviation form
+and in FP64 from the same start.  value = FP32 MLUPS over the whole run; the
+JSON holds the FP64 rate, their ratio, the relative L2 differences of
+rho / u / v after the last step and the C_L / C_D errors of the FP32 run.
+
 --form serve: the serving loop.  Requests (cases drawn as for the sweep with
 nx up to 4096 and 2000..20000 steps, log-uniform) arrive as a Poisson process
 at the rate that offers 80% of the device's capacity (measured first on a
@@ -51,7 +57,7 @@ WORKLOADS = {
        "(90deg lead), St 0.3, U 0.05",
     2: "8192x4096 lattice, NACA0012 chord 512 lu, 2048 markers, Re 2000, plunge 0.5c, St 0.3, U 0.05",
 }
-PROFILE_EVERY = 17  # coprime with the 8-step geometry batch: samples every phase of it
+PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every phase of it
 # HBM bytes per k_lattice_f64 launch on the 4096x2048 workload, from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
 # device copy measured the same way (the lattice traffic summary in profiles/)
@@ -299,6 +305,58 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+def bench_precision(args):
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    steps = args.precision_steps
+    case = cases.config(2, n_steps=steps + 8)
+    p = case.params
+    runs = {}
+    for name, prec in (("fp32", lbmib.PREC_FP32DEV), ("fp64", lbmib.PREC_FP64)):
+        g = lbmib.LBMIB(case, device=0, precision=prec)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(2500, steps - done)
+            g.evolution(k)
+            done += k
+            print("[precision] %s %d steps, %.1f s" % (name, done, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
+        dt = time.perf_counter() - t0
+        runs[name] = {"s": dt, "mlups": p.nx * p.ny * steps / dt / 1e6, "hist": g.history(),
+                      "fields": g.fields(with_f=False), "shape": g.shape()}
+        g.close()
+    f32, f64 = runs["fp32"], runs["fp64"]
+    field_l2 = {}
+    for k in ("rho", "u", "v"):
+        a, b = f32["fields"][k], f64["fields"][k]
+        field_l2[k] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    ce = coefficient_errors(f32["hist"], f64["hist"])
+    out = {
+        "metric": "MLUPS of the FP32-deviation form and its error against FP64 (precision workload)",
+        "value": round(f32["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(f32["s"] / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 lattice, f64 immersed boundary",
+        "data": "synthetic (generated NACA0012 markers and sinusoidal kinematics)",
+        "config": {"workload": "%s: %s, FP32 deviation form against FP64, %d steps from the start" % (
+            case.name, WORKLOADS[2], steps), "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
+        "fp64_MLUPS": round(f64["mlups"], 1),
+        "fp32_over_fp64": round(f32["mlups"] / f64["mlups"], 3),
+        "field_rel_L2_after_last_step": field_l2,
+        "coefficient_errors_last_half": {k: v for k, v in ce.items() if not k.startswith("_")},
+        "coefficient_error_notes": "mean C_L and mean C_D relative to their r.m.s. over the last half, "
+                                   "mean_CD_rel relative to |mean C_D|, peak |C_L| relative",
+        "launch_shape": f32["shape"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,9 +371,11 @@ def main():
                          "ib_cus=16,ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
     ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
                     help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
-    ap.add_argument("--form", choices=["step", "batch", "serve"], default="step",
+    ap.add_argument("--form", choices=["step", "batch", "serve", "precision"], default="step",
                     help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
-                         "serve: the continuous-batching service")
+                         "serve: the continuous-batching service; precision: FP32 against FP64 on the "
+                         "8192x4096 workload")
+    ap.add_argument("--precision-steps", type=int, default=50000, help="--form precision: steps of each run")
     ap.add_argument("--requests", type=int, default=500, help="--form serve: requests")
     ap.add_argument("--load", type=float, default=0.8, help="--form serve: offered load / capacity")
     ap.add_argument("--window", type=int, default=32, help="--form serve: steps per scheduling window")
@@ -345,6 +405,8 @@ def main():
         return bench_batch(args)
     if args.form == "serve":
         return bench_serve(args)
+    if args.form == "precision":
+        return bench_precision(args)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
     fp32 = args.precision == "fp32"

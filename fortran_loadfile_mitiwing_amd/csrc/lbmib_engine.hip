@@ -137,14 +137,28 @@ struct Dims {
     int plain = 0;                       // chain members with per-lane (not compact) coefficients
 };
 
-const void* solve2_kernel(int np2)
+// Staging units (steps per LDS ring unit) of the two solve kernels: the
+// largest whose rings fit beside the values, capped by shape.unit (0: no
+// cap).  The unit never changes results.
+int chain_unit(const Engine* E, const Dims& d)
 {
-    return solve2_unit(np2) == 32 ? (const void*)k_crit_solve2<32> : (const void*)k_crit_solve2<16>;
+    const int u = solve_unit(d.maxnpad);
+    return E->shape.unit && E->shape.unit < u ? 32 : u;
+}
+int sep_unit(const Engine* E, const Dims& d)
+{
+    const int u = solve2_unit(d.maxnp2);
+    return E->shape.unit && E->shape.unit < u ? 16 : u;
 }
 
-const void* solve_kernel(int npad, bool cmp)
+const void* solve2_kernel(int U)
 {
-    if (solve_unit(npad) == 64) return cmp ? (const void*)k_crit_solve<64, true> : (const void*)k_crit_solve<64, false>;
+    return U == 32 ? (const void*)k_crit_solve2<32> : (const void*)k_crit_solve2<16>;
+}
+
+const void* solve_kernel(int U, bool cmp)
+{
+    if (U == 64) return cmp ? (const void*)k_crit_solve<64, true> : (const void*)k_crit_solve<64, false>;
     return cmp ? (const void*)k_crit_solve<32, true> : (const void*)k_crit_solve<32, false>;
 }
 
@@ -212,12 +226,12 @@ int engine_prepare(Engine* E)
     HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)factor_smem_bytes()));
     for (bool cmp : {false, true})
-        HIP_TRY(hipFuncSetAttribute(solve_kernel(d.maxnpad, cmp), hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIP_TRY(hipFuncSetAttribute(solve_kernel(chain_unit(E, d), cmp), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)solve_smem_bytes(d.maxnpad)));
     if (d.sep) {
         HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor2, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)factor_smem_bytes()));
-        HIP_TRY(hipFuncSetAttribute(solve2_kernel(d.maxnp2), hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIP_TRY(hipFuncSetAttribute(solve2_kernel(sep_unit(E, d)), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)solve2_smem_bytes(d.maxnp2)));
     }
     return 0;
@@ -307,7 +321,7 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     // chain members of each coefficient layout (a kernel returns at once for
     // the other layout's and the separator form's members)
     const size_t smc = solve_smem_bytes(d.maxnpad);
-    const bool u64 = solve_unit(d.maxnpad) == 64;
+    const bool u64 = chain_unit(E, d) == 64;
     if (d.chain > d.plain) {
         if (u64) hipLaunchKernelGGL((k_crit_solve<64, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
         else hipLaunchKernelGGL((k_crit_solve<32, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
@@ -318,7 +332,7 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     }
     if (d.sep) {
         const size_t sm = solve2_smem_bytes(d.maxnp2);
-        if (solve2_unit(d.maxnp2) == 32)
+        if (sep_unit(E, d) == 32)
             hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
         else
             hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
@@ -583,6 +597,8 @@ int engine_set_shape(Engine* E, const Shape& s)
     if (s.graph != 0 && s.graph != 1) return fail(LBMIB_E_ARG, "graph must be 0 or 1");
     if (s.band != 0 && s.band != 1) return fail(LBMIB_E_ARG, "ib_collide_shared must be 0 or 1");
     if (s.rhs != 0 && s.rhs != 1) return fail(LBMIB_E_ARG, "ib_rhs_reserved must be 0 or 1");
+    if (s.unit != 0 && s.unit != 16 && s.unit != 32 && s.unit != 64)
+        return fail(LBMIB_E_ARG, "solve_unit must be 0, 16, 32 or 64");
     // Graph replay is switched off: its first run on the GPU (a three-case
     // batch, replaying captured 16-step windows on CU-masked streams) ended
     // the host process with a segmentation fault whose cause is not known.
@@ -707,6 +723,7 @@ Shape shape_in(const lbmib_launch_shape* s)
     o.graph = s->graph;
     o.band = s->ib_collide_shared;
     o.rhs = s->ib_rhs_reserved;
+    o.unit = s->solve_unit;
     return o;
 }
 
@@ -720,6 +737,7 @@ void shape_out(const Shape& s, lbmib_launch_shape* o)
     o->graph = s.graph;
     o->ib_collide_shared = s.band;
     o->ib_rhs_reserved = s.rhs;
+    o->solve_unit = s.unit;
 }
 
 }  // namespace
@@ -897,8 +915,9 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
         ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / timed;
         char buf[256];
         snprintf(buf, sizeof buf, "%s{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-                 "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"graph\":%d,\"ms_per_step\":%.5f}",
-                 firstrec ? "" : ",", s.npt, s.order, s.persist, s.ib_cus, s.band, s.rhs, s.graph, ms);
+                 "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"solve_unit\":%d,\"graph\":%d,"
+                 "\"ms_per_step\":%.5f}",
+                 firstrec ? "" : ",", s.npt, s.order, s.persist, s.ib_cus, s.band, s.rhs, s.unit, s.graph, ms);
         rep += buf;
         firstrec = false;
         return 0;
@@ -948,8 +967,9 @@ int lbmib_batch_autotune(lbmib_batch* b, int32_t trial_steps, lbmib_launch_shape
     if (rc) return rc;
     char buf[256];
     snprintf(buf, sizeof buf, "],\"best\":{\"nodes_per_thread\":%d,\"tile_order\":%d,\"persistent\":%d,\"ib_cus\":%d,"
-             "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"graph\":%d,\"ms_per_step\":%.5f}}", cur.npt,
-             cur.order, cur.persist, cur.ib_cus, cur.band, cur.rhs, cur.graph, cur_ms);
+             "\"ib_collide_shared\":%d,\"ib_rhs_reserved\":%d,\"solve_unit\":%d,\"graph\":%d,"
+             "\"ms_per_step\":%.5f}}", cur.npt, cur.order, cur.persist, cur.ib_cus, cur.band, cur.rhs, cur.unit,
+             cur.graph, cur_ms);
     rep += buf;
     if (best) shape_out(cur, best);
     if (report && report_len > 0) {

@@ -58,6 +58,7 @@ struct Shape {
     int graph = 0;
     int band = 0;
     int rhs = 1;
+    int unit = 0;
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in

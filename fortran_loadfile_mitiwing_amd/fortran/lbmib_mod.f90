@@ -43,6 +43,7 @@ module lbmib
     type, bind(C), public :: lbmib_launch_shape
         integer(c_int32_t) :: nodes_per_thread = 1, tile_order = 0, persistent = 0
         integer(c_int32_t) :: ib_cus = 16, graph = 0, ib_collide_shared = 0, ib_rhs_reserved = 1
+        integer(c_int32_t) :: solve_unit = 0
         integer(c_int32_t) :: reserved(1) = 0
     end type
 

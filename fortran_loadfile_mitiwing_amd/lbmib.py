@@ -63,19 +63,19 @@ class Params(C.Structure):
 class LaunchShape(C.Structure):
     _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
                 ("ib_cus", C.c_int32), ("graph", C.c_int32), ("ib_collide_shared", C.c_int32),
-                ("ib_rhs_reserved", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("ib_rhs_reserved", C.c_int32), ("solve_unit", C.c_int32), ("reserved", C.c_int32 * 1)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph",
-                                                    "ib_collide_shared", "ib_rhs_reserved")}
+                                                    "ib_collide_shared", "ib_rhs_reserved", "solve_unit")}
 
 
 def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=16, graph=0, ib_collide_shared=0,
-               ib_rhs_reserved=1) -> LaunchShape:
+               ib_rhs_reserved=1, solve_unit=0) -> LaunchShape:
     s = LaunchShape()
     s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
     s.ib_cus, s.graph, s.ib_collide_shared = ib_cus, graph, ib_collide_shared
-    s.ib_rhs_reserved = ib_rhs_reserved
+    s.ib_rhs_reserved, s.solve_unit = ib_rhs_reserved, solve_unit
     return s
 
 

@@ -192,10 +192,12 @@ typedef struct lbmib_launch_shape {
                                   solves' reserved ones */
     int32_t ib_rhs_reserved;  /* 1: the right-hand sides of the IB systems run in the solves' stream (on
                                  their reserved CUs), not beside the lattice update */
+    int32_t solve_unit;       /* cap on the steps the solves stage per LDS ring unit: 0 (none: the
+                                 largest that fits), 16, 32 or 64 */
     int32_t reserved[1];
 } lbmib_launch_shape;
 
-/* Launch shape of a single handle's steps (default 1, 0, 0, 16, 0, 0, 1; the
+/* Launch shape of a single handle's steps (default 1, 0, 0, 16, 0, 0, 1, 0; the
  * environment variable LBMIB_IB_CUS overrides ib_cus). */
 int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
 int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);

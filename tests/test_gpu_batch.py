@@ -134,6 +134,7 @@ SHAPES = [
     dict(ib_collide_shared=1, ib_cus=0, tile_order=1),
     dict(ib_rhs_reserved=0),
     dict(ib_rhs_reserved=1, ib_collide_shared=1, ib_cus=16),
+    dict(solve_unit=16),  # the smaller staging units of both solve kernels
     pytest.param(dict(graph=1), marks=pytest.mark.skip(reason=GRAPH_OFF)),
     pytest.param(dict(graph=1, nodes_per_thread=2, tile_order=1, persistent=8), marks=pytest.mark.skip(reason=GRAPH_OFF)),
 ]
