@@ -104,13 +104,21 @@ def coefficient_errors(h32, h64, skip_frac=0.5):
             "_mean_CD64": float(cd64.mean()), "_rms_CD64": rms_d}
 
 
-def run_batch(cs, prec, steps, tune, chunk=1000, label=""):
-    """Advance all cases together; returns (seconds, histories, timings, shape)."""
+def parse_shape(text):
+    return {k: int(v) for k, v in (kv.split("=") for kv in text.split(","))} if text else {}
+
+
+def run_batch(cs, prec, steps, tune, chunk=1000, label="", shape=None):
+    """Advance all cases together (launch shape: autotuned, or the given
+    one, or the library's); returns (seconds, histories, timings, shape)."""
     from fortran_loadfile_mitiwing_amd import lbmib
     sims = [lbmib.LBMIB(c, precision=prec) for c in cs]
     b = lbmib.Batch()
     for s_ in sims:
         b.add(s_)
+    if shape:
+        b.set_shape(**shape)
+        tune = False
     shape = b.autotune(trial_steps=32)["best"] if tune else b.shape()
     print("[batch] %s shape %s" % (label, shape), file=sys.stderr, flush=True)
     b.set_profiling(PROFILE_EVERY)
@@ -138,7 +146,8 @@ def bench_batch(args):
     interior = sum((c.params.nx - 2) * (c.params.ny - 2) for c in cs)
     print("[batch] %d cases, %d lattice nodes, %d steps each" % (len(cs), nodes, steps), file=sys.stderr, flush=True)
     if args.only_batched:
-        t_batch, h32, tm, shape = run_batch(cs, lbmib.PREC_FP32DEV, steps, not args.no_tune, label="fp32")
+        t_batch, h32, tm, shape = run_batch(cs, lbmib.PREC_FP32DEV, steps, not args.no_tune, label="fp32",
+                                            shape=parse_shape(args.shape))
         print(json.dumps({"form": "batch (batched FP32 only)", "ms_per_step": round(t_batch / steps * 1e3, 4),
                           "phases_ms": tm, "launch_shape": shape}))
         return
@@ -367,8 +376,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
     ap.add_argument("--shape", default="",
-                    help="launch shape of the step form, e.g. nodes_per_thread=2,tile_order=1,persistent=4,"
-                         "ib_cus=16,ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
+                    help="launch shape of the step form (and of --form batch --only-batched, instead of the "
+                         "autotuner), e.g. nodes_per_thread=2,tile_order=1,persistent=4,ib_cus=16,"
+                         "ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
     ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
                     help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
     ap.add_argument("--form", choices=["step", "batch", "serve", "precision"], default="step",
@@ -415,7 +425,7 @@ def main():
     solver = {"auto": lbmib.SOLVER_AUTO, "chain": lbmib.SOLVER_CHAIN, "separator": lbmib.SOLVER_SEPARATOR}[args.solver]
     g = lbmib.LBMIB(case, device=0, precision=lbmib.PREC_FP32DEV if fp32 else lbmib.PREC_FP64, ib_solver=solver)
     if args.shape:
-        g.set_shape(**{k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))})
+        g.set_shape(**parse_shape(args.shape))
     if W:
         g.evolution(W)
     # phase events on every PROFILE_EVERY-th timed step: they time the lattice
