@@ -1653,7 +1653,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ mc,
     s1[threadIdx.x] = a1;
     s2[threadIdx.x] = a2;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (unsigned off = 128; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
             s1[threadIdx.x] += s1[threadIdx.x + off];
             s2[threadIdx.x] += s2[threadIdx.x + off];
