@@ -402,7 +402,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             if (every > 0) {
                 while (E->gevpool.size() < (size_t)3 * (E->gev_used + 1)) {
                     hipEvent_t e;
-                    HIP_TRY(hipEventCreate(&e));
+                    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
                     E->gevpool.push_back(e);
                 }
                 gev = &E->gevpool[(size_t)3 * E->gev_used++];
@@ -426,7 +426,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             const size_t need = (size_t)Eper * (off / every + 1);
             while (E->evpool.size() < need + 2) {
                 hipEvent_t e;
-                HIP_TRY(hipEventCreate(&e));
+                HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
                 E->evpool.push_back(e);
             }
             ev = &E->evpool[2 + (size_t)Eper * (off / every)];
@@ -635,7 +635,7 @@ int engine_run(Engine* E, int k)
     if (every > 0) {
         while (E->evpool.size() < 2) {
             hipEvent_t e;
-            HIP_TRY(hipEventCreate(&e));
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
             E->evpool.push_back(e);
         }
         t0 = E->evpool[0];
