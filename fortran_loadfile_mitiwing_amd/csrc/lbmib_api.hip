@@ -497,16 +497,20 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         const bool want = sol == LBMIB_SOLVER_SEPARATOR || (sol == LBMIB_SOLVER_AUTO && D.prec == LBMIB_PREC_FP32DEV);
         const int b = D.bw, mT = (D.nseg - b) / 2, mB = D.nseg - b - mT;
         D.compact = b < CW ? 1 : 0;
-        D.dissect = want && D.compact && b >= 1 && b <= SEPMAXBW && sep_len1(b) <= SEP_L1 && sep_len2(b) <= SEP_L2 &&
-                    mT >= b && mB >= b && mT >= 64 ? 1 : 0;
+        const int np2 = (std::max(mT, mB) + 127) / 128 * 128;
+        const bool fits = b >= 1 && b <= SEPMAXBW && sep_len1(b) <= SEP_L1 && sep_len2(b) <= SEP_L2 && mT >= b &&
+                          mB >= b && mT >= 64;
+        // AUTO falls back to the single chain where the separator form does not apply
+        D.dissect = want && D.compact && fits && solve2_unit(np2) > 0 ? 1 : 0;
         D.mT = D.dissect ? mT : 0;
         D.mB = D.dissect ? mB : 0;
-        D.np2 = D.dissect ? (std::max(mT, mB) + 127) / 128 * 128 : 0;
-        if (sol == LBMIB_SOLVER_SEPARATOR && !D.dissect)
+        D.np2 = D.dissect ? np2 : 0;
+        if (sol == LBMIB_SOLVER_SEPARATOR && !D.dissect) {
+            if (D.compact && fits)
+                return fail(LBMIB_E_ARG, "too many segments for the separator solve's LDS (%d segments)", D.nseg);
             return fail(LBMIB_E_ARG, "the separator solve needs at least 2*64 + bandwidth segments and bandwidth "
                         "<= %d (have %d segments, bandwidth %d)", SEPMAXBW, D.nseg, b);
-        if (D.dissect && solve2_unit(D.np2) == 0)
-            return fail(LBMIB_E_ARG, "too many segments for the separator solve's LDS");
+        }
     }
     double* dkin;
     if (dalloc(h, &dkin, kin.size())) return LBMIB_E_HIP;
