@@ -70,6 +70,14 @@ def test_config0_200_steps(oracle_mod, lbmib_mod):
     assert_parity(o, g)
 
 
+def test_config0_full_run(oracle_mod, lbmib_mod):
+    """The CPU-runnable workload's whole run (400x200, 4000 steps, in calls of
+    1000 steps): still bit-identical at the end."""
+    case = cases.config(0)
+    o, g = run_pair(case, 4000, oracle_mod, lbmib_mod, chunks=[1000] * 4)
+    assert_parity(o, g)
+
+
 def test_pitch_plunge_small(oracle_mod, lbmib_mod):
     # the pitch + plunge workload at 1/16 scale
     case = cases.config(1, n_steps=80, scale_down=16)
