@@ -723,6 +723,18 @@ int lbmib_upload_state(lbmib_handle* h, const double* f)
     return 0;
 }
 
+int lbmib_set_step(lbmib_handle* h, int32_t n)
+{
+    if (!h) return fail(LBMIB_E_ARG, "null handle");
+    if (!h->have_motion) return fail(LBMIB_E_STATE, "lbmib_set_motion_table must come first");
+    if (h->batch) return fail(LBMIB_E_STATE, "the case belongs to a batch");
+    if (n < 0 || n > h->max_steps) return fail(LBMIB_E_ARG, "step %d outside 0..%d", n, h->max_steps);
+    h->steps_done = n;
+    h->geo_fresh = true;
+    h->last_flag_step[0] = h->last_flag_step[1] = -1;
+    return 0;
+}
+
 int lbmib_residual(lbmib_handle* h, double* err)
 {
     if (!h || !err) return fail(LBMIB_E_ARG, "null argument");
@@ -767,6 +779,27 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
     }
     if (glx) HIP_TRY(hipMemcpy(glx, S.glx, sizeof(double) * na, hipMemcpyDeviceToHost));
     if (gly) HIP_TRY(hipMemcpy(gly, S.gly, sizeof(double) * na, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1)
+{
+    if (!h || !xb1 || !yb1) return fail(LBMIB_E_ARG, "null argument");
+    if (!h->have_motion) return fail(LBMIB_E_STATE, "lbmib_set_motion_table must come first");
+    // move_boundary's rigid motion of the shifted polygon (airfoil.f90:256-263)
+    // with the kinematics row of the last step taken (of step 1 before any)
+    const int n = std::max(1, h->steps_done);
+    const double* row = &h->step_rows[(size_t)(n - 1) * h->ncols];
+    const int nw = h->p.n_wings, ppw = h->p.pts_per_wing;
+    for (int w = 0; w < nw; ++w) {
+        const double* c = row + 1 + 9 * w;
+        const double sn = std::sin(c[0]), ca = std::cos(c[0]), sa = c[3], sb = c[6];
+        for (int i = 0; i < ppw; ++i) {
+            const double x = h->xb0[w * ppw + i], y = h->yb0[w * ppw + i];
+            xb1[w * ppw + i] = ca * x + sn * y + sa + h->p.xc;
+            yb1[w * ppw + i] = -sn * x + ca * y + sb + h->p.yc;
+        }
+    }
     return 0;
 }
 

@@ -384,7 +384,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
     const int Eper = 5;
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr;
-        if ((phase + off) % B == 0) {
+        if ((phase + off) % B == 0 || (fresh && off == 0)) {
             if (off > fupto) {
                 int rc = enqueue_forces(E, d, fupto, off - 1, E->gstep - 1);
                 if (rc) return rc;
@@ -454,7 +454,7 @@ int upload_clocks(Engine* E, const std::vector<int>& len_end, const std::vector<
         k.n0 = h->steps_done + 1;
         k.n_end = len_end[c];
         k.flag_from = final_end[c] - 1;
-        k.pad = 0;
+        k.fresh = h->geo_fresh ? 1 : 0;
     }
     HIP_TRY(hipMemcpy(E->dclock, E->hclock.data(), sizeof(CaseClock) * nc, hipMemcpyHostToDevice));
     return 0;
@@ -485,6 +485,7 @@ void advance_members(Engine* E, const std::vector<int>& len_end, const std::vect
         lbmib_handle* h = E->members[c];
         const int n0 = h->steps_done + 1, n1 = len_end[c];
         for (int n = std::max(n0, final_end[c] - 1); n <= n1; ++n) h->last_flag_step[n & 1] = n;
+        if (n1 >= n0) h->geo_fresh = false;
         h->steps_done = std::max(h->steps_done, n1);
     }
 }
@@ -649,7 +650,7 @@ int engine_run(Engine* E, int k)
         bool fresh = false;
         for (size_t c = 0; c < nc; ++c) {
             kwin = std::max(kwin, final_end[c] - E->members[c]->steps_done);
-            fresh = fresh || E->members[c]->steps_done == 0;
+            fresh = fresh || E->members[c]->geo_fresh;
         }
         if (kwin <= 0) break;
         const int phase = E->members[0]->steps_done % B;

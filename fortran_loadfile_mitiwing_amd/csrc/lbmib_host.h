@@ -41,6 +41,7 @@ struct lbmib_handle {
     int ncols = 0, max_steps = 0;
     double motion_end_time = 0.0;
     int steps_done = 0;
+    bool geo_fresh = true;   // the next run builds the geometry of its first step's batch first
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
     std::vector<void*> allocs;        // device allocations to free
     long long device_bytes = 0;       // their total size

@@ -128,12 +128,12 @@ struct DevCase {
 // n >= flag_from.  Cases of one window may stand at different steps, but all
 // at the same position inside a geometry batch.
 struct CaseClock {
-    int n0, n_end, flag_from, pad;
+    int n0, n_end, flag_from;
+    int fresh;  // 1: no geometry built for this case's steps yet (new run or restart)
 };
 
-// Geometry launch modes: GEO_FIRST builds batch 0 of a case that takes its
-// first step at this offset; GEO_NEXT builds the batch after the one that
-// starts at this offset.
+// Geometry launch modes: GEO_FIRST builds the batch of a fresh case's first
+// step; GEO_NEXT builds the batch after the current one.
 enum GeoMode { GEO_FIRST = 0, GEO_NEXT = 1 };
 
 // One slot's arrays.

@@ -352,17 +352,21 @@ __device__ __forceinline__ bool box_cell(const int* b, int q, int& i, int& j)
 }
 
 // The geometry batch that a launch at window offset `off` builds for a case:
-// GEO_FIRST builds batch 0 when the case takes step 1 at this offset,
-// GEO_NEXT the batch after the one starting at this offset.  Block s of the
+// GEO_FIRST builds the batch holding the step a fresh case takes at offset 0,
+// GEO_NEXT the batch after the one holding the step at this offset (at
+// batch starts, or at offset 0 for a fresh case).  Block s of the
 // batch works on step n0b + s (1-based); returns false when there is none.
 __device__ __forceinline__ bool geo_step(const DevCase& D, const CaseClock& k, int off, int mode, int s,
                                          int& step0)
 {
     const int n = k.n0 + off;
-    if (n > k.n_end || (n - 1) % D.batch != 0) return false;
+    // a fresh case (first step of a run or of a restart) builds at offset 0
+    // whatever its position inside the batch; the others at batch starts
+    const bool start = k.fresh && off == 0;
+    if (n > k.n_end || ((n - 1) % D.batch != 0 && !start)) return false;
     int j = (n - 1) / D.batch;
     if (mode == GEO_FIRST) {
-        if (n != 1) return false;
+        if (!start) return false;
     } else {
         j += 1;
     }

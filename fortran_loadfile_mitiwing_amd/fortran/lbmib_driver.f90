@@ -4,6 +4,9 @@
 !   motion_params.dat               list-directed rows, 1+9*wing_count columns (mode.f90:68-142)
 !   force.dat                       one '(1000f15.8)' row per step (solution.f90:349-376)
 !   airfoil_motion.dat              one '(1000f15.8)' row per step (solution.f90:378-390)
+!   F<n>.dat                        Tecplot flow field and airfoil zones (output, solution.f90:129-238)
+!   saved_flow_feild.dat            binary restart state (saveflowfield / readflowfield,
+!                                   solution.f90:1604-1839)
 ! and takes the parameters that the reference sets in ini_mesh / main from a
 ! namelist file (first argument, default lbmib.nml).
 program lbmib_driver
@@ -18,18 +21,25 @@ program lbmib_driver
     character(len=256) :: boundary_dir = '.', motion_file = 'motion_params.dat'
     character(len=256) :: force_file = 'force.dat', motion_out_file = 'airfoil_motion.dat'
     character(len=256) :: fields_file = ''
+    ! flow-field output every output_every steps (0: none) at every
+    ! output_stride-th node (output_stride_x/y, main.f90:33-34); binary
+    ! restart state every restart_every steps (0: none); restart_in resumes
+    ! from such a file
+    integer :: output_every = 0, output_stride = 2, restart_every = 0
+    character(len=256) :: output_dir = '.', restart_out = 'saved_flow_feild.dat', restart_in = ''
     namelist /lbmib_case/ nx, ny, ox, oy, scale, UU, xc, yc, xp, yp, s, smooth, wing_count, &
         points_per_wing, boundary_dir, motion_file, force_file, motion_out_file, fields_file, &
-        chunk, device, report_every
+        chunk, device, report_every, output_every, output_stride, restart_every, output_dir, &
+        restart_out, restart_in
 
     type(lbmib_params) :: p
     type(c_ptr) :: h
     character(len=256) :: nml_file, fname
     character(len=32) :: idx_text
     real(c_double), allocatable :: xb0(:), yb0(:), motion(:, :), hist(:, :), row(:)
-    real(c_double), allocatable, target :: rho(:, :), u(:, :), v(:, :)
-    real(c_double) :: err
-    integer :: unit_id, ios, wing, i0, ps, ncols, nrows, nmax, n, k, cols, col
+    real(c_double), allocatable, target :: rho(:, :), u(:, :), v(:, :), f(:, :, :)
+    real(c_double) :: err, feq0(9)
+    integer :: unit_id, ios, wing, i0, ps, ncols, nrows, nmax, n, k, cols, col, n0
     integer(c_int32_t) :: got
     character(len=1024) :: line
 
@@ -127,34 +137,69 @@ program lbmib_driver
     call lbmib_check(lbmib_max_steps(h, got), 'lbmib_max_steps')
     nmax = got
     write(*, '(a, a, i0, a)') trim(lbmib_version_string()), ': ', nmax, ' steps'
+    allocate(rho(nx, ny), u(nx, ny), v(nx, ny))
+    if (restart_every .gt. 0 .or. len_trim(restart_in) .gt. 0) allocate(f(nx, ny, 9))
+    ! the stale ring populations f0 and the coarse-to-fine copies the
+    ! restart file carries stay at the initial equilibrium: f of any node now
+    if (restart_every .gt. 0) then
+        call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(f)), 'lbmib_download')
+        feq0 = f(1, 1, :)
+    endif
 
-    ! the main loop: evolution(mesh, n, nforce, h) for n = 1 .. nmax, the
-    ! residual reported every report_every steps
+    ! readflowfield (main.f90:320-324): resume after the saved step
     n = 0
+    if (len_trim(restart_in) .gt. 0) then
+        call read_restart(trim(restart_in), n)
+        call lbmib_check(lbmib_set_step(h, int(n, c_int32_t)), 'lbmib_set_step')
+        call lbmib_check(lbmib_upload_state(h, f), 'lbmib_upload_state')
+        write(*, '(a, i0)') 'resumed after step ', n
+    endif
+    n0 = n
+
+    ! the main loop: evolution(mesh, n, nforce, h) for n = n0+1 .. nmax, the
+    ! residual reported every report_every steps, the flow field and the
+    ! restart state written on their schedules
     do while (n .lt. nmax)
         k = min(chunk, nmax - n)
         if (report_every .gt. 0) k = min(k, report_every - mod(n, report_every))
+        if (output_every .gt. 0) k = min(k, output_every - mod(n, output_every))
+        if (restart_every .gt. 0) k = min(k, restart_every - mod(n, restart_every))
         call lbmib_check(lbmib_step(h, int(k, c_int32_t)), 'lbmib_step')
         n = n + k
         if (report_every .gt. 0) then
-            if (mod(n, report_every) .eq. 0 .and. n .ge. 2) then
+            if (mod(n, report_every) .eq. 0 .and. n .ge. n0 + 2) then
                 call lbmib_check(lbmib_residual(h, err), 'lbmib_residual')
                 write(*, '(a, i0, a, es14.6)') 'step ', n, ' err ', err
             endif
         endif
+        if (output_every .gt. 0) then
+            if (mod(n, output_every) .eq. 0) call write_tecplot(n)
+        endif
+        if (restart_every .gt. 0) then
+            if (mod(n, restart_every) .eq. 0) call write_restart(n)
+        endif
     enddo
 
-    ! force.dat and airfoil_motion.dat rows
+    ! force.dat and airfoil_motion.dat rows (appended after a restart, as
+    ! the reference continues its force file)
     cols = 4 + 12 * wing_count
     allocate(hist(cols, max(nmax, 1)), row(cols))
     call lbmib_check(lbmib_get_history(h, hist, int(nmax, c_int32_t), got), 'lbmib_get_history')
-    open(newunit=unit_id, file=trim(force_file), status='replace', action='write')
-    do k = 1, nmax
+    if (n0 .gt. 0) then
+        open(newunit=unit_id, file=trim(force_file), status='unknown', position='append', action='write')
+    else
+        open(newunit=unit_id, file=trim(force_file), status='replace', action='write')
+    endif
+    do k = n0 + 1, nmax
         write(unit_id, '(1000f15.8)') hist(:, k)
     enddo
     close(unit_id)
-    open(newunit=unit_id, file=trim(motion_out_file), status='replace', action='write')
-    do k = 1, nmax
+    if (n0 .gt. 0) then
+        open(newunit=unit_id, file=trim(motion_out_file), status='unknown', position='append', action='write')
+    else
+        open(newunit=unit_id, file=trim(motion_out_file), status='replace', action='write')
+    endif
+    do k = n0 + 1, nmax
         ! with the dynamics off the body's extra velocity and shift are zero
         row(1) = hist(1, k)
         col = 2
@@ -167,11 +212,167 @@ program lbmib_driver
     close(unit_id)
 
     if (len_trim(fields_file) .gt. 0) then
-        allocate(rho(nx, ny), u(nx, ny), v(nx, ny))
         call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_null_ptr), 'lbmib_download')
         open(newunit=unit_id, file=trim(fields_file), status='replace', access='stream', form='unformatted')
         write(unit_id) rho, u, v
         close(unit_id)
     endif
     call lbmib_check(lbmib_destroy(h), 'lbmib_destroy')
+
+contains
+
+    ! output (solution.f90:129-238) for one mesh without domain shifting:
+    ! a point zone of X, Y, U/UU, V/UU, the pressure rho/3, the vorticity
+    ! (central differences, one-sided on the edges), the IB-node mask and the
+    ! vorticity rate (left zero: its routine is not called, main.f90:391) at
+    ! every output_stride-th node; a triangle zone of the marker polygons
+    ! with one centre point per wing.
+    subroutine write_tecplot(cycle)
+        integer, intent(in) :: cycle
+        integer :: i, j, a, o, out_nx, out_ny, npts, kk, jp
+        integer(c_int32_t) :: na
+        integer, allocatable :: body(:, :)
+        integer(c_int32_t), allocatable, target :: pi(:), pj(:)
+        real(c_double), allocatable :: xb1(:), yb1(:)
+        real(c_double) :: dvx, duy
+        character(len=32) :: nm
+        call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_null_ptr), 'lbmib_download')
+        allocate(body(nx, ny))
+        body = 0
+        na = 0
+        call lbmib_check(lbmib_ib_nodes(h, na, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), &
+                         'lbmib_ib_nodes')
+        if (na .gt. 0) then
+            allocate(pi(na), pj(na))
+            call lbmib_check(lbmib_ib_nodes(h, na, c_loc(pi), c_loc(pj), c_null_ptr, c_null_ptr, c_null_ptr), &
+                             'lbmib_ib_nodes')
+            do a = 1, na
+                body(pi(a) + 1, pj(a) + 1) = 1
+            enddo
+        endif
+        npts = wing_count * points_per_wing
+        allocate(xb1(npts), yb1(npts))
+        call lbmib_check(lbmib_markers(h, xb1, yb1), 'lbmib_markers')
+        write(nm, '(i0)') cycle
+        open(newunit=o, file=trim(output_dir) // '/F' // trim(nm) // '.dat', status='replace', action='write')
+        write(o, '(a)') 'VARIABLES = "X" "Y" "U" "V" "Presure" "Vorticity" "body" "dvordt" '
+        out_nx = (nx - 1) / output_stride + 1
+        out_ny = (ny - 1) / output_stride + 1
+        ! the reference writes the zone headers list-directed; an explicit
+        ! format keeps each on one line whatever the compiler's record length
+        write(o, '(a, i0, a, i0, a, es24.16)') ' ZONE T="Flow feild", I=', out_nx, ',J=', out_ny, ',SOLUTIONTIME=', &
+            cycle * UU * scale
+        do j = 1, ny, output_stride
+            do i = 1, nx, output_stride
+                if (i .eq. nx) then
+                    dvx = (1.5 * v(i, j) - 2.0 * v(i - 1, j) + 0.5 * v(i - 2, j)) / (UU * scale)
+                else if (i .eq. 1) then
+                    dvx = (-1.5 * v(i, j) + 2.0 * v(i + 1, j) - 0.5 * v(i + 2, j)) / (UU * scale)
+                else
+                    dvx = (v(i + 1, j) - v(i - 1, j)) / (UU * 2.0 * scale)
+                endif
+                if (j .eq. ny) then
+                    duy = (1.5 * u(i, j) - 2.0 * u(i, j - 1) + 0.5 * u(i, j - 2)) / (UU * scale)
+                else if (j .eq. 1) then
+                    duy = (-1.5 * u(i, j) + 2.0 * u(i, j + 1) - 0.5 * u(i, j + 2)) / (UU * scale)
+                else
+                    duy = (u(i, j + 1) - u(i, j - 1)) / (UU * 2.0 * scale)
+                endif
+                write(o, '(25e16.8)') ox + dble(i - 1) * scale, oy + dble(j - 1) * scale, u(i, j) / UU, &
+                    v(i, j) / UU, rho(i, j) / 3.0d0, (dvx - duy), dble(body(i, j)), 0.0d0
+            enddo
+        enddo
+        write(o, '(a, i0, a, i0, a, a, es24.16)') ' ZONE T="airfoil", NODES=', npts + wing_count, ' ELEMENTS,=', &
+            wing_count * (points_per_wing - 1), ', DATAPACKING=POINT, ZONETYPE=FETRIANGLE', ',SOLUTIONTIME=', &
+            cycle * UU * scale
+        do i = 1, npts
+            write(o, '(8f14.8)') xb1(i), yb1(i), 255d0, 0.0d0, 0.0d0, 0.0d0, 0.0d0, 0.0d0
+        enddo
+        do i = 1, wing_count
+            j = (i - 1) * points_per_wing + 1
+            kk = j + (points_per_wing - 1) / 2
+            write(o, '(8f14.8)') 0.5d0 * (xb1(j) + xb1(kk)), 0.5d0 * (yb1(j) + yb1(kk)), 255d0, 0.0d0, 0.0d0, &
+                0.0d0, 0.0d0, 0.0d0
+        enddo
+        do i = 1, wing_count
+            j = (i - 1) * points_per_wing + 1
+            do jp = j, j + points_per_wing - 2
+                write(o, '(3I5.3)') jp, jp + 1, npts + i
+            enddo
+        enddo
+        close(o)
+    end subroutine
+
+    ! saveflowfield (solution.f90:1604-1707): the step, the wing count, the
+    ! rigid-body dynamics state (zero: the dynamics are off), the domain
+    ! shift count (zero), then per node u, v and f(1:9), then the edge
+    ! records (coarse-to-fine copies and stale f0, all at the initial
+    ! equilibrium for one mesh).  Raw stream, as form='binary' writes it.
+    subroutine write_restart(cycle)
+        integer, intent(in) :: cycle
+        integer :: i, j, o
+        integer(c_int32_t) :: i4
+        real(c_double), allocatable :: zeros(:)
+        call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(f)), 'lbmib_download')
+        allocate(zeros(11 * wing_count))
+        zeros = 0.0d0
+        open(newunit=o, file=trim(output_dir) // '/' // trim(restart_out), status='replace', access='stream', &
+             form='unformatted', action='write')
+        i4 = cycle
+        write(o) i4
+        i4 = wing_count
+        write(o) i4
+        write(o) zeros
+        i4 = 0
+        write(o) i4
+        do j = 1, ny
+            do i = 1, nx
+                write(o) u(i, j), v(i, j), f(i, j, :)
+            enddo
+        enddo
+        do j = 2, ny - 1
+            write(o) feq0, feq0, feq0, feq0
+        enddo
+        do j = 1, nx
+            write(o) feq0, feq0, feq0, feq0
+        enddo
+        close(o)
+    end subroutine
+
+    ! readflowfield (solution.f90:1709-1839): the step and the populations;
+    ! the velocity fields are recomputed by the next step and the edge
+    ! records hold the initial equilibrium, so both are skipped.
+    subroutine read_restart(fname, cycle)
+        character(len=*), intent(in) :: fname
+        integer, intent(out) :: cycle
+        integer :: i, j, o, rios
+        integer(c_int32_t) :: i4, nwf
+        real(c_double), allocatable :: skip(:)
+        real(c_double) :: uu1, vv1
+        open(newunit=o, file=fname, status='old', access='stream', form='unformatted', action='read', iostat=rios)
+        if (rios .ne. 0) then
+            write(*, *) 'cannot open restart file: ', fname
+            stop 1
+        endif
+        read(o) i4
+        cycle = i4
+        read(o) nwf
+        if (nwf .ne. wing_count) then
+            write(*, *) 'restart file wing count ', nwf, ' differs from wing_count ', wing_count
+            stop 1
+        endif
+        allocate(skip(11 * wing_count))
+        read(o) skip
+        read(o) i4
+        do j = 1, ny
+            do i = 1, nx
+                read(o, iostat=rios) uu1, vv1, f(i, j, :)
+                if (rios .ne. 0) then
+                    write(*, *) 'restart file too short for a ', nx, ' x ', ny, ' lattice: ', fname
+                    stop 1
+                endif
+            enddo
+        enddo
+        close(o)
+    end subroutine
 end program lbmib_driver

@@ -49,7 +49,8 @@ module lbmib
 
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
-    public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings
+    public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings, lbmib_set_step
+    public :: lbmib_markers
     public :: lbmib_destroy, lbmib_error_message, lbmib_version_string, lbmib_check
     ! batched form (many cases advanced together) and footprints
     public :: lbmib_set_shape, lbmib_get_shape, lbmib_device_bytes, lbmib_mem_info
@@ -112,6 +113,18 @@ module lbmib
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: h
             real(c_double), intent(in) :: f(*)
+        end function
+
+        integer(c_int) function lbmib_markers(h, xb1, yb1) bind(C, name="lbmib_markers")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: xb1(*), yb1(*)
+        end function
+
+        integer(c_int) function lbmib_set_step(h, n) bind(C, name="lbmib_set_step")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: n
         end function
 
         integer(c_int) function lbmib_residual(h, err) bind(C, name="lbmib_residual")

@@ -85,7 +85,7 @@ _lib = None
 
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
-           "lbmib_upload_state", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH):
     L.lbmib_get_history.argtypes = [vp, _dp, C.c_int32, _ip]
     L.lbmib_download.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.lbmib_upload_state.argtypes = [vp, _dp]
+    L.lbmib_set_step.argtypes = [vp, C.c_int32]
+    L.lbmib_markers.argtypes = [vp, _dp, _dp]
     L.lbmib_residual.argtypes = [vp, _dp]
     L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
@@ -245,11 +247,22 @@ class LBMIB:
             out["f"] = f.reshape(9, ny, nx)
         return out
 
-    def upload(self, f: np.ndarray):
+    def upload(self, f: np.ndarray, step: Optional[int] = None):
+        """Populations of the state after `step` steps (default: the
+        current step count): readflowfield's restore of a saved run."""
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         if f.size != 9 * self.N:
             raise ValueError("f must have 9*nx*ny entries")
+        if step is not None:
+            _check(load().lbmib_set_step(self.h, int(step)))
         _check(load().lbmib_upload_state(self.h, _d(f)))
+
+    def markers(self):
+        """Marker polygon points (x, y) of every wing at the last step taken."""
+        m = self.p.n_wings * self.p.pts_per_wing
+        x = np.zeros(m); y = np.zeros(m)
+        _check(load().lbmib_markers(self.h, _d(x), _d(y)))
+        return x, y
 
     def ib_nodes(self):
         n = C.c_int32()

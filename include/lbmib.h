@@ -129,6 +129,14 @@ int lbmib_download(lbmib_handle* h, double* rho, double* u, double* v, double* f
  * uploads f (9*nx*ny); the next step streams from it. */
 int lbmib_upload_state(lbmib_handle* h, const double* f);
 
+/* Replaces readflowfield's step counter (cycle1, solution.f90:1750): the
+ * next lbmib_step takes step n + 1, with that step's kinematics and
+ * geometry (0 <= n <= lbmib_max_steps).  Call it before
+ * lbmib_upload_state to restart a run from a saved state.  Fields and the
+ * residual of earlier steps are no longer available; history rows up to n
+ * stay as they were.  Not for a case that belongs to a batch. */
+int lbmib_set_step(lbmib_handle* h, int32_t n);
+
 /* Replaces erranalysis (solution.f90:74-127): ||u-u0|| / (||u|| + 1e-30)
  * over the interior between the last two steps -- the value the reference
  * prints at the start of the following step. */
@@ -140,6 +148,12 @@ int lbmib_residual(lbmib_handle* h, double* err);
  * with NULL pointers. */
 int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_t* wing,
                    double* glx, double* gly);
+
+/* Marker polygon of every wing at the last step taken, n_wings *
+ * pts_per_wing points (move_boundary's xb1/yb1, airfoil.f90:256-263; the
+ * points of step 1 before any step), for the airfoil zone of the flow-field
+ * output (solution.f90:219-233). */
+int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1);
 
 /* Device timing of the hot path (the reference's cpu_time table,
  * main.f90:338-369, 420-433), measured with HIP events on the streams the
