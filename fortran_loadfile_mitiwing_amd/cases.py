@@ -269,6 +269,25 @@ def reference_boundary_case(boundary_xy: Tuple[np.ndarray, np.ndarray], nx: int 
     return Case(p, np.asarray(xb), np.asarray(yb), motion, n_steps, "reference_boundary")
 
 
+def tandem_reference_case(boundary_xy: Tuple[np.ndarray, np.ndarray], nx: int = 321, ny: int = 121,
+                          scale: float = 0.025, Re: float = 100.0, n_steps: int = 40, U: float = 0.05,
+                          spacing: float = 1.6) -> Case:
+    """The reference's two-wing setup (wing_count = 2, main.f90:18; its
+    boundary.dat and boundary1.dat hold the same 205-point polygon): wing 2
+    trails wing 1 by ``spacing`` chords (a constant in its sa column,
+    mode.f90:161-172) and heaves in antiphase."""
+    xb, yb = np.asarray(boundary_xy[0]), np.asarray(boundary_xy[1])
+    ox = -(nx // 4) * scale
+    oy = -(ny // 2) * scale
+    s = relaxation_rates(U, scale, Re, REFERENCE_S)
+    m = sinusoid_motion(n_steps, scale * U, 0.5, 10.0, 0.3, n_wings=2)
+    w2 = 1 + 9
+    m[:, w2 + 3] += spacing
+    m[:, w2 + 6:w2 + 9] *= -1.0
+    p = CaseParams(nx=nx, ny=ny, ox=ox, oy=oy, scale=scale, U=U, s=s, n_wings=2, pts_per_wing=len(xb))
+    return Case(p, np.concatenate([xb, xb]), np.concatenate([yb, yb]), m, n_steps, "tandem_reference")
+
+
 SWEEP_SEED = 20261203
 
 

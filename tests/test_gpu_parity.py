@@ -99,6 +99,19 @@ def test_reference_boundary_file(oracle_mod, lbmib_mod):
     assert_parity(o, g)
 
 
+def test_two_wings_reference_layout(oracle_mod, lbmib_mod):
+    """wing_count = 2 on the reference's polygon files (boundary.dat and
+    boundary1.dat are the same 205 points): tandem wings, antiphase heave;
+    per-wing force columns, IB node sets and fields bit-identical."""
+    xy = cases.read_boundary(os.path.join(GOLDEN, "boundary.dat"), 205)
+    case = cases.tandem_reference_case(xy, n_steps=40)
+    o, g = run_pair(case, 40, oracle_mod, lbmib_mod, chunks=[17, 23])
+    assert_parity(o, g)
+    h = g.history()
+    assert h.shape[1] == 4 + 12 * 2
+    assert not np.allclose(h[:, 1:4], h[:, 4:7])  # the wings feel different forces
+
+
 def test_residual_matches_definition(oracle_mod, lbmib_mod):
     case = cases.config(0, n_steps=30, scale_down=2)
     o, g = run_pair(case, 30, oracle_mod, lbmib_mod, chunks=[10, 20])
