@@ -29,6 +29,12 @@ and in FP64 from the same start.  value = FP32 MLUPS over the whole run; the
 JSON holds the FP64 rate, their ratio, the relative L2 differences of
 rho / u / v after the last step and the C_L / C_D errors of the FP32 run.
 
+--form post: post-processing of a step of the step workload on the device:
+output's vorticity with one vor_mom_rate update per call, and cal_sur_cp.
+value = nodes per second of the vorticity kernel; roofline over its 64
+algorithmic bytes per node (u, v and two history planes read, the vorticity,
+two history planes and the rate written).
+
 --form serve: the serving loop.  Requests (cases drawn as for the sweep with
 nx up to 4096 and 2000..20000 steps, log-uniform) arrive as a Poisson process
 at the rate that offers 80% of the device's capacity (measured first on a
@@ -366,6 +372,69 @@ def bench_precision(args):
     print(json.dumps(out))
 
 
+def bench_post(args):
+    """Post-processing on the step workload: output's vorticity with one
+    vor_mom_rate update per call (k_vorticity) and cal_sur_cp
+    (k_surface_pressure), on the device from the field of the last step."""
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    K, W = args.steps, args.warmup
+    case = cases.config(1, n_steps=W + 8)
+    p = case.params
+    g = lbmib.LBMIB(case, device=0)
+    g.evolution(max(W, 2))
+    for _ in range(3):
+        g.vorticity(rate=True, copy=False)
+    g.set_profiling(1)  # events around each vorticity kernel
+    t0 = time.perf_counter()
+    for _ in range(K):
+        g.vorticity(rate=True, copy=False)
+    dt = time.perf_counter() - t0
+    tm = g.timings()
+    n = p.nx * p.ny
+    kern_s = tm["vorticity_ms"] * 1e-3
+    bpn = 64  # u, v, two history planes read; vorticity, two history planes, rate written
+    achieved = n * bpn / kern_s / 1e9
+    t1 = time.perf_counter()
+    for _ in range(K):
+        g.surface_pressure()
+    sp_ms = (time.perf_counter() - t1) / K * 1e3
+    out = {
+        "metric": "vorticity + vor_mom_rate nodes per second on the device (post-processing of one step)",
+        "value": round(n / kern_s / 1e6, 1),
+        "unit": "Mnodes/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(dt / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA0012 markers and sinusoidal kinematics)",
+        "config": {"workload": "%s: %s, field after %d steps" % (case.name, WORKLOADS[1], max(W, 2)),
+                   "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_vorticity",
+                     "bytes_per_launch": n * bpn, "mean_launch_ms": round(kern_s * 1e3, 5)},
+        "surface_pressure_call_ms": round(sp_ms, 4),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        fl = g.fields(with_f=False)
+        st = pyoracle.VorticityRate(p.nx, p.ny, p.U, p.scale)
+        reps, t2 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t2 < 5.0:
+            pyoracle.vorticity(fl["u"], fl["v"], p.U, p.scale)
+            st.update(fl["u"], fl["v"])
+            reps += 1
+        cpu_s = (time.perf_counter() - t2) / reps
+        out["cpu_baseline"] = {"value": round(n / cpu_s / 1e6, 2), "unit": "Mnodes/s", "cores": 1, "kind": "port",
+                               "sample": "%d calls of the oracle's vorticity + vor_mom_rate on the same field" % reps}
+    g.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,7 +450,7 @@ def main():
                          "ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
     ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
                     help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
-    ap.add_argument("--form", choices=["step", "batch", "serve", "precision"], default="step",
+    ap.add_argument("--form", choices=["step", "batch", "serve", "precision", "post"], default="step",
                     help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
                          "serve: the continuous-batching service; precision: FP32 against FP64 on the "
                          "8192x4096 workload")
@@ -417,6 +486,8 @@ def main():
         return bench_serve(args)
     if args.form == "precision":
         return bench_precision(args)
+    if args.form == "post":
+        return bench_post(args)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
     fp32 = args.precision == "fp32"

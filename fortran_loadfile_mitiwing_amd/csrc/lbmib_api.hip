@@ -732,6 +732,11 @@ int lbmib_set_step(lbmib_handle* h, int32_t n)
     h->steps_done = n;
     h->geo_fresh = true;
     h->last_flag_step[0] = h->last_flag_step[1] = -1;
+    // a resumed run starts vor_mom_rate's history afresh, as a new process does
+    if (h->vort_hist) {
+        HIP_TRY(hipSetDevice(h->dev));
+        HIP_TRY(hipMemset(h->vort_hist, 0, sizeof(double) * 3 * h->D.N));
+    }
     return 0;
 }
 
@@ -753,6 +758,83 @@ int lbmib_residual(lbmib_handle* h, double* err)
     double t1 = 0.0, t2 = 0.0;
     for (int b = 0; b < blocks; ++b) { t1 += part[2 * b]; t2 += part[2 * b + 1]; }
     *err = std::sqrt(t1) / (std::sqrt(t2) + 1e-30);
+    return 0;
+}
+
+// Device memory of the post-processing calls, allocated on first use and
+// kept until lbmib_destroy.
+static int diag_alloc(lbmib_handle* h, double** p, size_t bytes)
+{
+    if (*p) return 0;
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, bytes));
+    h->allocs.push_back(q);
+    h->device_bytes += (long long)bytes;
+    *p = static_cast<double*>(q);
+    return 0;
+}
+
+int lbmib_vorticity(lbmib_handle* h, int32_t rate, double* vor, double* dvordt)
+{
+    if (!h) return fail(LBMIB_E_ARG, "null handle");
+    if (dvordt && !rate) return fail(LBMIB_E_ARG, "dvordt is given by the rate update only (rate != 0)");
+    HIP_TRY(hipSetDevice(h->dev));
+    HIP_TRY(hipDeviceSynchronize());
+    const long long N = h->D.N;
+    const int slot = h->steps_done & 1;
+    if (h->last_flag_step[slot] != h->steps_done)
+        return fail(LBMIB_E_STATE, "velocity field of step %d was not recorded", h->steps_done);
+    if (int rc = diag_alloc(h, &h->vort, sizeof(double) * N)) return rc;
+    if (rate && !h->vort_hist) {
+        if (int rc = diag_alloc(h, &h->vort_hist, sizeof(double) * 3 * N)) return rc;
+        HIP_TRY(hipMemset(h->vort_hist, 0, sizeof(double) * 3 * N));
+    }
+    double* hv = rate ? h->vort_hist : nullptr;
+    // with profiling on, the kernel is timed by events on its own stream
+    lbmib::Engine* E = h->own;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (E && E->prof_every > 0) {
+        HIP_TRY(hipEventCreate(&ev[0]));
+        HIP_TRY(hipEventCreate(&ev[1]));
+        HIP_TRY(hipEventRecord(ev[0], nullptr));
+    }
+    hipLaunchKernelGGL(k_vorticity, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, nullptr,
+                       (const double*)h->D.macro[slot], h->D.nx, h->D.ny, N, h->D.U, h->D.scale, h->vort, hv,
+                       hv ? hv + N : nullptr, hv ? hv + 2 * N : nullptr);
+    HIP_TRY(hipGetLastError());
+    if (ev[0]) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventRecord(ev[1], nullptr));
+        HIP_TRY(hipEventSynchronize(ev[1]));
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        E->tms[9] += ms;
+        E->tms[10] += 1.0;
+    }
+    if (vor) HIP_TRY(hipMemcpy(vor, h->vort, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (dvordt) HIP_TRY(hipMemcpy(dvordt, h->vort_hist + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (!vor && !dvordt) HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+int lbmib_surface_pressure(lbmib_handle* h, double* p, int32_t n)
+{
+    if (!h || !p) return fail(LBMIB_E_ARG, "null argument");
+    if (h->steps_done < 1 || h->geo_fresh) return fail(LBMIB_E_STATE, "no step taken yet");
+    if (h->p.smooth != 1)
+        return fail(LBMIB_E_ARG, "cal_sur_cp interpolates with the smoothed kernel (smooth = 1), the case uses smooth = 0");
+    const int ns = h->D.nseg;
+    if (n < ns) return fail(LBMIB_E_ARG, "room for %d markers needed (got %d)", ns, n);
+    HIP_TRY(hipSetDevice(h->dev));
+    HIP_TRY(hipDeviceSynchronize());
+    if (int rc = diag_alloc(h, &h->surf, sizeof(double) * ns)) return rc;
+    const Slot S = slot_of(h->D, (h->steps_done - 1) % h->D.nslot);
+    hipLaunchKernelGGL(k_surface_pressure, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, nullptr,
+                       (const int*)S.dcnt, (const int*)S.didx, (const double*)S.dval, (const double*)S.rhoadj, ns,
+                       h->surf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(p, h->surf, sizeof(double) * ns, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -43,6 +43,9 @@ struct lbmib_handle {
     int steps_done = 0;
     bool geo_fresh = true;   // the next run builds the geometry of its first step's batch first
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
+    double* vort = nullptr;           // post-processing: vorticity plane,
+    double* vort_hist = nullptr;      //   vor_mom_rate's vor, vor1 and dvordt planes,
+    double* surf = nullptr;           //   surface pressure per marker
     std::vector<void*> allocs;        // device allocations to free
     long long device_bytes = 0;       // their total size
     int dev = 0;

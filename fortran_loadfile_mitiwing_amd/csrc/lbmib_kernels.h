@@ -198,6 +198,8 @@ __global__ void k_clock_advance(CaseClock* __restrict__, int ncases, int steps);
 size_t factor_smem_bytes();
 size_t solve_smem_bytes(int npad);
 __global__ void k_residual(const double*, const double*, int, int, long long, double*);
+__global__ void k_vorticity(const double*, int, int, long long, double, double, double*, double*, double*, double*);
+__global__ void k_surface_pressure(const int*, const int*, const double*, const double*, int, double*);
 template <typename T> __global__ void k_nan_check(const T*, long long, int*);
 
 }  // namespace lbmib

@@ -1670,6 +1670,66 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ mc,
     }
 }
 
+// The vorticity column of output (solution.f90:181-210): second-order
+// one-sided differences on the first and last column (row), central ones
+// elsewhere, in units of U/scale.  With hv set, the same pass is one call of
+// vor_mom_rate (solution.f90:1880-1897) on the interior nodes: hv and hv1 hold
+// the vorticity of the last two calls and shift by one, dvdt gets the
+// second-order backward difference.  (vor_mom_rate's vor2 is written and
+// never read, so it is not kept.)  One thread per node in x-fastest order;
+// the j-1 and j+1 rows of u come through L2, so u and v are read from HBM
+// about once.
+__global__ __launch_bounds__(256) void k_vorticity(const double* __restrict__ m, int nx, int ny, long long N,
+                                                   double UU, double scale, double* __restrict__ vor,
+                                                   double* __restrict__ hv, double* __restrict__ hv1,
+                                                   double* __restrict__ dvdt)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const int j = (int)(c / nx), i = (int)(c - (long long)j * nx);
+    const double* u = m + N;
+    const double* v = m + 2 * N;
+    double dvx, duy;
+    if (i == nx - 1)
+        dvx = (1.5 * v[c] - 2.0 * v[c - 1] + 0.5 * v[c - 2]) / (UU * scale);
+    else if (i == 0)
+        dvx = (-1.5 * v[c] + 2.0 * v[c + 1] - 0.5 * v[c + 2]) / (UU * scale);
+    else
+        dvx = (v[c + 1] - v[c - 1]) / (UU * 2.0 * scale);
+    if (j == ny - 1)
+        duy = (1.5 * u[c] - 2.0 * u[c - nx] + 0.5 * u[c - 2 * nx]) / (UU * scale);
+    else if (j == 0)
+        duy = (-1.5 * u[c] + 2.0 * u[c + nx] - 0.5 * u[c + 2 * nx]) / (UU * scale);
+    else
+        duy = (u[c + nx] - u[c - nx]) / (UU * 2.0 * scale);
+    const double w = dvx - duy;
+    vor[c] = w;
+    if (hv && i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+        const double tem1 = hv1[c], tem2 = hv[c];
+        hv1[c] = tem2;
+        hv[c] = w;
+        dvdt[c] = (1.5 * w - 2.0 * tem2 + 0.5 * tem1) / (UU * scale);
+    }
+}
+
+// cal_sur_cp (solution.f90:1912-2118): the pressure rho/3 of the IB nodes
+// interpolated onto each marker midpoint with the step's kernel weights
+// phi_x phi_y, summed in ascending node order as matmulvocxs does
+// (boundary_condition.f90:1887-1900).  One lane per marker.
+__global__ __launch_bounds__(64) void k_surface_pressure(const int* __restrict__ dcnt, const int* __restrict__ didx,
+                                                         const double* __restrict__ dval,
+                                                         const double* __restrict__ rhoadj, int nseg,
+                                                         double* __restrict__ out)
+{
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= nseg) return;
+    const int n = min(dcnt[k], BOXMAX);
+    double bb = 0.0;
+    for (int p = 0; p < n; ++p)
+        bb = bb + dval[k * BOXMAX + p] * (rhoadj[didx[k * BOXMAX + p]] / 3.0);
+    out[k] = bb;
+}
+
 // NaN guard over the populations.
 template <typename T>
 __global__ __launch_bounds__(256) void k_nan_check(const T* __restrict__ f, long long n, int* __restrict__ flag)

@@ -26,11 +26,14 @@ program lbmib_driver
     ! restart state every restart_every steps (0: none); restart_in resumes
     ! from such a file
     integer :: output_every = 0, output_stride = 2, restart_every = 0
+    ! vorticity_rate = 1 calls vor_mom_rate after every step (the call
+    ! main.f90:391 leaves commented out) and fills the dvordt column
+    integer :: vorticity_rate = 0
     character(len=256) :: output_dir = '.', restart_out = 'saved_flow_feild.dat', restart_in = ''
     namelist /lbmib_case/ nx, ny, ox, oy, scale, UU, xc, yc, xp, yp, s, smooth, wing_count, &
         points_per_wing, boundary_dir, motion_file, force_file, motion_out_file, fields_file, &
         chunk, device, report_every, output_every, output_stride, restart_every, output_dir, &
-        restart_out, restart_in
+        restart_out, restart_in, vorticity_rate
 
     type(lbmib_params) :: p
     type(c_ptr) :: h
@@ -38,6 +41,7 @@ program lbmib_driver
     character(len=32) :: idx_text
     real(c_double), allocatable :: xb0(:), yb0(:), motion(:, :), hist(:, :), row(:)
     real(c_double), allocatable, target :: rho(:, :), u(:, :), v(:, :), f(:, :, :)
+    real(c_double), allocatable, target :: vor(:, :), dvr(:, :)
     real(c_double) :: err, feq0(9)
     integer :: unit_id, ios, wing, i0, ps, ncols, nrows, nmax, n, k, cols, col, n0
     integer(c_int32_t) :: got
@@ -137,7 +141,8 @@ program lbmib_driver
     call lbmib_check(lbmib_max_steps(h, got), 'lbmib_max_steps')
     nmax = got
     write(*, '(a, a, i0, a)') trim(lbmib_version_string()), ': ', nmax, ' steps'
-    allocate(rho(nx, ny), u(nx, ny), v(nx, ny))
+    allocate(rho(nx, ny), u(nx, ny), v(nx, ny), vor(nx, ny), dvr(nx, ny))
+    dvr = 0.0d0
     if (restart_every .gt. 0 .or. len_trim(restart_in) .gt. 0) allocate(f(nx, ny, 9))
     ! the stale ring populations f0 and the coarse-to-fine copies the
     ! restart file carries stay at the initial equilibrium: f of any node now
@@ -164,8 +169,20 @@ program lbmib_driver
         if (report_every .gt. 0) k = min(k, report_every - mod(n, report_every))
         if (output_every .gt. 0) k = min(k, output_every - mod(n, output_every))
         if (restart_every .gt. 0) k = min(k, restart_every - mod(n, restart_every))
+        if (vorticity_rate .ne. 0) k = 1
         call lbmib_check(lbmib_step(h, int(k, c_int32_t)), 'lbmib_step')
         n = n + k
+        if (vorticity_rate .ne. 0) then
+            if (output_every .gt. 0) then
+                if (mod(n, output_every) .eq. 0) then
+                    call lbmib_check(lbmib_vorticity(h, 1_c_int32_t, c_loc(vor), c_loc(dvr)), 'vor_mom_rate')
+                else
+                    call lbmib_check(lbmib_vorticity(h, 1_c_int32_t, c_null_ptr, c_null_ptr), 'vor_mom_rate')
+                endif
+            else
+                call lbmib_check(lbmib_vorticity(h, 1_c_int32_t, c_null_ptr, c_null_ptr), 'vor_mom_rate')
+            endif
+        endif
         if (report_every .gt. 0) then
             if (mod(n, report_every) .eq. 0 .and. n .ge. n0 + 2) then
                 call lbmib_check(lbmib_residual(h, err), 'lbmib_residual')
@@ -223,10 +240,11 @@ contains
 
     ! output (solution.f90:129-238) for one mesh without domain shifting:
     ! a point zone of X, Y, U/UU, V/UU, the pressure rho/3, the vorticity
-    ! (central differences, one-sided on the edges), the IB-node mask and the
-    ! vorticity rate (left zero: its routine is not called, main.f90:391) at
-    ! every output_stride-th node; a triangle zone of the marker polygons
-    ! with one centre point per wing.
+    ! (central differences, one-sided on the edges, computed on the device),
+    ! the IB-node mask and the vorticity rate (zero unless vorticity_rate is
+    ! set: the reference leaves its routine uncalled, main.f90:391) at every
+    ! output_stride-th node; a triangle zone of the marker polygons with one
+    ! centre point per wing.
     subroutine write_tecplot(cycle)
         integer, intent(in) :: cycle
         integer :: i, j, a, o, out_nx, out_ny, npts, kk, jp
@@ -234,9 +252,11 @@ contains
         integer, allocatable :: body(:, :)
         integer(c_int32_t), allocatable, target :: pi(:), pj(:)
         real(c_double), allocatable :: xb1(:), yb1(:)
-        real(c_double) :: dvx, duy
         character(len=32) :: nm
         call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_null_ptr), 'lbmib_download')
+        ! with vorticity_rate the step's vor_mom_rate call has filled vor and dvr
+        if (vorticity_rate .eq. 0) &
+            call lbmib_check(lbmib_vorticity(h, 0_c_int32_t, c_loc(vor), c_null_ptr), 'lbmib_vorticity')
         allocate(body(nx, ny))
         body = 0
         na = 0
@@ -264,22 +284,8 @@ contains
             cycle * UU * scale
         do j = 1, ny, output_stride
             do i = 1, nx, output_stride
-                if (i .eq. nx) then
-                    dvx = (1.5 * v(i, j) - 2.0 * v(i - 1, j) + 0.5 * v(i - 2, j)) / (UU * scale)
-                else if (i .eq. 1) then
-                    dvx = (-1.5 * v(i, j) + 2.0 * v(i + 1, j) - 0.5 * v(i + 2, j)) / (UU * scale)
-                else
-                    dvx = (v(i + 1, j) - v(i - 1, j)) / (UU * 2.0 * scale)
-                endif
-                if (j .eq. ny) then
-                    duy = (1.5 * u(i, j) - 2.0 * u(i, j - 1) + 0.5 * u(i, j - 2)) / (UU * scale)
-                else if (j .eq. 1) then
-                    duy = (-1.5 * u(i, j) + 2.0 * u(i, j + 1) - 0.5 * u(i, j + 2)) / (UU * scale)
-                else
-                    duy = (u(i, j + 1) - u(i, j - 1)) / (UU * 2.0 * scale)
-                endif
                 write(o, '(25e16.8)') ox + dble(i - 1) * scale, oy + dble(j - 1) * scale, u(i, j) / UU, &
-                    v(i, j) / UU, rho(i, j) / 3.0d0, (dvx - duy), dble(body(i, j)), 0.0d0
+                    v(i, j) / UU, rho(i, j) / 3.0d0, vor(i, j), dble(body(i, j)), dvr(i, j)
             enddo
         enddo
         write(o, '(a, i0, a, i0, a, a, es24.16)') ' ZONE T="airfoil", NODES=', npts + wing_count, ' ELEMENTS,=', &

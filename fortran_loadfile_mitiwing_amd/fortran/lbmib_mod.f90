@@ -22,7 +22,7 @@ module lbmib
     integer(c_int32_t), parameter, public :: LBMIB_E_NAN = -7
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP64 = 0
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP32DEV = 1
-    integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 9
+    integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 11
     integer(c_int32_t), parameter, public :: LBMIB_GEOMETRY_BATCH = 16
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_AUTO = 0
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_CHAIN = 1
@@ -50,7 +50,7 @@ module lbmib
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
     public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings, lbmib_set_step
-    public :: lbmib_markers
+    public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure
     public :: lbmib_destroy, lbmib_error_message, lbmib_version_string, lbmib_check
     ! batched form (many cases advanced together) and footprints
     public :: lbmib_set_shape, lbmib_get_shape, lbmib_device_bytes, lbmib_mem_info
@@ -119,6 +119,20 @@ module lbmib
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: h
             real(c_double), intent(out) :: xb1(*), yb1(*)
+        end function
+
+        ! vor, dvordt: c_loc of nx*ny arrays or c_null_ptr
+        integer(c_int) function lbmib_vorticity(h, rate, vor, dvordt) bind(C, name="lbmib_vorticity")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h, vor, dvordt
+            integer(c_int32_t), value :: rate
+        end function
+
+        integer(c_int) function lbmib_surface_pressure(h, p, n) bind(C, name="lbmib_surface_pressure")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: p(*)
+            integer(c_int32_t), value :: n
         end function
 
         integer(c_int) function lbmib_set_step(h, n) bind(C, name="lbmib_set_step")

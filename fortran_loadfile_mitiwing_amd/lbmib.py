@@ -42,7 +42,7 @@ GEOMETRY_BATCH = 16  # steps per geometry batch (LBMIB_GEOMETRY_BATCH)
 SOLVER_AUTO = 0       # LBMIB_SOLVER_*: chain for FP64, separator form for FP32
 SOLVER_CHAIN = 1
 SOLVER_SEPARATOR = 2
-NTIMINGS = 9  # LBMIB_NTIMINGS
+NTIMINGS = 11  # LBMIB_NTIMINGS
 
 
 class LBMIBError(RuntimeError):
@@ -85,7 +85,7 @@ _lib = None
 
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
-           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
@@ -113,6 +113,8 @@ def load(path: str = LIB_PATH):
     L.lbmib_upload_state.argtypes = [vp, _dp]
     L.lbmib_set_step.argtypes = [vp, C.c_int32]
     L.lbmib_markers.argtypes = [vp, _dp, _dp]
+    L.lbmib_vorticity.argtypes = [vp, C.c_int32, _dp, _dp]
+    L.lbmib_surface_pressure.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_residual.argtypes = [vp, _dp]
     L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
@@ -264,6 +266,24 @@ class LBMIB:
         _check(load().lbmib_markers(self.h, _d(x), _d(y)))
         return x, y
 
+    def vorticity(self, rate: bool = False, copy: bool = True):
+        """output's vorticity (ny, nx) of the last step; with ``rate`` also one
+        vor_mom_rate update and its dvordt.  Returns (vor, dvordt or None);
+        with copy=False nothing comes back to the host."""
+        p = self.p
+        vor = np.zeros((p.ny, p.nx)) if copy else None
+        dv = np.zeros((p.ny, p.nx)) if (copy and rate) else None
+        _check(load().lbmib_vorticity(self.h, 1 if rate else 0, _d(vor) if vor is not None else None,
+                                      _d(dv) if dv is not None else None))
+        return vor, dv
+
+    def surface_pressure(self) -> np.ndarray:
+        """cal_sur_cp: pressure at each marker midpoint of the last step."""
+        n = self.p.n_wings * (self.p.pts_per_wing - 1)
+        out = np.zeros(n)
+        _check(load().lbmib_surface_pressure(self.h, _d(out), n))
+        return out
+
     def ib_nodes(self):
         n = C.c_int32()
         L = load()
@@ -305,7 +325,8 @@ def _timings(t) -> dict:
     ns, nb = max(t[2], 1.0), max(t[8], 1.0)
     return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / ns, ib_solve_ms=t[4] / ns,
                 ib_collide_ms=t[5] / ns, geometry_batch_ms=t[6] / nb, factor_batch_ms=t[7] / nb,
-                steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]))
+                steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]),
+                vorticity_ms=t[9] / max(t[10], 1.0), vorticity_calls=int(t[10]))
 
 
 class Batch:

@@ -155,6 +155,24 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
  * output (solution.f90:219-233). */
 int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1);
 
+/* Replaces the vorticity of output (solution.f90:181-210) and vor_mom_rate
+ * (solution.f90:1841-1900, main.f90:391), computed on the device from the
+ * last step's velocity field.  vor (nx*ny) gets dv/dx - du/dy in units of
+ * U/scale, one-sided on the edge columns and rows.  With rate != 0 the call
+ * is also one vor_mom_rate call: the vorticity history of the last two calls
+ * (zero before the first, and again after lbmib_set_step) shifts by one and
+ * dvordt (nx*ny, may be NULL) gets the second-order backward difference on
+ * the interior, zero on the edges; call it once per step for the reference's
+ * per-step rate.  vor may be NULL (the result stays on the device).  The
+ * first call allocates the planes (one, four with rate). */
+int lbmib_vorticity(lbmib_handle* h, int32_t rate, double* vor, double* dvordt);
+
+/* Replaces cal_sur_cp (solution.f90:1912-2118): the pressure rho/3 of the
+ * last step's IB nodes interpolated onto each marker midpoint with the
+ * smoothed kernel, one value per marker segment (n >= the case's segment
+ * count, n_wings * (pts_per_wing - 1)).  Needs smooth = 1. */
+int lbmib_surface_pressure(lbmib_handle* h, double* p, int32_t n);
+
 /* Device timing of the hot path (the reference's cpu_time table,
  * main.f90:338-369, 420-433), measured with HIP events on the streams the
  * kernels run on.  lbmib_set_profiling(h, every) resets the counters and,
@@ -168,8 +186,10 @@ int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1);
  *   [5] ms in the forced collide of the IB nodes ([3]-[5] summed over the
  *       sampled steps)
  *   [6] ms of geometry batches before the factorisation, [7] ms of their
- *       factorisations, [8] batches timed (geometry overlaps the steps). */
-#define LBMIB_NTIMINGS 9
+ *       factorisations, [8] batches timed (geometry overlaps the steps)
+ *   [9] ms in the vorticity kernel of lbmib_vorticity, [10] its calls timed
+ *       (a handle's own counters; zero for a batch). */
+#define LBMIB_NTIMINGS 11
 int lbmib_set_profiling(lbmib_handle* h, int32_t every);
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 

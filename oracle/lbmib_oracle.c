@@ -1332,3 +1332,80 @@ void orc_markers(const orc_case* c, double* xb2, double* yb2, double* ub, double
     if (vb) memcpy(vb, c->vb, sizeof(double) * c->nseg);
     if (arc_l) memcpy(arc_l, c->arc_l, sizeof(double) * c->nseg);
 }
+
+/* ------------------------------------------------------------------ */
+/* post-processing                                                     */
+/* ------------------------------------------------------------------ */
+
+/* The Vorticity column of output (solution.f90:181-210): dv/dx - du/dy in
+ * units of U/scale, second-order one-sided differences on the first and last
+ * column (row), central differences elsewhere. */
+void orc_vorticity(int nx, int ny, double UU, double scale, const double* u, const double* v, double* vor)
+{
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const size_t c = (size_t)j * nx + i;
+            double dvx, duy;
+            if (i == nx - 1)
+                dvx = (1.5 * v[c] - 2.0 * v[c - 1] + 0.5 * v[c - 2]) / (UU * scale);
+            else if (i == 0)
+                dvx = (-1.5 * v[c] + 2.0 * v[c + 1] - 0.5 * v[c + 2]) / (UU * scale);
+            else
+                dvx = (v[c + 1] - v[c - 1]) / (UU * 2.0 * scale);
+            if (j == ny - 1)
+                duy = (1.5 * u[c] - 2.0 * u[c - nx] + 0.5 * u[c - 2 * (size_t)nx]) / (UU * scale);
+            else if (j == 0)
+                duy = (-1.5 * u[c] + 2.0 * u[c + nx] - 0.5 * u[c + 2 * (size_t)nx]) / (UU * scale);
+            else
+                duy = (u[c + nx] - u[c - nx]) / (UU * 2.0 * scale);
+            vor[c] = dvx - duy;
+        }
+}
+
+/* One call of vor_mom_rate (solution.f90:1880-1897) on the interior nodes:
+ * vor and vor1 hold the vorticity of the last two calls (shifted here), the
+ * rate is the second-order backward difference in units of U/scale per step.
+ * Edge nodes are left untouched. */
+void orc_vorticity_rate(int nx, int ny, double UU, double scale, const double* u, const double* v,
+                        double* vor, double* vor1, double* dvordt)
+{
+    for (int j = 1; j < ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+            const size_t c = (size_t)j * nx + i;
+            const double tem1 = vor1[c];
+            const double tem2 = vor[c];
+            vor1[c] = tem2;
+            const double dvx = (v[c + 1] - v[c - 1]) / (UU * 2.0 * scale);
+            const double duy = (u[c + nx] - u[c - nx]) / (UU * 2.0 * scale);
+            const double tem = dvx - duy;
+            vor[c] = tem;
+            dvordt[c] = (1.5 * tem - 2.0 * tem2 + 0.5 * tem1) / (UU * scale);
+        }
+}
+
+/* cal_sur_cp (solution.f90:1912-2118, single process): the pressure at each
+ * marker midpoint, sum over the IB nodes in list order of
+ * phi(x_i - xb2) phi(y_i - yb2) pre_i with the smoothed kernel (smooth = 1),
+ * nodes outside the kernel's support skipped (matmulvocxs over the sparse
+ * rows, boundary_condition.f90:1869-1900).  pre = NULL takes the case's own
+ * pressure field rho/3; any other field of nx*ny values may be given.  One
+ * value per marker of the live IB (nseg). */
+int orc_surface_pressure(const orc_case* c, const double* pre, double* cp)
+{
+    if (!pre) pre = c->pre;
+    const double sc = c->p.scale;
+    for (int k = 0; k < c->nseg; ++k) {
+        double bb = 0.0;
+        for (int a = 0; a < c->nadj; ++a) {
+            const double x0 = c->p.ox + sc * (double)c->adj_i[a];
+            const double y0 = c->p.oy + sc * (double)c->adj_j[a];
+            const double dx = fabs((x0 - c->xb2[k]) / sc);
+            const double dy = fabs((y0 - c->yb2[k]) / sc);
+            if (dx > 2.5 || dy > 2.5) continue;
+            const double w = orc_delta(dx, 1) * orc_delta(dy, 1);
+            bb = bb + w * pre[(size_t)c->adj_j[a] * c->nx + c->adj_i[a]];
+        }
+        cp[k] = bb;
+    }
+    return c->nseg;
+}

@@ -106,6 +106,14 @@ void orc_markers(const orc_case* c, double* xb2, double* yb2, double* ub, double
 int orc_build_ib_set(const orc_case* c, const double* xb2, const double* yb2,
                      int* pi, int* pj, int* wing, int max_nodes);
 
+/* Post-processing: output's vorticity column (solution.f90:181-210), one
+ * vor_mom_rate call (solution.f90:1841-1900) and cal_sur_cp
+ * (solution.f90:1912-2118). */
+void orc_vorticity(int nx, int ny, double UU, double scale, const double* u, const double* v, double* vor);
+void orc_vorticity_rate(int nx, int ny, double UU, double scale, const double* u, const double* v,
+                        double* vor, double* vor1, double* dvordt);
+int orc_surface_pressure(const orc_case* c, const double* pre, double* cp);
+
 /* ---- component-level functions (unit tests) ---- */
 void orc_weights(double w[9]);                    /* D2Q9model.f90:24-32 (REAL(4) weights) */
 void orc_minv(double minv[81]);                   /* nizhen on matm (solution.f90:17-27,1462-1517), row-major */

@@ -68,6 +68,9 @@ def lib():
         L.orc_motion_step_table.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_solve.argtypes = [_dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp]
         L.orc_collide_node.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
+        L.orc_vorticity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_vorticity_rate.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.orc_surface_pressure.argtypes = [C.c_void_p, _dp, _dp]
         _lib = L
     return _lib
 
@@ -124,6 +127,34 @@ def motion_step_table(motion: np.ndarray, dt: float) -> np.ndarray:
     rc = lib().orc_motion_step_table(_d(m), rows, ncols, dt, _d(out), n)
     assert rc == n, rc
     return out
+
+
+def vorticity(u: np.ndarray, v: np.ndarray, U: float, scale: float) -> np.ndarray:
+    """output's vorticity column (solution.f90:181-210) of (ny, nx) fields."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    ny, nx = u.shape
+    out = np.zeros((ny, nx))
+    lib().orc_vorticity(nx, ny, U, scale, _d(u), _d(v), _d(out))
+    return out
+
+
+class VorticityRate:
+    """vor_mom_rate's state (solution.f90:1841-1900): vorticity of the last
+    two calls, zero before the first, and the rate of the last call."""
+
+    def __init__(self, nx: int, ny: int, U: float, scale: float):
+        self.nx, self.ny, self.U, self.scale = nx, ny, U, scale
+        self.vor = np.zeros((ny, nx))
+        self.vor1 = np.zeros((ny, nx))
+        self.dvordt = np.zeros((ny, nx))
+
+    def update(self, u: np.ndarray, v: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lib().orc_vorticity_rate(self.nx, self.ny, self.U, self.scale, _d(u), _d(v),
+                                 _d(self.vor), _d(self.vor1), _d(self.dvordt))
+        return self.dvordt.copy()
 
 
 def solve(G: np.ndarray, S: np.ndarray, b: np.ndarray, perm: np.ndarray, solver: int) -> np.ndarray:
@@ -221,6 +252,17 @@ class Oracle:
         a = [np.zeros(n) for _ in range(5)]
         lib().orc_markers(self.h, *[_d(x) for x in a])
         return dict(xb2=a[0], yb2=a[1], ub=a[2], vb=a[3], arc_l=a[4])
+
+    def surface_pressure(self, pre=None) -> np.ndarray:
+        """cal_sur_cp on the last step's markers and IB nodes; ``pre`` (ny, nx)
+        replaces the case's own rho/3."""
+        n = lib().orc_n_segments(self.h)
+        out = np.zeros(n)
+        if pre is not None:
+            pre = np.ascontiguousarray(pre, dtype=np.float64).reshape(-1)
+            assert pre.size == self.N
+        lib().orc_surface_pressure(self.h, _d(pre) if pre is not None else None, _d(out))
+        return out
 
     def build_ib_set(self, xb2, yb2, max_nodes=1 << 20):
         xb2 = np.ascontiguousarray(xb2, dtype=np.float64)

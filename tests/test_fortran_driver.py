@@ -171,3 +171,30 @@ def test_driver_tecplot_flow_field_file(tmp_path, oracle_mod):
     tri = [ln.split() for ln in rest[1 + npts + p.n_wings:]]
     assert len(tri) == p.n_wings * (p.pts_per_wing - 1)
     assert tri[0] == ["001", "002", "%03d" % (npts + 1)]
+
+
+@pytest.mark.gpu
+def test_driver_vorticity_rate_column(tmp_path, oracle_mod):
+    """vorticity_rate = 1 calls vor_mom_rate after every step (the call the
+    reference leaves commented out, main.f90:391); F<n>.dat's dvordt column
+    then holds the rate of the oracle's history updated at every step."""
+    exe = driver()
+    case = cases.config(0, n_steps=12, scale_down=2)
+    d = str(tmp_path)
+    write_inputs(d, case, output_every=12, vorticity_rate=1, output_stride=1)
+    run_driver(exe, d)
+    xr, yr = cases.read_boundary(os.path.join(d, "boundary.dat"), len(case.xb0))
+    case.xb0, case.yb0 = xr, yr
+    p = case.params
+    o = oracle_mod.Oracle(case)
+    st = oracle_mod.VorticityRate(p.nx, p.ny, p.U, p.scale)
+    for _ in range(12):
+        o.step(1)
+        fl = o.fields()
+        rate = st.update(fl["u"], fl["v"])
+    vor = oracle_mod.vorticity(fl["u"], fl["v"], p.U, p.scale)
+    lines = open(os.path.join(d, "F12.dat")).read().splitlines()
+    data = np.array([[float(x) for x in ln.split()] for ln in lines[2:2 + p.nx * p.ny]]).reshape(p.ny, p.nx, 8)
+    assert np.allclose(data[:, :, 5], vor, rtol=1e-7, atol=1e-9)
+    assert np.allclose(data[:, :, 7], rate, rtol=1e-7, atol=1e-9)
+    assert np.abs(rate).max() > 0.0 and np.all(data[0, :, 7] == 0.0)
