@@ -183,6 +183,21 @@ class CaseParams:
         return self.scale * self.U
 
 
+@dataclasses.dataclass
+class Dynamics:
+    """Rigid-body dynamics of wing 1 (solve_dynamic_equations, main.f90:101-129):
+    x-translation driven by the IB force, optionally a pitch equation with
+    par1/par2 (zero in the reference's build)."""
+    enabled: bool = True
+    rk4: bool = True
+    closed_boundary: bool = True
+    solid_den: float = 10.0
+    area_airfoil: float = 6.807946476488794e-2
+    par1: float = 0.0
+    par2: float = 0.0
+    alphad: float = 10.0 * math.pi / 180.0
+
+
 def relaxation_rates(U: float, scale: float, Re: float,
                      fixed: Sequence[float] = STANDARD_S) -> Tuple[float, ...]:
     """s(1..9) with s8 = s9 = 1/tau, tau = 3 nu + 1/2, nu = U (1/scale)/Re

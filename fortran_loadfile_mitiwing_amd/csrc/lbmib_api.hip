@@ -334,6 +334,12 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     }
     TRY(dalloc(h, &D.f0L, 9 * (size_t)ny));
     TRY(dalloc(h, &D.f0R, 9 * (size_t)ny));
+    {
+        double* body = nullptr;  // body offset terms, zero until the dynamics move the body
+        TRY(dalloc(h, &body, 32));
+        HIP_TRY(hipMemset(body, 0, sizeof(double) * 32));
+        D.body = body;
+    }
     TRY(dalloc(h, &D.macro[0], 3 * (size_t)N));
     TRY(dalloc(h, &D.macro[1], 3 * (size_t)N));
     // init (solution.f90:34-59): f = f0 = feq(1, U, 0); rho = 1, u = U, v = 0
@@ -476,7 +482,10 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         }
         if (close) D.bw = D.nseg - 1;
         const double sc = h->p.scale;
-        int i0 = (int)std::floor((X0 - h->p.ox) / sc) - 6, i1 = (int)std::ceil((X1 - h->p.ox) / sc) + 6;
+        // with the dynamics the body drifts up to one node from its prescribed
+        // path before the domain follows it
+        const int mx = h->dyn.enabled ? 9 : 6;
+        int i0 = (int)std::floor((X0 - h->p.ox) / sc) - mx, i1 = (int)std::ceil((X1 - h->p.ox) / sc) + mx;
         int j0 = (int)std::floor((Y0 - h->p.oy) / sc) - 6, j1 = (int)std::ceil((Y1 - h->p.oy) / sc) + 6;
         i0 = std::max(0, i0); j0 = std::max(0, j0);
         i1 = std::min(D.nx - 1, i1); j1 = std::min(D.ny - 1, j1);
@@ -518,6 +527,12 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
     D.kin = dkin;
     D.steps_cap = maxn;
     if (dalloc(h, &D.hist, (size_t)maxn * 3 * nw)) return LBMIB_E_HIP;
+    if (h->dyn.enabled) {
+        // each step's geometry follows the body state after the step before
+        if (dalloc(h, &D.raw, (size_t)maxn * 3 * nw)) return LBMIB_E_HIP;
+        D.batch = 1;
+        h->own->per_step = true;
+    }
     // slot pools
     {
         const long long ns = D.nseg, cap = D.cap, W = D.bw + 1, grid = (long long)D.gw * D.gh;
@@ -603,6 +618,227 @@ int nan_guard_check(lbmib_handle* h)
 
 extern "C" {
 
+// ---- rigid-body dynamics (the host side of the reference's own host code) ----
+
+// update_body_extra_from_rel (mode.f90:285-307): running sums over wings
+static void body_extra(lbmib_handle* h)
+{
+    double av = 0.0, as = 0.0, ap = 0.0;
+    for (int w = 0; w < 8; ++w) h->ext[0][w] = h->ext[1][w] = h->ext[2][w] = 0.0;
+    for (int w = 0; w < h->p.n_wings; ++w) {
+        av = av + h->rel[2][w];
+        as = as + h->rel[4][w];
+        ap = ap + h->rel[6][w];
+        h->ext[0][w] = av;
+        h->ext[1][w] = as;
+        h->ext[2][w] = ap;
+    }
+}
+
+// rk4 (boundary_condition.f90:3168-3202): the acceleration is held over the
+// four stages; the pitch equation reads alphad and the last moment
+static void body_rk4(const lbmib_handle* h, double s1, double& s, double& s2, double v1, double& v, double a1,
+                     double jd1, double& jd, double jsd1, double& jsd, double p1, double p2)
+{
+    const double alphad = h->dyn.alphad, mom = h->mom;
+    double k11 = v1, k21 = a1;
+    double k12 = v1 + 0.5 * k21, k22 = a1;
+    double k13 = v1 + 0.5 * k22, k23 = a1;
+    double k14 = v1 + k23, k24 = a1;
+    s = s1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    v = v1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+    s2 = s2 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    k11 = jsd1;
+    k21 = p1 * (jd1 + alphad) + p2 * mom;
+    k12 = jsd1 + 0.5 * k21;
+    k22 = p1 * (jd1 + alphad + 0.5 * k11) + p2 * mom;
+    k13 = jsd1 + 0.5 * k22;
+    k23 = p1 * (jd1 + alphad + 0.5 * k12) + p2 * mom;
+    k14 = jsd1 + k23;
+    k24 = p1 * (jd1 + alphad + k13) + p2 * mom;
+    jd = jd1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    jsd = jsd1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+}
+
+// the dynamics block of immersed_boundary_Wu (boundary_condition.f90:1676-1719)
+// on the raw per-wing forces of the step
+static void body_update(lbmib_handle* h, const double* raw)
+{
+    const double sc = h->p.scale, UU = h->p.U;
+    const lbmib_dynamics& d = h->dyn;
+    double (*r)[8] = h->rel;
+    double force1 = 0.0;
+    for (int w = 0; w < h->p.n_wings; ++w) force1 = force1 + raw[3 * w];
+    h->mom = raw[2];
+    r[1][0] = r[0][0];
+    if (!d.closed_boundary)
+        r[0][0] = force1 / (d.solid_den / (sc * sc));
+    else
+        r[0][0] = force1 / (d.solid_den * d.area_airfoil / (sc * sc));
+    r[3][0] = r[2][0];
+    r[5][0] = r[4][0];
+    r[7][0] = r[8][0];
+    r[9][0] = r[10][0];
+    if (!d.rk4) {
+        r[2][0] = r[0][0] + r[2][0];
+        r[4][0] = r[4][0] + r[3][0];
+        r[6][0] = r[6][0] + r[3][0];
+        r[8][0] = r[7][0] + r[9][0];
+        r[10][0] = r[9][0] + d.par1 * UU * UU * sc * sc * (r[8][0] + d.alphad) + d.par2 * (sc * sc * sc * sc) * h->mom;
+    } else {
+        body_rk4(h, r[5][0], r[4][0], r[6][0], r[3][0], r[2][0], r[0][0], r[7][0], r[8][0], r[9][0], r[10][0],
+                 d.par1 * UU * UU * sc * sc, d.par2 * (sc * sc * sc * sc));
+    }
+    body_extra(h);
+}
+
+// the body offset terms the geometry and force kernels add (airfoil.f90:262,
+// boundary_condition.f90:1548, 1672)
+static int upload_body(lbmib_handle* h)
+{
+    double b[32] = {};
+    for (int w = 0; w < h->p.n_wings; ++w) {
+        b[4 * w] = h->ext[1][w] / (1.0 / h->p.scale);
+        b[4 * w + 1] = (double)h->movenum * h->p.scale;
+        b[4 * w + 2] = h->ext[0][w];
+        b[4 * w + 3] = h->ext[2][w];
+    }
+    HIP_TRY(hipMemcpy(const_cast<double*>(h->D.body), b, sizeof b, hipMemcpyHostToDevice));
+    std::memcpy(h->body_terms, b, sizeof b);
+    return 0;
+}
+
+// move_mesh on the device state after the last step: populations, the
+// refilled edge column's stale f0, and the recorded rho/u/v of the step
+// and the step before
+static int move_domain(lbmib_handle* h, int dr)
+{
+    DevCase& D = h->D;
+    const long long N = D.N;
+    const int nx = D.nx, ny = D.ny, slot = h->steps_done & 1;
+    RingW w;
+    for (int k = 0; k < 9; ++k) w.w[k] = D.W[k];
+    const dim3 gp((unsigned)((9 * N + 255) / 256)), gr((unsigned)((9 * ny + 255) / 256)), b(256);
+    double* f0col = dr == 1 ? D.f0L : D.f0R;
+    if (D.prec == LBMIB_PREC_FP64) {
+        hipLaunchKernelGGL(k_move_ring<double>, gr, b, 0, nullptr, (const double*)D.f[slot], f0col, nx, ny, N, dr, w);
+        hipLaunchKernelGGL(k_move_pops<double>, gp, b, 0, nullptr, (const double*)D.f[slot], D.f[slot ^ 1], nx, N, dr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(D.f[slot], D.f[slot ^ 1], sizeof(double) * 9 * N, hipMemcpyDeviceToDevice, nullptr));
+    } else {
+        hipLaunchKernelGGL(k_move_ring<float>, gr, b, 0, nullptr, (const float*)D.g[slot], f0col, nx, ny, N, dr, w);
+        hipLaunchKernelGGL(k_move_pops<float>, gp, b, 0, nullptr, (const float*)D.g[slot], D.g[slot ^ 1], nx, N, dr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(D.g[slot], D.g[slot ^ 1], sizeof(float) * 9 * N, hipMemcpyDeviceToDevice, nullptr));
+    }
+    if (h->last_flag_step[slot] == h->steps_done) {
+        const dim3 gm((unsigned)((3 * N + 255) / 256));
+        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[slot], D.macro[slot ^ 1], nx, N, dr, 0);
+        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[slot], D.macro[slot ^ 1], nx, N, dr, 1);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+// One step with the dynamics on: the body offsets of the state so far, the
+// step (its geometry built first), the body update from the step's raw
+// forces, the airfoil_motion row, and the domain shift when the body has
+// moved a node (main.f90:370-388).
+static int dynamics_step(lbmib_handle* h)
+{
+    const int nw = h->p.n_wings;
+    if (int rc = upload_body(h)) return rc;
+    h->geo_fresh = true;
+    if (int rc = engine_run(h->own, 1)) return rc;
+    if (int rc = nan_guard_check(h)) return rc;
+    const int n = h->steps_done;
+    double raw[24];
+    HIP_TRY(hipMemcpy(raw, h->D.raw + (size_t)(n - 1) * 3 * nw, sizeof(double) * 3 * nw, hipMemcpyDeviceToHost));
+    body_update(h, raw);
+    const double UU = h->p.U, sc = h->p.scale;
+    h->mrows.resize((size_t)n * (1 + 3 * nw));
+    double* m = &h->mrows[(size_t)(n - 1) * (1 + 3 * nw)];
+    int mc = 0;
+    m[mc++] = (double)(n - 1) * (sc * UU);
+    for (int w = 0; w < nw; ++w) {
+        m[mc++] = 0.0;
+        m[mc++] = h->ext[0][w] / UU;
+        m[mc++] = h->ext[1][w] / (1.0 / sc);
+    }
+    double& pos = h->rel[6][0];
+    if (pos >= 1.0 || pos <= -1.0) {
+        const int dr = pos >= 1.0 ? 1 : -1;
+        if (int rc = move_domain(h, dr)) return rc;
+        pos = dr == 1 ? pos - 1.0 : pos + 1.0;
+        body_extra(h);
+        h->movenum = h->movenum + dr;
+    }
+    return 0;
+}
+
+int lbmib_set_dynamics(lbmib_handle* h, const lbmib_dynamics* d)
+{
+    if (!h || !d) return fail(LBMIB_E_ARG, "null argument");
+    if (h->have_motion) return fail(LBMIB_E_STATE, "set the dynamics before the motion table");
+    if (h->batch) return fail(LBMIB_E_STATE, "the case belongs to a batch");
+    if (d->enabled != 0 && d->enabled != 1) return fail(LBMIB_E_ARG, "enabled must be 0 or 1");
+    if ((d->rk4 != 0 && d->rk4 != 1) || (d->closed_boundary != 0 && d->closed_boundary != 1) || d->reserved != 0)
+        return fail(LBMIB_E_ARG, "rk4 and closed_boundary must be 0 or 1, reserved 0");
+    if (d->enabled && !(d->solid_den > 0.0)) return fail(LBMIB_E_ARG, "solid_den must be positive");
+    if (d->enabled && d->closed_boundary && !(d->area_airfoil > 0.0))
+        return fail(LBMIB_E_ARG, "area_airfoil must be positive for a closed boundary");
+    h->dyn = *d;
+    return 0;
+}
+
+int lbmib_get_body_state(const lbmib_handle* h, double* state, int32_t n)
+{
+    if (!h || !state) return fail(LBMIB_E_ARG, "null argument");
+    const int nw = h->p.n_wings;
+    if (n < 11 * nw + 1) return fail(LBMIB_E_ARG, "room for %d values needed", 11 * nw + 1);
+    int o = 0;
+    for (int a = 0; a < 11; ++a)
+        for (int w = 0; w < nw; ++w) state[o++] = h->rel[a][w];
+    state[o] = (double)h->movenum;
+    return 0;
+}
+
+int lbmib_set_body_state(lbmib_handle* h, const double* state, int32_t n)
+{
+    if (!h || !state) return fail(LBMIB_E_ARG, "null argument");
+    const int nw = h->p.n_wings;
+    if (n != 11 * nw + 1) return fail(LBMIB_E_ARG, "%d values expected", 11 * nw + 1);
+    const double mv = state[11 * nw];
+    if (!(mv == std::floor(mv)) || std::fabs(mv) > 1e9) return fail(LBMIB_E_ARG, "movenum must be an integer");
+    int o = 0;
+    for (int a = 0; a < 11; ++a)
+        for (int w = 0; w < nw; ++w) h->rel[a][w] = state[o++];
+    h->movenum = (int)mv;
+    body_extra(h);
+    return 0;
+}
+
+int lbmib_get_motion_rows(const lbmib_handle* h, double* rows, int32_t max_rows, int32_t* n_rows)
+{
+    if (!h) return fail(LBMIB_E_ARG, "null handle");
+    if (n_rows) *n_rows = h->steps_done;
+    if (!rows || max_rows <= 0) return 0;
+    const int nw = h->p.n_wings, cols = 1 + 3 * nw;
+    const int n = std::min(max_rows, h->steps_done);
+    const double dt = h->p.scale * h->p.U;
+    for (int r = 0; r < n; ++r) {
+        double* row = rows + (size_t)r * cols;
+        if (h->dyn.enabled && (size_t)(r + 1) * cols <= h->mrows.size()) {
+            std::memcpy(row, &h->mrows[(size_t)r * cols], sizeof(double) * cols);
+        } else {
+            row[0] = (double)r * dt;
+            for (int c = 1; c < cols; ++c) row[c] = 0.0;
+        }
+    }
+    return 0;
+}
+
 int lbmib_step(lbmib_handle* h, int32_t n_steps)
 {
     if (!h) return fail(LBMIB_E_ARG, "null handle");
@@ -611,6 +847,11 @@ int lbmib_step(lbmib_handle* h, int32_t n_steps)
     if (n_steps < 0) return fail(LBMIB_E_ARG, "n_steps must be >= 0");
     if (h->steps_done + n_steps > h->max_steps)
         return fail(LBMIB_E_ARG, "step %d beyond the motion table (last step %d)", h->steps_done + n_steps, h->max_steps);
+    if (h->dyn.enabled) {
+        for (int s = 0; s < n_steps; ++s)
+            if (int rc = dynamics_step(h)) return rc;
+        return 0;
+    }
     int rc = engine_run(h->own, n_steps);
     if (rc) return rc;
     return nan_guard_check(h);
@@ -732,6 +973,8 @@ int lbmib_set_step(lbmib_handle* h, int32_t n)
     h->steps_done = n;
     h->geo_fresh = true;
     h->last_flag_step[0] = h->last_flag_step[1] = -1;
+    const size_t mcols = 1 + 3 * (size_t)h->p.n_wings;
+    if (h->mrows.size() > (size_t)n * mcols) h->mrows.resize((size_t)n * mcols);
     // a resumed run starts vor_mom_rate's history afresh, as a new process does
     if (h->vort_hist) {
         HIP_TRY(hipSetDevice(h->dev));
@@ -876,9 +1119,11 @@ int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1)
     for (int w = 0; w < nw; ++w) {
         const double* c = row + 1 + 9 * w;
         const double sn = std::sin(c[0]), ca = std::cos(c[0]), sa = c[3], sb = c[6];
+        // the body offsets of the step (the dynamics' state before its update)
+        const double es = h->body_terms[4 * w], mn = h->body_terms[4 * w + 1];
         for (int i = 0; i < ppw; ++i) {
             const double x = h->xb0[w * ppw + i], y = h->yb0[w * ppw + i];
-            xb1[w * ppw + i] = ca * x + sn * y + sa + h->p.xc;
+            xb1[w * ppw + i] = ca * x + sn * y + sa + h->p.xc + es - mn;
             yb1[w * ppw + i] = -sn * x + ca * y + sb + h->p.yc;
         }
     }

@@ -415,11 +415,16 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
                 HIP_TRY(hipEventRecord(E->ev_first[bi % BR], st));
                 wfirst = E->ev_first[bi % BR];
             }
-            if ((rc = launch_geometry(E, d, off, GEO_NEXT))) return rc;
-            if ((rc = launch_factor(E, d, off, GEO_NEXT, gev))) return rc;
-            HIP_TRY(hipEventRecord(E->ev_next[bi % BR], st));
-            wnext = prev_next;
-            prev_next = E->ev_next[bi % BR];
+            if (!E->per_step) {
+                if ((rc = launch_geometry(E, d, off, GEO_NEXT))) return rc;
+                if ((rc = launch_factor(E, d, off, GEO_NEXT, gev))) return rc;
+                HIP_TRY(hipEventRecord(E->ev_next[bi % BR], st));
+                wnext = prev_next;
+                prev_next = E->ev_next[bi % BR];
+            } else if (gev) {
+                HIP_TRY(hipEventRecord(gev[1], st));
+                HIP_TRY(hipEventRecord(gev[2], st));
+            }
         }
         hipEvent_t* ev = nullptr;
         if (every > 0 && off % every == 0) {
@@ -787,6 +792,7 @@ int lbmib_batch_add(lbmib_batch* b, lbmib_handle* h)
     if (!b || !h) return fail(LBMIB_E_ARG, "null argument");
     if (h->batch) return fail(LBMIB_E_STATE, "the case already belongs to a batch");
     if (!h->have_motion) return fail(LBMIB_E_STATE, "set the case's motion table before adding it to a batch");
+    if (h->dyn.enabled) return fail(LBMIB_E_STATE, "a case with body dynamics takes its steps alone");
     if (h->dev != b->dev) return fail(LBMIB_E_ARG, "the case lives on device %d, the batch on %d", h->dev, b->dev);
     Engine* E = b->E;
     if (!E->members.empty()) {

@@ -43,6 +43,16 @@ struct lbmib_handle {
     int steps_done = 0;
     bool geo_fresh = true;   // the next run builds the geometry of its first step's batch first
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
+    // rigid-body dynamics: parameters, state (wing-indexed, as the
+    // reference's body_rel_* / body_extra_* arrays), domain shift count,
+    // airfoil_motion rows of the steps taken
+    lbmib_dynamics dyn{};
+    double rel[11][8] = {};           // a, a_prev, v, v_prev, s, s_prev, pos, theta_prev, theta, tt_prev, tt
+    double ext[3][8] = {};            // body_extra_v, _s, _pos
+    double mom = 0.0;                 // raw moment of wing 1 in the last step
+    int movenum = 0;
+    double body_terms[32] = {};       // the offset terms the last step's kernels used
+    std::vector<double> mrows;
     double* vort = nullptr;           // post-processing: vorticity plane,
     double* vort_hist = nullptr;      //   vor_mom_rate's vor, vor1 and dvordt planes,
     double* surf = nullptr;           //   surface pressure per marker
@@ -78,6 +88,7 @@ struct Engine {
     int dev = 0;
     int ncu = 0;
     int B = LBMIB_GEOMETRY_BATCH;    // steps per geometry batch
+    bool per_step = false;           // geometry of each step only, built after the step before (body dynamics)
     Shape shape;
     static constexpr int NSTREAM = 4;
     hipStream_t s[NSTREAM] = {};

@@ -118,6 +118,9 @@ struct DevCase {
     double* rhs;            // [2][nseg] U_B - D u*, zig-zag order
     double* y;              // [2][nseg] (D D^T)^-1 rhs, segment order
     double* hist;           // [steps_cap][3*nw] C_x, C_y, C_M per wing
+    double* raw;            // [steps_cap][nw][3] raw force x, y and moment per wing (dynamics only), or null
+    const double* body;     // [nw][4] body offset terms of the next step: body_extra_s/(1/scale),
+                            // movenum*scale, body_extra_v, body_extra_pos (zero with the dynamics off)
     int* err;               // [1] ErrBits
     double* red;            // reduction scratch
 };
@@ -198,6 +201,12 @@ __global__ void k_clock_advance(CaseClock* __restrict__, int ncases, int steps);
 size_t factor_smem_bytes();
 size_t solve_smem_bytes(int npad);
 __global__ void k_residual(const double*, const double*, int, int, long long, double*);
+struct RingW { double w[9]; };  // the weights, by value
+template <typename T>
+__global__ void k_move_pops(const T*, T*, int, long long, int);
+template <typename T>
+__global__ void k_move_ring(const T*, double*, int, int, long long, int, RingW);
+__global__ void k_move_macro(double*, double*, int, long long, int, int);
 __global__ void k_vorticity(const double*, int, int, long long, double, double, double*, double*, double*, double*);
 __global__ void k_surface_pressure(const int*, const int*, const double*, const double*, int, double*);
 template <typename T> __global__ void k_nan_check(const T*, long long, int*);

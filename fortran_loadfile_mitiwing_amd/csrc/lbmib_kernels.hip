@@ -418,8 +418,11 @@ __global__ __launch_bounds__(256) void k_geo_markers(const DevCase* __restrict__
     const double* kn = D.kin + ((long long)step * D.nw + w) * KIN;
     double sn = kn[0], ca = kn[1], om = kn[2], sa = kn[3], sau = kn[4], sb = kn[5], sbv = kn[6];
     double xa = D.xb0[p], ya = D.yb0[p], xb = D.xb0[p + 1], yb = D.yb0[p + 1];
-    double x1a = ca * xa + sn * ya + sa + D.xc;
-    double x1b = ca * xb + sn * yb + sa + D.xc;
+    // body offsets of the dynamics (zero without): body_extra_s/(1/scale) and
+    // the domain shift movenum*mesh_max_scale (airfoil.f90:261-262)
+    const double es = D.body[4 * w], mn = D.body[4 * w + 1];
+    double x1a = ca * xa + sn * ya + sa + D.xc + es - mn;
+    double x1b = ca * xb + sn * yb + sa + D.xc + es - mn;
     double y1a = -sn * xa + ca * ya + sb + D.yc;
     double y1b = -sn * xb + ca * yb + sb + D.yc;
     double X = 0.5 * (x1b + x1a);
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(256) void k_geo_markers(const DevCase* __restrict__
     S.yb2[s] = Y;
     double xs = xb + xa, ys = yb + ya;
     double U = D.U;
-    S.ub[s] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + sau);
+    S.ub[s] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + sau) + D.body[4 * w + 2];
     S.vb[s] = U * (-om * ca * 0.5 * xs - om * sn * 0.5 * ys + sbv);
     int i1, i2, j1, j2;
     axis_box(X, D.ox, D.scale, D.nx, i1, i2);
@@ -1527,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     const Slot S = slot_of(D, step % D.nslot);
     __shared__ double terms[2][FCH][4];
     __shared__ int wing[2][FCH];
-    __shared__ double sab[8][2];  // body offsets sa, sb of each wing at this step
+    __shared__ double sab[8][3];  // body offsets sa, sb and body_extra_pos of each wing at this step
     const int tid = threadIdx.x;
     const int na = *S.nadj;
     const double sc = D.scale;
@@ -1535,6 +1538,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
         const double* kn = D.kin + ((long long)step * D.nw + tid) * KIN;
         sab[tid][0] = kn[3];
         sab[tid][1] = kn[5];
+        sab[tid][2] = D.body[4 * tid + 3];
     }
     __syncthreads();
     // terms of chunk ch into buffer ch & 1, by threads t0 .. t0 + nt - 1
@@ -1561,7 +1565,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
                 const int w = aw[u];
                 const double t1 = 2.0 * rho[u] * gx[u];
                 const double t2 = 2.0 * rho[u] * gy[u];
-                const double x0 = (D.ox + sc * (double)ax[u] - (D.xc + sab[w][0])) / sc;
+                const double x0 = (D.ox + sc * (double)ax[u] - (D.xc + sab[w][0])) / sc - sab[w][2];
                 const double y0 = (D.oy + sc * (double)ay[u] - (D.yc + sab[w][1])) / sc;
                 terms[b][q][0] = t1;
                 terms[b][q][1] = t2;
@@ -1575,7 +1579,9 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     int curw = 0;
     double* hist = D.hist + (long long)step * 3 * D.nw;
     const double U = D.U;
+    double* raw = D.raw ? D.raw + (long long)step * 3 * D.nw : nullptr;
     auto flush = [&](int w) {
+        if (raw) { raw[3 * w] = F0; raw[3 * w + 1] = F1; raw[3 * w + 2] = M; }
         hist[3 * w + 0] = 2.0 * F0 / (U * U * (1.0 / sc));
         hist[3 * w + 1] = 2.0 * F1 / (U * U * (1.0 / sc));
         hist[3 * w + 2] = -2.0 * M / (U * U * (1.0 / (sc * sc)));
@@ -1728,6 +1734,55 @@ __global__ __launch_bounds__(64) void k_surface_pressure(const int* __restrict__
     for (int p = 0; p < n; ++p)
         bb = bb + dval[k * BOXMAX + p] * (rhoadj[didx[k * BOXMAX + p]] / 3.0);
     out[k] = bb;
+}
+
+// move_mesh (move_mesh.f90:16-58 for dr = 1, 172-214 for dr = -1): the
+// populations of every node take those of the node dr columns further, the
+// column without a source keeps its own.  Out of place; the caller copies
+// back.
+template <typename T>
+__global__ __launch_bounds__(256) void k_move_pops(const T* __restrict__ src, T* __restrict__ dst, int nx, long long N,
+                                                   int dr)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 9 * N) return;
+    const int i = (int)((t % N) % nx);
+    const bool keep = dr == 1 ? i == nx - 1 : i == 0;
+    dst[t] = src[keep ? t : t + dr];
+}
+template __global__ void k_move_pops<double>(const double*, double*, int, long long, int);
+template __global__ void k_move_pops<float>(const float*, float*, int, long long, int);
+
+// The stale f0 of the edge column that the shift refills (f0(1,:) from
+// f(2,:) for dr = 1, f0(Nx,:) from f(Nx-1,:) for dr = -1), as full FP64
+// populations; the other edge column keeps its stale f0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_move_ring(const T* __restrict__ src, double* __restrict__ f0col, int nx,
+                                                   int ny, long long N, int dr, RingW w)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 9 * ny) return;
+    const int k = t / ny, j = t - k * ny;
+    const int i = dr == 1 ? 1 : nx - 2;
+    f0col[(long long)k * ny + j] = lift(src[(long long)k * N + (long long)j * nx + i], w.w[k]);
+}
+template __global__ void k_move_ring<double>(const double*, double*, int, int, long long, int, RingW);
+template __global__ void k_move_ring<float>(const float*, double*, int, int, long long, int, RingW);
+
+// The recorded rho/u/v of the step (cur) and of the step before (prev): in
+// pass 0 prev takes cur shifted (u0/v0 of move_mesh), in pass 1 cur takes
+// prev back; the column without a source keeps its values in both.
+__global__ __launch_bounds__(256) void k_move_macro(double* __restrict__ cur, double* __restrict__ prev, int nx,
+                                                    long long N, int dr, int pass)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 3 * N) return;
+    const int i = (int)((t % N) % nx);
+    if (dr == 1 ? i == nx - 1 : i == 0) return;
+    if (pass == 0)
+        prev[t] = cur[t + dr];
+    else
+        cur[t] = prev[t];
 }
 
 // NaN guard over the populations.

@@ -39,6 +39,15 @@ module lbmib
         integer(c_int32_t) :: reserved(5) = 0
     end type
 
+    ! struct lbmib_dynamics (include/lbmib.h): rigid-body dynamics of wing 1,
+    ! defaults as the reference's parameters (main.f90:76, 101-129)
+    type, bind(C) :: lbmib_dynamics
+        integer(c_int32_t) :: enabled = 0, rk4 = 1, closed_boundary = 1, reserved = 0
+        real(c_double) :: solid_den = 10.0d0, area_airfoil = 6.807946476488794d-2
+        real(c_double) :: par1 = 0.0d0, par2 = 0.0d0
+        real(c_double) :: alphad = 10.0d0 * 3.141592653589793d0 / 180.0d0
+    end type
+
     ! struct lbmib_launch_shape (include/lbmib.h): how steps are launched
     type, bind(C), public :: lbmib_launch_shape
         integer(c_int32_t) :: nodes_per_thread = 1, tile_order = 0, persistent = 0
@@ -51,6 +60,9 @@ module lbmib
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
     public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings, lbmib_set_step
     public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure
+    ! rigid-body dynamics
+    public :: lbmib_dynamics, lbmib_set_dynamics, lbmib_get_body_state, lbmib_set_body_state
+    public :: lbmib_get_motion_rows
     public :: lbmib_destroy, lbmib_error_message, lbmib_version_string, lbmib_check
     ! batched form (many cases advanced together) and footprints
     public :: lbmib_set_shape, lbmib_get_shape, lbmib_device_bytes, lbmib_mem_info
@@ -119,6 +131,35 @@ module lbmib
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: h
             real(c_double), intent(out) :: xb1(*), yb1(*)
+        end function
+
+        integer(c_int) function lbmib_set_dynamics(h, d) bind(C, name="lbmib_set_dynamics")
+            import :: c_int, c_ptr, lbmib_dynamics
+            type(c_ptr), value :: h
+            type(lbmib_dynamics), intent(in) :: d
+        end function
+
+        integer(c_int) function lbmib_get_body_state(h, state, n) bind(C, name="lbmib_get_body_state")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: state(*)
+            integer(c_int32_t), value :: n
+        end function
+
+        integer(c_int) function lbmib_set_body_state(h, state, n) bind(C, name="lbmib_set_body_state")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: state(*)
+            integer(c_int32_t), value :: n
+        end function
+
+        ! rows(1 + 3*wing_count, max_rows)
+        integer(c_int) function lbmib_get_motion_rows(h, rows, max_rows, n_rows) bind(C, name="lbmib_get_motion_rows")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: rows(*)
+            integer(c_int32_t), value :: max_rows
+            integer(c_int32_t), intent(out) :: n_rows
         end function
 
         ! vor, dvordt: c_loc of nx*ny arrays or c_null_ptr

@@ -60,6 +60,12 @@ class Params(C.Structure):
                 ("nan_guard", C.c_int32), ("ib_solver", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
+class DynamicsParams(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("rk4", C.c_int32), ("closed_boundary", C.c_int32),
+                ("reserved", C.c_int32), ("solid_den", C.c_double), ("area_airfoil", C.c_double),
+                ("par1", C.c_double), ("par2", C.c_double), ("alphad", C.c_double)]
+
+
 class LaunchShape(C.Structure):
     _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
                 ("ib_cus", C.c_int32), ("graph", C.c_int32), ("ib_collide_shared", C.c_int32),
@@ -85,7 +91,8 @@ _lib = None
 
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
-           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure", "lbmib_set_dynamics", "lbmib_get_body_state",
+           "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
@@ -113,6 +120,10 @@ def load(path: str = LIB_PATH):
     L.lbmib_upload_state.argtypes = [vp, _dp]
     L.lbmib_set_step.argtypes = [vp, C.c_int32]
     L.lbmib_markers.argtypes = [vp, _dp, _dp]
+    L.lbmib_set_dynamics.argtypes = [vp, C.POINTER(DynamicsParams)]
+    L.lbmib_get_body_state.argtypes = [vp, _dp, C.c_int32]
+    L.lbmib_set_body_state.argtypes = [vp, _dp, C.c_int32]
+    L.lbmib_get_motion_rows.argtypes = [vp, _dp, C.c_int32, C.POINTER(C.c_int32)]
     L.lbmib_vorticity.argtypes = [vp, C.c_int32, _dp, _dp]
     L.lbmib_surface_pressure.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_residual.argtypes = [vp, _dp]
@@ -169,7 +180,7 @@ class LBMIB:
     """One solver case resident on one GPU."""
 
     def __init__(self, case, precision: int = PREC_FP64, device: int = 0, nan_guard: int = 0,
-                 ib_solver: int = SOLVER_AUTO):
+                 ib_solver: int = SOLVER_AUTO, dynamics=None):
         L = load()
         self.case = case
         self.p = case.params
@@ -179,6 +190,13 @@ class LBMIB:
         h = C.c_void_p()
         _check(L.lbmib_create(C.byref(prm), _d(xb), _d(yb), C.byref(h)))
         self.h = h
+        if dynamics is not None:  # a cases.Dynamics
+            d = DynamicsParams()
+            d.enabled, d.rk4 = int(dynamics.enabled), int(dynamics.rk4)
+            d.closed_boundary, d.reserved = int(dynamics.closed_boundary), 0
+            d.solid_den, d.area_airfoil = dynamics.solid_den, dynamics.area_airfoil
+            d.par1, d.par2, d.alphad = dynamics.par1, dynamics.par2, dynamics.alphad
+            _check(L.lbmib_set_dynamics(self.h, C.byref(d)))
         mt = np.ascontiguousarray(case.motion, dtype=np.float64)
         _check(L.lbmib_set_motion_table(self.h, _d(mt), mt.shape[0], mt.shape[1]))
         self.N = self.p.nx * self.p.ny
@@ -265,6 +283,26 @@ class LBMIB:
         x = np.zeros(m); y = np.zeros(m)
         _check(load().lbmib_markers(self.h, _d(x), _d(y)))
         return x, y
+
+    def body_state(self) -> np.ndarray:
+        """11 per-wing arrays in the restart file's order, then movenum."""
+        out = np.zeros(11 * self.p.n_wings + 1)
+        _check(load().lbmib_get_body_state(self.h, _d(out), out.size))
+        return out
+
+    def set_body_state(self, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        _check(load().lbmib_set_body_state(self.h, _d(st), st.size))
+
+    def motion_rows(self) -> np.ndarray:
+        """airfoil_motion.dat rows of the steps taken."""
+        n = C.c_int32()
+        L = load()
+        _check(L.lbmib_get_motion_rows(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 1 + 3 * self.p.n_wings))
+        if n.value:
+            _check(L.lbmib_get_motion_rows(self.h, _d(out), n.value, C.byref(n)))
+        return out
 
     def vorticity(self, rate: bool = False, copy: bool = True):
         """output's vorticity (ny, nx) of the last step; with ``rate`` also one

@@ -155,6 +155,41 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
  * output (solution.f90:219-233). */
 int lbmib_markers(const lbmib_handle* h, double* xb1, double* yb1);
 
+/* Rigid-body dynamics of wing 1, driven by the IB force (replaces
+ * solve_dynamic_equations and its parameters, main.f90:76,101-129; the
+ * dynamics block of immersed_boundary_Wu, boundary_condition.f90:1676-1731;
+ * rk4, 3168-3202; update_body_extra_from_rel, mode.f90:285-307), and the
+ * domain shift that follows the body one node at a time (main.f90:370-388,
+ * move_mesh.f90).  The body offsets enter the markers (airfoil.f90:261-262),
+ * their velocities (boundary_condition.f90:1546-1548) and the moment arm
+ * (1672).  Set before lbmib_set_motion_table.  A case with the dynamics on
+ * takes its steps one at a time (each step's geometry follows the force of
+ * the step before) and cannot join a batch. */
+typedef struct lbmib_dynamics {
+    int32_t enabled;          /* solve_dynamic_equations */
+    int32_t rk4;              /* if_rk4: 1 the four-stage update, 0 the explicit one */
+    int32_t closed_boundary;  /* mass solid_den*area_airfoil/scale^2 (1) or solid_den/scale^2 (0) */
+    int32_t reserved;         /* 0 */
+    double solid_den;         /* > 0 */
+    double area_airfoil;
+    double par1, par2;        /* pitch equation coefficients */
+    double alphad;            /* radians */
+} lbmib_dynamics;
+int lbmib_set_dynamics(lbmib_handle* h, const lbmib_dynamics* d);
+
+/* The body state as the restart file holds it (saveflowfield,
+ * solution.f90:1650-1653): body_rel_a, _a_prev, _v, _v_prev, _s, _s_prev,
+ * _pos, _theta_prev, _theta, _temp_theta_prev, _temp_theta (n_wings values
+ * each), then movenum: 11*n_wings + 1 values.  Set it after lbmib_set_step
+ * to resume a run (readflowfield, 1750-1760). */
+int lbmib_get_body_state(const lbmib_handle* h, double* state, int32_t n);
+int lbmib_set_body_state(lbmib_handle* h, const double* state, int32_t n);
+
+/* airfoil_motion.dat rows (solution.f90:378-388), one per step taken:
+ * time, then per wing 0, body_extra_v/UU, body_extra_s*scale (1 + 3*n_wings
+ * columns; zero offsets with the dynamics off). */
+int lbmib_get_motion_rows(const lbmib_handle* h, double* rows, int32_t max_rows, int32_t* n_rows);
+
 /* Replaces the vorticity of output (solution.f90:181-210) and vor_mom_rate
  * (solution.f90:1841-1900, main.f90:391), computed on the device from the
  * last step's velocity field.  vor (nx*ny) gets dv/dx - du/dy in units of

@@ -359,8 +359,17 @@ struct orc_case {
     double angle[MAXW], omega[MAXW], omgac[MAXW], sa[MAXW], sau[MAXW], saac[MAXW], sb[MAXW],
         sbv[MAXW], sbac[MAXW];
     double wing_force[MAXW][2], wing_moment[MAXW];
-    /* history */
+    /* rigid-body dynamics (main.f90:60-66; index 0 is wing 1) and the
+     * domain shift count movenum (main.f90:121) */
+    orc_dynamics dyn;
+    double rel_a[MAXW], rel_a_prev[MAXW], rel_v[MAXW], rel_v_prev[MAXW], rel_s[MAXW], rel_s_prev[MAXW],
+        rel_pos[MAXW], rel_theta[MAXW], rel_theta_prev[MAXW], rel_tt[MAXW], rel_tt_prev[MAXW];
+    double extra_v[MAXW], extra_s[MAXW], extra_pos[MAXW];
+    double mom; /* com_date's mom: the raw moment of wing 1 in the last IB call */
+    int movenum;
+    /* history: force rows and airfoil_motion rows */
     double* hist;
+    double* mrows;
     int hist_cols, hist_rows, hist_cap;
     int n; /* steps done */
     double err;
@@ -385,7 +394,7 @@ void orc_destroy(orc_case* c)
     free(c->arc_l); free(c->ub); free(c->vb);
     free(c->adj_i); free(c->adj_j); free(c->adj_w); free(c->box);
     free(c->glx); free(c->gly); free(c->rho_adj);
-    free(c->step_table); free(c->hist);
+    free(c->step_table); free(c->hist); free(c->mrows);
     free(c);
 }
 
@@ -486,6 +495,7 @@ orc_case* orc_create(const orc_params* p, const double* xb0, const double* yb0,
     c->hist_cols = 4 + 12 * nw;
     c->hist_cap = 1024;
     c->hist = (double*)malloc(sizeof(double) * c->hist_cap * c->hist_cols);
+    c->mrows = (double*)malloc(sizeof(double) * c->hist_cap * (1 + 3 * c->p.n_wings));
     c->n = 0;
     c->err = 0.0;
     return c;
@@ -596,8 +606,10 @@ static int move_boundary(orc_case* c)
         int ps = wg * ppw, ss = wg * (ppw - 1);
         double ca = cos(c->angle[wg]), sn = sin(c->angle[wg]);
         for (int i0 = ps; i0 < ps + ppw; ++i0) {
-            /* body_extra_s and movenum stay zero with dynamics off (main.f90:103,319) */
-            c->xb1[i0] = ca * c->xb0[i0] + sn * c->yb0[i0] + c->sa[wg] + c->p.xc;
+            /* body_extra_s and movenum are zero with the dynamics off (main.f90:103,319);
+             * the shift per domain move is mesh_max_scale, the lattice spacing (ini_mesh.f90:21) */
+            c->xb1[i0] = ca * c->xb0[i0] + sn * c->yb0[i0] + c->sa[wg] + c->p.xc +
+                         c->extra_s[wg] / (1.0 / c->p.scale) - (double)c->movenum * c->p.scale;
             c->yb1[i0] = -sn * c->xb0[i0] + ca * c->yb0[i0] + c->sb[wg] + c->p.yc;
         }
         for (int i0 = ss; i0 < ss + ppw - 1; ++i0) {
@@ -867,6 +879,137 @@ int orc_solve(const double* G, const double* S, const double* b, int n,
     return rc;
 }
 
+/* ---------------- rigid-body dynamics ---------------- */
+
+/* update_body_extra_from_rel (mode.f90:285-307): running sums over wings */
+static void update_body_extra(orc_case* c)
+{
+    double av = 0.0, as = 0.0, ap = 0.0;
+    for (int w = 0; w < MAXW; ++w) { c->extra_v[w] = 0.0; c->extra_s[w] = 0.0; c->extra_pos[w] = 0.0; }
+    for (int w = 0; w < c->p.n_wings; ++w) {
+        av = av + c->rel_v[w];
+        as = as + c->rel_s[w];
+        ap = ap + c->rel_pos[w];
+        c->extra_v[w] = av;
+        c->extra_s[w] = as;
+        c->extra_pos[w] = ap;
+    }
+}
+
+/* rk4 (boundary_condition.f90:3168-3202): four stages with the acceleration
+ * held over the step; the pitch equation reads alphad and the last moment */
+static void rk4(const orc_case* c, double s1, double* s, double* s2, double v1, double* v, double a1, double jd1,
+                double* jd, double jsd1, double* jsd, double p1, double p2)
+{
+    const double alphad = c->dyn.alphad, mom = c->mom;
+    double k11 = v1, k21 = a1;
+    double k12 = v1 + 0.5 * k21, k22 = a1;
+    double k13 = v1 + 0.5 * k22, k23 = a1;
+    double k14 = v1 + k23, k24 = a1;
+    *s = s1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    *v = v1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+    *s2 = *s2 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    k11 = jsd1;
+    k21 = p1 * (jd1 + alphad) + p2 * mom;
+    k12 = jsd1 + 0.5 * k21;
+    k22 = p1 * (jd1 + alphad + 0.5 * k11) + p2 * mom;
+    k13 = jsd1 + 0.5 * k22;
+    k23 = p1 * (jd1 + alphad + 0.5 * k12) + p2 * mom;
+    k14 = jsd1 + k23;
+    k24 = p1 * (jd1 + alphad + k13) + p2 * mom;
+    *jd = jd1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    *jsd = jsd1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+}
+
+/* the dynamics block of immersed_boundary_Wu (boundary_condition.f90:1676-1719,
+ * 1731): wing 1 accelerates by the total raw x force over its mass */
+static void body_update(orc_case* c)
+{
+    const double sc = c->p.scale, UU = c->p.U;
+    const orc_dynamics* d = &c->dyn;
+    double force1 = 0.0;
+    for (int w = 0; w < c->p.n_wings; ++w) force1 = force1 + c->wing_force[w][0];
+    c->rel_a_prev[0] = c->rel_a[0];
+    if (!d->closed_boundary)
+        c->rel_a[0] = force1 / (d->solid_den / (sc * sc));
+    else
+        c->rel_a[0] = force1 / (d->solid_den * d->area_airfoil / (sc * sc));
+    c->rel_v_prev[0] = c->rel_v[0];
+    c->rel_s_prev[0] = c->rel_s[0];
+    c->rel_theta_prev[0] = c->rel_theta[0];
+    c->rel_tt_prev[0] = c->rel_tt[0];
+    if (!d->rk4) {
+        c->rel_v[0] = c->rel_a[0] + c->rel_v[0];
+        c->rel_s[0] = c->rel_s[0] + c->rel_v_prev[0];
+        c->rel_pos[0] = c->rel_pos[0] + c->rel_v_prev[0];
+        c->rel_theta[0] = c->rel_theta_prev[0] + c->rel_tt_prev[0];
+        c->rel_tt[0] = c->rel_tt_prev[0] + d->par1 * UU * UU * sc * sc * (c->rel_theta[0] + d->alphad) +
+                       d->par2 * (sc * sc * sc * sc) * c->mom;
+    } else {
+        rk4(c, c->rel_s_prev[0], &c->rel_s[0], &c->rel_pos[0], c->rel_v_prev[0], &c->rel_v[0], c->rel_a[0],
+            c->rel_theta_prev[0], &c->rel_theta[0], c->rel_tt_prev[0], &c->rel_tt[0], d->par1 * UU * UU * sc * sc,
+            d->par2 * (sc * sc * sc * sc));
+    }
+    update_body_extra(c);
+}
+
+/* move_mesh (move_mesh.f90:1-118 for dr = 1, 186-214 for dr = -1; one mesh,
+ * one process, no periodic edges): every array shifts by one column towards
+ * -dr; the column that has no source keeps its values.  u0/v0 take the
+ * shifted velocity and pre the shifted density (not rho/3), as there. */
+static void move_mesh(orc_case* c, int dr)
+{
+    const int nx = c->nx, ny = c->ny;
+    const size_t N = c->N;
+    const int k0 = dr == 1 ? 0 : 1, k1 = dr == 1 ? nx - 2 : nx - 1;
+    for (int j = 0; j < ny; ++j)
+        for (int k = k0; k <= k1; ++k) {
+            const size_t d = (size_t)j * nx + k, s = d + dr;
+            for (int q = 0; q < Q; ++q) c->f0[q * N + d] = c->f[q * N + s];
+            c->u0[d] = c->u[s];
+            c->v0[d] = c->v[s];
+            c->pre[d] = c->rho[s];
+        }
+    for (int j = 0; j < ny; ++j)
+        for (int k = k0; k <= k1; ++k) {
+            const size_t d = (size_t)j * nx + k;
+            for (int q = 0; q < Q; ++q) c->f[q * N + d] = c->f0[q * N + d];
+            c->u[d] = c->u0[d];
+            c->v[d] = c->v0[d];
+            c->rho[d] = c->pre[d];
+        }
+}
+
+void orc_move_mesh(orc_case* c, int dr) { move_mesh(c, dr >= 0 ? 1 : -1); }
+
+int orc_set_dynamics(orc_case* c, const orc_dynamics* d)
+{
+    if (c->n != 0 || !d) return -1;
+    if (d->enabled && !(d->solid_den > 0.0)) return -1;
+    c->dyn = *d;
+    return 0;
+}
+
+int orc_body_state(const orc_case* c, double* out)
+{
+    const int nw = c->p.n_wings;
+    const double* arr[11] = {c->rel_a, c->rel_a_prev, c->rel_v, c->rel_v_prev, c->rel_s, c->rel_s_prev,
+                             c->rel_pos, c->rel_theta_prev, c->rel_theta, c->rel_tt_prev, c->rel_tt};
+    int o = 0;
+    for (int a = 0; a < 11; ++a)
+        for (int w = 0; w < nw; ++w) out[o++] = arr[a][w];
+    out[o++] = (double)c->movenum;
+    return o;
+}
+
+int orc_motion_rows(const orc_case* c, double* rows, int max_rows)
+{
+    const int cols = 1 + 3 * c->p.n_wings;
+    const int n = c->hist_rows < max_rows ? c->hist_rows : max_rows;
+    if (rows) memcpy(rows, c->mrows, sizeof(double) * (size_t)n * cols);
+    return c->hist_rows;
+}
+
 /* per-marker sparse row of D = xx1: box nodes sorted by IB list index */
 typedef struct drow {
     int cnt;
@@ -902,7 +1045,7 @@ static int immersed_boundary(orc_case* c)
         for (int i = ss; i < ss + spw; ++i) {
             double xs = c->xb0[ps + i - ss + 1] + c->xb0[ps + i - ss];
             double ys = c->yb0[ps + i - ss + 1] + c->yb0[ps + i - ss];
-            c->ub[i] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + c->sau[wg]);
+            c->ub[i] = U * (-om * sn * 0.5 * xs + om * ca * 0.5 * ys + c->sau[wg]) + c->extra_v[wg];
             c->vb[i] = U * (-om * ca * 0.5 * xs - om * sn * 0.5 * ys + c->sbv[wg]);
         }
     }
@@ -1098,10 +1241,12 @@ static int immersed_boundary(orc_case* c)
             double t2 = 2.0 * c->rho_adj[k] * c->gly[k];
             c->wing_force[wg][0] = c->wing_force[wg][0] - t1;
             c->wing_force[wg][1] = c->wing_force[wg][1] - t2;
-            double x0 = (ox + sc * (double)(c->adj_i[k]) - (c->p.xc + c->sa[wg])) / sc;
+            double x0 = (ox + sc * (double)(c->adj_i[k]) - (c->p.xc + c->sa[wg])) / sc - c->extra_pos[wg];
             double y0 = (oy + sc * (double)(c->adj_j[k]) - (c->p.yc + c->sb[wg])) / sc;
             c->wing_moment[wg] = c->wing_moment[wg] - x0 * t2 + y0 * t1;
         }
+        c->mom = c->wing_moment[0];
+        if (c->dyn.enabled) body_update(c);
     }
     free(ustar); free(vstar); free(bu); free(bv); free(rhs); free(G); free(y);
     free(xx1); free(rows);
@@ -1257,6 +1402,18 @@ static int evolution(orc_case* c, int cycle1)
     if (c->hist_rows >= c->hist_cap) {
         c->hist_cap *= 2;
         c->hist = (double*)realloc(c->hist, sizeof(double) * c->hist_cap * c->hist_cols);
+        c->mrows = (double*)realloc(c->mrows, sizeof(double) * c->hist_cap * (1 + 3 * nw));
+    }
+    /* airfoil_motion row (solution.f90:378-388): after the IB's dynamics update */
+    {
+        double* m = c->mrows + (size_t)c->hist_rows * (1 + 3 * nw);
+        int mc = 0;
+        m[mc++] = time;
+        for (int wg = 0; wg < nw; ++wg) {
+            m[mc++] = 0.0;
+            m[mc++] = c->extra_v[wg] / U;
+            m[mc++] = c->extra_s[wg] / (1.0 / sc);
+        }
     }
     double* row = c->hist + (size_t)c->hist_rows * c->hist_cols;
     int col = 0;
@@ -1280,6 +1437,18 @@ int orc_step(orc_case* c, int nsteps)
         if (n >= 2) erranalysis(c);
         int rc = evolution(c, n);
         if (rc < 0) return rc;
+        /* the domain follows the body one node at a time (main.f90:370-388) */
+        if (c->rel_pos[0] >= 1.0) {
+            move_mesh(c, 1);
+            c->rel_pos[0] = c->rel_pos[0] - 1.0;
+            update_body_extra(c);
+            c->movenum = c->movenum + 1;
+        } else if (c->rel_pos[0] <= -1.0) {
+            move_mesh(c, -1);
+            c->rel_pos[0] = c->rel_pos[0] + 1.0;
+            update_body_extra(c);
+            c->movenum = c->movenum - 1;
+        }
         c->n = n;
     }
     return 0;

@@ -60,6 +60,20 @@ typedef struct orc_params {
 
 typedef struct orc_case orc_case;
 
+/* Rigid-body dynamics of wing 1 driven by the IB force (solve_dynamic_equations,
+ * main.f90:103; boundary_condition.f90:1694-1719, 3168-3202) and the domain
+ * shift that follows the body (main.f90:370-388, move_mesh.f90). */
+typedef struct orc_dynamics {
+    int enabled;           /* solve_dynamic_equations */
+    int rk4;               /* if_rk4 (boundary_condition.f90:1527-1529): 1 rk4, 0 explicit update */
+    int closed_boundary;   /* main.f90:101: mass solid_den*area_airfoil/scale^2 (1) or solid_den/scale^2 (0) */
+    int reserved;
+    double solid_den;      /* main.f90:123 */
+    double area_airfoil;   /* main.f90:125 */
+    double par1, par2;     /* main.f90:128-129 */
+    double alphad;         /* main.f90:76, radians */
+} orc_dynamics;
+
 /* init (solution.f90:1-72) + read_boundary's post-processing (airfoil.f90:235-241)
  * + build_motion_step_table (mode.f90:245-269).
  * xb0/yb0: n_wings*pts_per_wing polygon points in chord units, body frame, as
@@ -105,6 +119,17 @@ void orc_markers(const orc_case* c, double* xb2, double* yb2, double* ub, double
 /* Rebuild the IB node set from given midpoints (test hook on move_boundary). */
 int orc_build_ib_set(const orc_case* c, const double* xb2, const double* yb2,
                      int* pi, int* pj, int* wing, int max_nodes);
+
+/* Dynamics: set before the first step; state per wing in the restart file's
+ * order (solution.f90:1650-1652): body_rel_a, _a_prev, _v, _v_prev, _s,
+ * _s_prev, _pos, _theta_prev, _theta, _temp_theta_prev, _temp_theta (each
+ * n_wings values), then movenum; motion rows of airfoil_motion.dat
+ * (solution.f90:378-388), 1 + 3*n_wings per step. */
+int orc_set_dynamics(orc_case* c, const orc_dynamics* d);
+int orc_body_state(const orc_case* c, double* out);
+int orc_motion_rows(const orc_case* c, double* rows, int max_rows);
+/* Test hook: one move_mesh of the current state (dr = 1 or -1). */
+void orc_move_mesh(orc_case* c, int dr);
 
 /* Post-processing: output's vorticity column (solution.f90:181-210), one
  * vor_mom_rate call (solution.f90:1841-1900) and cal_sur_cp

@@ -28,6 +28,12 @@ class OrcParams(C.Structure):
                 ("n_threads", C.c_int)]
 
 
+class OrcDynamics(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("rk4", C.c_int), ("closed_boundary", C.c_int), ("reserved", C.c_int),
+                ("solid_den", C.c_double), ("area_airfoil", C.c_double), ("par1", C.c_double),
+                ("par2", C.c_double), ("alphad", C.c_double)]
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
@@ -68,6 +74,10 @@ def lib():
         L.orc_motion_step_table.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_solve.argtypes = [_dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp]
         L.orc_collide_node.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
+        L.orc_set_dynamics.argtypes = [C.c_void_p, C.POINTER(OrcDynamics)]
+        L.orc_body_state.argtypes = [C.c_void_p, _dp]
+        L.orc_motion_rows.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.orc_move_mesh.argtypes = [C.c_void_p, C.c_int]
         L.orc_vorticity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_vorticity_rate.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_surface_pressure.argtypes = [C.c_void_p, _dp, _dp]
@@ -252,6 +262,32 @@ class Oracle:
         a = [np.zeros(n) for _ in range(5)]
         lib().orc_markers(self.h, *[_d(x) for x in a])
         return dict(xb2=a[0], yb2=a[1], ub=a[2], vb=a[3], arc_l=a[4])
+
+    def set_dynamics(self, dyn):
+        """Switch on the body dynamics (a cases.Dynamics) before the first step."""
+        d = OrcDynamics()
+        d.enabled, d.rk4, d.closed_boundary = int(dyn.enabled), int(dyn.rk4), int(dyn.closed_boundary)
+        d.solid_den, d.area_airfoil = dyn.solid_den, dyn.area_airfoil
+        d.par1, d.par2, d.alphad = dyn.par1, dyn.par2, dyn.alphad
+        if lib().orc_set_dynamics(self.h, C.byref(d)) != 0:
+            raise ValueError("set_dynamics refused")
+
+    def body_state(self) -> np.ndarray:
+        """11 per-wing arrays in the restart file's order, then movenum."""
+        out = np.zeros(11 * self.p.n_wings + 1)
+        lib().orc_body_state(self.h, _d(out))
+        return out
+
+    def move_mesh(self, dr: int):
+        """Test hook: one domain shift of the current state."""
+        lib().orc_move_mesh(self.h, dr)
+
+    def motion_rows(self) -> np.ndarray:
+        cols = 1 + 3 * self.p.n_wings
+        n = lib().orc_motion_rows(self.h, None, 0)
+        out = np.zeros((n, cols))
+        lib().orc_motion_rows(self.h, _d(out), n)
+        return out
 
     def surface_pressure(self, pre=None) -> np.ndarray:
         """cal_sur_cp on the last step's markers and IB nodes; ``pre`` (ny, nx)
