@@ -35,6 +35,10 @@ value = nodes per second of the vorticity kernel; roofline over its 64
 algorithmic bytes per node (u, v and two history planes read, the vorticity,
 two history planes and the rate written).
 
+--form dynamics: the step workload with the rigid-body dynamics of wing 1 on
+(the reference's body density and area, rk4): value = MLUPS of whole steps,
+each a window of its own (its geometry follows the force of the step before).
+
 --form serve: the serving loop.  Requests (cases drawn as for the sweep with
 nx up to 4096 and 2000..20000 steps, log-uniform) arrive as a Poisson process
 at the rate that offers 80% of the device's capacity (measured first on a
@@ -372,6 +376,54 @@ def bench_precision(args):
     print(json.dumps(out))
 
 
+def bench_dynamics(args):
+    """The step workload with the rigid-body dynamics of wing 1 on (the
+    reference's body parameters): each step's geometry is built after the
+    force of the step before, so the steps run one window each."""
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    K, W = args.steps, args.warmup
+    case = cases.config(1, n_steps=K + W + 8)
+    p = case.params
+    dyn = cases.Dynamics()
+    g = lbmib.LBMIB(case, device=0, dynamics=dyn)
+    if W:
+        g.evolution(W)
+    g.set_profiling(PROFILE_EVERY)
+    t0 = time.perf_counter()
+    g.evolution(K)
+    dt = time.perf_counter() - t0
+    tm = g.timings()
+    cells = p.nx * p.ny
+    interior = (p.nx - 2) * (p.ny - 2)
+    lat_s = tm["lattice_ms"] * 1e-3
+    achieved = interior * BYTES_PER_NODE_F64 / lat_s / 1e9
+    st = g.body_state()
+    out = {
+        "metric": "MLUPS (whole FP64 timestep with the rigid-body dynamics of wing 1)",
+        "value": round(cells * K / dt / 1e6, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(dt / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA0012 markers and sinusoidal kinematics)",
+        "config": {"workload": "%s: %s, rigid-body dynamics (solid_den %g, area %g, rk4)" % (
+            case.name, WORKLOADS[1], dyn.solid_den, dyn.area_airfoil), "nx": p.nx, "ny": p.ny,
+            "markers": p.pts_per_wing - 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_lattice<double>",
+                     "bytes_per_launch": interior * BYTES_PER_NODE_F64, "mean_launch_ms": round(lat_s * 1e3, 5)},
+        "phases_ms": {k: round(v, 5) for k, v in tm.items()},
+        "body": {"velocity": st[2], "drift_nodes": st[4], "domain_shifts": int(st[-1])},
+    }
+    g.close()
+    print(json.dumps(out))
+
+
 def bench_post(args):
     """Post-processing on the step workload: output's vorticity with one
     vor_mom_rate update per call (k_vorticity) and cal_sur_cp
@@ -450,7 +502,7 @@ def main():
                          "ib_collide_shared=0,ib_rhs_reserved=1 (default: the library's)")
     ap.add_argument("--solver", choices=["auto", "chain", "separator"], default="auto",
                     help="IB solve of the step form (auto: one chain in FP64, separator form in FP32)")
-    ap.add_argument("--form", choices=["step", "batch", "serve", "precision", "post"], default="step",
+    ap.add_argument("--form", choices=["step", "batch", "serve", "precision", "post", "dynamics"], default="step",
                     help="step: one case's timestep (default); batch: the 96-case parameter sweep; "
                          "serve: the continuous-batching service; precision: FP32 against FP64 on the "
                          "8192x4096 workload")
@@ -488,6 +540,8 @@ def main():
         return bench_precision(args)
     if args.form == "post":
         return bench_post(args)
+    if args.form == "dynamics":
+        return bench_dynamics(args)
     from fortran_loadfile_mitiwing_amd import cases, lbmib
     K, W = args.steps, args.warmup
     fp32 = args.precision == "fp32"

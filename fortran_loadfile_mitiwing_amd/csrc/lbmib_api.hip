@@ -819,6 +819,28 @@ int lbmib_set_body_state(lbmib_handle* h, const double* state, int32_t n)
     return 0;
 }
 
+int lbmib_get_edge_f0(lbmib_handle* h, double* first, double* last)
+{
+    if (!h) return fail(LBMIB_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->dev));
+    HIP_TRY(hipDeviceSynchronize());
+    const size_t n = 9 * (size_t)h->D.ny;
+    if (first) HIP_TRY(hipMemcpy(first, h->D.f0L, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (last) HIP_TRY(hipMemcpy(last, h->D.f0R, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbmib_set_edge_f0(lbmib_handle* h, const double* first, const double* last)
+{
+    if (!h) return fail(LBMIB_E_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->dev));
+    HIP_TRY(hipDeviceSynchronize());
+    const size_t n = 9 * (size_t)h->D.ny;
+    if (first) HIP_TRY(hipMemcpy(h->D.f0L, first, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (last) HIP_TRY(hipMemcpy(h->D.f0R, last, sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
 int lbmib_get_motion_rows(const lbmib_handle* h, double* rows, int32_t max_rows, int32_t* n_rows)
 {
     if (!h) return fail(LBMIB_E_ARG, "null handle");

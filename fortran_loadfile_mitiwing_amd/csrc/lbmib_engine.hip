@@ -242,7 +242,9 @@ int engine_prepare(Engine* E)
 int enqueue_forces(Engine* E, const Dims& d, int a, int b, long long gs_b)
 {
     if (b < a) return 0;
-    hipStream_t st = E->s[2];
+    // with per-step geometry the forces gate the next step: they go to the
+    // solves' stream, whose reserved CUs are idle once the IB nodes collide
+    hipStream_t st = E->per_step ? E->s[1] : E->s[2];
     HIP_TRY(hipStreamWaitEvent(st, E->ev_ib[gs_b % Engine::RING], 0));
     hipLaunchKernelGGL(k_forces, dim3(b - a + 1, d.nc), dim3(256), 0, st, (const DevCase*)E->dcases,
                        (const CaseClock*)E->dclock, a);
@@ -292,7 +294,7 @@ int launch_factor(Engine* E, const Dims& d, int off, int mode, hipEvent_t* gev)
 constexpr int LAG = Engine::RING / 2;
 
 int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
-                hipEvent_t* ev)
+                hipEvent_t wnodes, hipEvent_t* ev)
 {
     const int R = Engine::RING;
     const long long gs = E->gstep;
@@ -351,7 +353,7 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (ev) HIP_TRY(hipEventRecord(ev[4], sb));
     HIP_TRY(hipEventRecord(E->ev_ib[gs % R], sb));
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
-    if (wfirst) HIP_TRY(hipStreamWaitEvent(s0, wfirst, 0));
+    if (wfirst) HIP_TRY(hipStreamWaitEvent(s0, wnodes ? wnodes : wfirst, 0));
     if (wnext) HIP_TRY(hipStreamWaitEvent(s0, wnext, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
     const int npt = E->shape.npt == 2 ? 2 : 1;
@@ -383,7 +385,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
     const int every = prof ? E->prof_every : 0;
     const int Eper = 5;
     for (int off = 0; off < len; ++off) {
-        hipEvent_t wfirst = nullptr, wnext = nullptr;
+        hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
         if ((phase + off) % B == 0 || (fresh && off == 0)) {
             if (off > fupto) {
                 int rc = enqueue_forces(E, d, fupto, off - 1, E->gstep - 1);
@@ -410,7 +412,11 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             }
             int rc;
             if (fresh && off == 0) {
+                // the lattice update needs only the node sets; the IB
+                // solves wait for the factorisation as well
                 if ((rc = launch_geometry(E, d, off, GEO_FIRST))) return rc;
+                HIP_TRY(hipEventRecord(E->ev_nodes[bi % BR], st));
+                wnodes = E->ev_nodes[bi % BR];
                 if ((rc = launch_factor(E, d, off, GEO_FIRST, nullptr))) return rc;
                 HIP_TRY(hipEventRecord(E->ev_first[bi % BR], st));
                 wfirst = E->ev_first[bi % BR];
@@ -436,7 +442,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             }
             ev = &E->evpool[2 + (size_t)Eper * (off / every)];
         }
-        int rc = launch_step(E, d, off, off > 0, wfirst, wnext, ev);
+        int rc = launch_step(E, d, off, off > 0, wfirst, wnext, wnodes, ev);
         if (rc) return rc;
     }
     if (len > fupto) return enqueue_forces(E, d, fupto, len - 1, E->gstep - 1);
@@ -565,7 +571,7 @@ int engine_create(int dev, Engine** out)
     for (int i = 0; i < Engine::RING; ++i)
         for (hipEvent_t* ev : {&E->ev_ib[i], &E->ev_step[i], &E->ev_rhs[i], &E->ev_solve[i]}) ok = ok && mk(ev) == hipSuccess;
     for (int i = 0; i < Engine::BRING; ++i)
-        for (hipEvent_t* ev : {&E->ev_first[i], &E->ev_next[i]}) ok = ok && mk(ev) == hipSuccess;
+        for (hipEvent_t* ev : {&E->ev_first[i], &E->ev_next[i], &E->ev_nodes[i]}) ok = ok && mk(ev) == hipSuccess;
     ok = ok && mk(&E->ev_fork) == hipSuccess;
     for (int i = 0; i < Engine::NSTREAM; ++i) ok = ok && mk(&E->ev_join[i]) == hipSuccess;
     if (!ok) {
@@ -592,7 +598,7 @@ void engine_destroy(Engine* E)
         for (hipEvent_t e : {E->ev_ib[i], E->ev_step[i], E->ev_rhs[i], E->ev_solve[i]})
             if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < Engine::BRING; ++i)
-        for (hipEvent_t e : {E->ev_first[i], E->ev_next[i]})
+        for (hipEvent_t e : {E->ev_first[i], E->ev_next[i], E->ev_nodes[i]})
             if (e) (void)hipEventDestroy(e);
     if (E->ev_fork) (void)hipEventDestroy(E->ev_fork);
     for (hipEvent_t e : E->ev_join)

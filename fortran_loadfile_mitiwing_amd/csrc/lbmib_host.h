@@ -97,6 +97,7 @@ struct Engine {
     hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {}, ev_solve[RING] = {};
     static constexpr int BRING = 8;  // per-boundary events
     hipEvent_t ev_first[BRING] = {}, ev_next[BRING] = {}, ev_fork = nullptr, ev_join[NSTREAM] = {};
+    hipEvent_t ev_nodes[BRING] = {};  // a fresh batch's node sets built (its factorisation may still run)
     long long gstep = 0, gbound = 0;
     std::vector<lbmib_handle*> members;
     // device tables

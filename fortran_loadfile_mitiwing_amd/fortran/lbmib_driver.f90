@@ -29,13 +29,23 @@ program lbmib_driver
     ! vorticity_rate = 1 calls vor_mom_rate after every step (the call
     ! main.f90:391 leaves commented out) and fills the dvordt column
     integer :: vorticity_rate = 0
+    ! rigid-body dynamics of wing 1 (main.f90:76, 101-129): off by default,
+    ! as the reference's parameter solve_dynamic_equations
+    integer :: solve_dynamic_equations = 0, if_rk4 = 1, closed_boundary = 1
+    real(c_double) :: solid_den = 10.0d0, area_airfoil = 6.807946476488794d-2
+    real(c_double) :: par1 = 0.0d0, par2 = 0.0d0, alphad = 10.0d0 * 3.141592653589793d0 / 180.0d0
     character(len=256) :: output_dir = '.', restart_out = 'saved_flow_feild.dat', restart_in = ''
     namelist /lbmib_case/ nx, ny, ox, oy, scale, UU, xc, yc, xp, yp, s, smooth, wing_count, &
         points_per_wing, boundary_dir, motion_file, force_file, motion_out_file, fields_file, &
         chunk, device, report_every, output_every, output_stride, restart_every, output_dir, &
-        restart_out, restart_in, vorticity_rate
+        restart_out, restart_in, vorticity_rate, solve_dynamic_equations, if_rk4, closed_boundary, &
+        solid_den, area_airfoil, par1, par2, alphad
 
     type(lbmib_params) :: p
+    type(lbmib_dynamics) :: dyn
+    real(c_double), allocatable :: body_state(:), mrows(:)
+    real(c_double), allocatable, target :: edge_first(:, :), edge_last(:, :)
+    integer :: movenum = 0
     type(c_ptr) :: h
     character(len=256) :: nml_file, fname
     character(len=32) :: idx_text
@@ -136,6 +146,18 @@ program lbmib_driver
     p%pts_per_wing = points_per_wing
     p%device = device
     call lbmib_check(lbmib_create(p, xb0, yb0, h), 'lbmib_create')
+    if (solve_dynamic_equations .ne. 0) then
+        dyn%enabled = 1
+        dyn%rk4 = if_rk4
+        dyn%closed_boundary = closed_boundary
+        dyn%solid_den = solid_den
+        dyn%area_airfoil = area_airfoil
+        dyn%par1 = par1
+        dyn%par2 = par2
+        dyn%alphad = alphad
+        call lbmib_check(lbmib_set_dynamics(h, dyn), 'lbmib_set_dynamics')
+    endif
+    allocate(body_state(11 * wing_count + 1), edge_first(ny, 9), edge_last(ny, 9))
     call lbmib_check(lbmib_set_motion_table(h, motion, int(nrows, c_int32_t), int(ncols, c_int32_t)), &
                      'lbmib_set_motion_table')
     call lbmib_check(lbmib_max_steps(h, got), 'lbmib_max_steps')
@@ -157,6 +179,9 @@ program lbmib_driver
         call read_restart(trim(restart_in), n)
         call lbmib_check(lbmib_set_step(h, int(n, c_int32_t)), 'lbmib_set_step')
         call lbmib_check(lbmib_upload_state(h, f), 'lbmib_upload_state')
+        call lbmib_check(lbmib_set_body_state(h, body_state, int(size(body_state), c_int32_t)), &
+                         'lbmib_set_body_state')
+        call lbmib_check(lbmib_set_edge_f0(h, c_loc(edge_first), c_loc(edge_last)), 'lbmib_set_edge_f0')
         write(*, '(a, i0)') 'resumed after step ', n
     endif
     n0 = n
@@ -216,15 +241,12 @@ program lbmib_driver
     else
         open(newunit=unit_id, file=trim(motion_out_file), status='replace', action='write')
     endif
+    ! the body's extra velocity and shift per wing (zero with the dynamics off)
+    col = 1 + 3 * wing_count
+    allocate(mrows(col * max(nmax, 1)))
+    call lbmib_check(lbmib_get_motion_rows(h, mrows, int(nmax, c_int32_t), got), 'lbmib_get_motion_rows')
     do k = n0 + 1, nmax
-        ! with the dynamics off the body's extra velocity and shift are zero
-        row(1) = hist(1, k)
-        col = 2
-        do wing = 1, wing_count
-            row(col:col + 2) = 0.0d0
-            col = col + 3
-        enddo
-        write(unit_id, '(1000f15.8)') row(1:col - 1)
+        write(unit_id, '(1000f15.8)') mrows((k - 1) * col + 1:k * col)
     enddo
     close(unit_id)
 
@@ -273,6 +295,10 @@ contains
         npts = wing_count * points_per_wing
         allocate(xb1(npts), yb1(npts))
         call lbmib_check(lbmib_markers(h, xb1, yb1), 'lbmib_markers')
+        ! the domain has followed the body movenum nodes (solution.f90:210)
+        call lbmib_check(lbmib_get_body_state(h, body_state, int(size(body_state), c_int32_t)), &
+                         'lbmib_get_body_state')
+        movenum = nint(body_state(size(body_state)))
         write(nm, '(i0)') cycle
         open(newunit=o, file=trim(output_dir) // '/F' // trim(nm) // '.dat', status='replace', action='write')
         write(o, '(a)') 'VARIABLES = "X" "Y" "U" "V" "Presure" "Vorticity" "body" "dvordt" '
@@ -284,7 +310,8 @@ contains
             cycle * UU * scale
         do j = 1, ny, output_stride
             do i = 1, nx, output_stride
-                write(o, '(25e16.8)') ox + dble(i - 1) * scale, oy + dble(j - 1) * scale, u(i, j) / UU, &
+                write(o, '(25e16.8)') ox + dble(i - 1) * scale + dble(movenum) * scale, oy + dble(j - 1) * scale, &
+                    u(i, j) / UU, &
                     v(i, j) / UU, rho(i, j) / 3.0d0, vor(i, j), dble(body(i, j)), dvr(i, j)
             enddo
         enddo
@@ -310,26 +337,27 @@ contains
     end subroutine
 
     ! saveflowfield (solution.f90:1604-1707): the step, the wing count, the
-    ! rigid-body dynamics state (zero: the dynamics are off), the domain
-    ! shift count (zero), then per node u, v and f(1:9), then the edge
-    ! records (coarse-to-fine copies and stale f0, all at the initial
-    ! equilibrium for one mesh).  Raw stream, as form='binary' writes it.
+    ! rigid-body state and the domain shift count, then per node u, v and
+    ! f(1:9), then the edge records: the coarse-to-fine copies (the initial
+    ! equilibrium for one mesh) and the stale f0 of the edge columns (from
+    ! the device) and rows (the initial equilibrium: no step reads them, so
+    ! the device does not keep them).  Raw stream, as form='binary' writes it.
     subroutine write_restart(cycle)
         integer, intent(in) :: cycle
-        integer :: i, j, o
+        integer :: i, j, o, nb
         integer(c_int32_t) :: i4
-        real(c_double), allocatable :: zeros(:)
         call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(f)), 'lbmib_download')
-        allocate(zeros(11 * wing_count))
-        zeros = 0.0d0
+        nb = 11 * wing_count
+        call lbmib_check(lbmib_get_body_state(h, body_state, int(nb + 1, c_int32_t)), 'lbmib_get_body_state')
+        call lbmib_check(lbmib_get_edge_f0(h, c_loc(edge_first), c_loc(edge_last)), 'lbmib_get_edge_f0')
         open(newunit=o, file=trim(output_dir) // '/' // trim(restart_out), status='replace', access='stream', &
              form='unformatted', action='write')
         i4 = cycle
         write(o) i4
         i4 = wing_count
         write(o) i4
-        write(o) zeros
-        i4 = 0
+        write(o) body_state(1:nb)
+        i4 = nint(body_state(nb + 1))
         write(o) i4
         do j = 1, ny
             do i = 1, nx
@@ -337,24 +365,29 @@ contains
             enddo
         enddo
         do j = 2, ny - 1
-            write(o) feq0, feq0, feq0, feq0
+            write(o) feq0, edge_first(j, :), feq0, edge_last(j, :)
         enddo
         do j = 1, nx
-            write(o) feq0, feq0, feq0, feq0
+            if (j .eq. 1) then
+                write(o) feq0, edge_first(ny, :), feq0, edge_first(1, :)
+            else if (j .eq. nx) then
+                write(o) feq0, edge_last(ny, :), feq0, edge_last(1, :)
+            else
+                write(o) feq0, feq0, feq0, feq0
+            endif
         enddo
         close(o)
     end subroutine
 
-    ! readflowfield (solution.f90:1709-1839): the step and the populations;
-    ! the velocity fields are recomputed by the next step and the edge
-    ! records hold the initial equilibrium, so both are skipped.
+    ! readflowfield (solution.f90:1709-1839): the step, the rigid-body state,
+    ! the populations and the stale f0 of the edge columns; the velocity
+    ! fields are recomputed by the next step.
     subroutine read_restart(fname, cycle)
         character(len=*), intent(in) :: fname
         integer, intent(out) :: cycle
-        integer :: i, j, o, rios
+        integer :: i, j, o, rios, nb
         integer(c_int32_t) :: i4, nwf
-        real(c_double), allocatable :: skip(:)
-        real(c_double) :: uu1, vv1
+        real(c_double) :: uu1, vv1, skip(9)
         open(newunit=o, file=fname, status='old', access='stream', form='unformatted', action='read', iostat=rios)
         if (rios .ne. 0) then
             write(*, *) 'cannot open restart file: ', fname
@@ -367,9 +400,10 @@ contains
             write(*, *) 'restart file wing count ', nwf, ' differs from wing_count ', wing_count
             stop 1
         endif
-        allocate(skip(11 * wing_count))
-        read(o) skip
+        nb = 11 * wing_count
+        read(o) body_state(1:nb)
         read(o) i4
+        body_state(nb + 1) = dble(i4)
         do j = 1, ny
             do i = 1, nx
                 read(o, iostat=rios) uu1, vv1, f(i, j, :)
@@ -378,6 +412,18 @@ contains
                     stop 1
                 endif
             enddo
+        enddo
+        do j = 2, ny - 1
+            read(o) skip, edge_first(j, :), skip, edge_last(j, :)
+        enddo
+        do j = 1, nx
+            if (j .eq. 1) then
+                read(o) skip, edge_first(ny, :), skip, edge_first(1, :)
+            else if (j .eq. nx) then
+                read(o) skip, edge_last(ny, :), skip, edge_last(1, :)
+            else
+                read(o) skip, skip, skip, skip
+            endif
         enddo
         close(o)
     end subroutine

@@ -62,7 +62,7 @@ module lbmib
     public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure
     ! rigid-body dynamics
     public :: lbmib_dynamics, lbmib_set_dynamics, lbmib_get_body_state, lbmib_set_body_state
-    public :: lbmib_get_motion_rows
+    public :: lbmib_get_motion_rows, lbmib_get_edge_f0, lbmib_set_edge_f0
     public :: lbmib_destroy, lbmib_error_message, lbmib_version_string, lbmib_check
     ! batched form (many cases advanced together) and footprints
     public :: lbmib_set_shape, lbmib_get_shape, lbmib_device_bytes, lbmib_mem_info
@@ -151,6 +151,17 @@ module lbmib
             type(c_ptr), value :: h
             real(c_double), intent(in) :: state(*)
             integer(c_int32_t), value :: n
+        end function
+
+        ! first, last: f0(1,:,:) and f0(Nx,:,:) as (Ny, 9) arrays, c_loc or c_null_ptr
+        integer(c_int) function lbmib_get_edge_f0(h, first, last) bind(C, name="lbmib_get_edge_f0")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h, first, last
+        end function
+
+        integer(c_int) function lbmib_set_edge_f0(h, first, last) bind(C, name="lbmib_set_edge_f0")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h, first, last
         end function
 
         ! rows(1 + 3*wing_count, max_rows)

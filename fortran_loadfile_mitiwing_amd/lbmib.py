@@ -92,7 +92,7 @@ _lib = None
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
            "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure", "lbmib_set_dynamics", "lbmib_get_body_state",
-           "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_get_edge_f0", "lbmib_set_edge_f0", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
@@ -124,6 +124,8 @@ def load(path: str = LIB_PATH):
     L.lbmib_get_body_state.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_set_body_state.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_get_motion_rows.argtypes = [vp, _dp, C.c_int32, C.POINTER(C.c_int32)]
+    L.lbmib_get_edge_f0.argtypes = [vp, _dp, _dp]
+    L.lbmib_set_edge_f0.argtypes = [vp, _dp, _dp]
     L.lbmib_vorticity.argtypes = [vp, C.c_int32, _dp, _dp]
     L.lbmib_surface_pressure.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_residual.argtypes = [vp, _dp]
@@ -293,6 +295,17 @@ class LBMIB:
     def set_body_state(self, state):
         st = np.ascontiguousarray(state, dtype=np.float64)
         _check(load().lbmib_set_body_state(self.h, _d(st), st.size))
+
+    def edge_f0(self):
+        """Stale f0 of the first and last columns, each (9, ny)."""
+        a = np.zeros((9, self.p.ny)); b = np.zeros((9, self.p.ny))
+        _check(load().lbmib_get_edge_f0(self.h, _d(a), _d(b)))
+        return a, b
+
+    def set_edge_f0(self, first, last):
+        a = np.ascontiguousarray(first, dtype=np.float64)
+        b = np.ascontiguousarray(last, dtype=np.float64)
+        _check(load().lbmib_set_edge_f0(self.h, _d(a), _d(b)))
 
     def motion_rows(self) -> np.ndarray:
         """airfoil_motion.dat rows of the steps taken."""

@@ -185,6 +185,14 @@ int lbmib_set_dynamics(lbmib_handle* h, const lbmib_dynamics* d);
 int lbmib_get_body_state(const lbmib_handle* h, double* state, int32_t n);
 int lbmib_set_body_state(lbmib_handle* h, const double* state, int32_t n);
 
+/* The stale f0 of the first and last lattice columns (f0(1,:,:) and
+ * f0(Nx,:,:) of the restart file's edge records, solution.f90:1663-1668,
+ * 1750-1756), each ny*9 values with j fastest.  They hold the initial
+ * equilibrium until a domain shift refills one of them.  Either pointer may
+ * be NULL. */
+int lbmib_get_edge_f0(lbmib_handle* h, double* first, double* last);
+int lbmib_set_edge_f0(lbmib_handle* h, const double* first, const double* last);
+
 /* airfoil_motion.dat rows (solution.f90:378-388), one per step taken:
  * time, then per wing 0, body_extra_v/UU, body_extra_s*scale (1 + 3*n_wings
  * columns; zero offsets with the dynamics off). */

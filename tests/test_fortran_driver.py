@@ -198,3 +198,41 @@ def test_driver_vorticity_rate_column(tmp_path, oracle_mod):
     assert np.allclose(data[:, :, 5], vor, rtol=1e-7, atol=1e-9)
     assert np.allclose(data[:, :, 7], rate, rtol=1e-7, atol=1e-9)
     assert np.abs(rate).max() > 0.0 and np.all(data[0, :, 7] == 0.0)
+
+
+@pytest.mark.gpu
+def test_driver_dynamics_and_restart_after_a_domain_shift(tmp_path, oracle_mod):
+    """solve_dynamic_equations = 1: force.dat and airfoil_motion.dat rows
+    against the oracle's; a restart file written after the domain shift
+    (body state, movenum, refilled edge f0) resumes bit-identically."""
+    exe = driver()
+    case = cases.config(0, n_steps=520, scale_down=2)
+    d = str(tmp_path)
+    dyn = dict(solve_dynamic_equations=1, solid_den=200.0)
+    write_inputs(d, case, nml="a.nml", restart_every=500, force_file="force_a.dat", fields="a.bin",
+                 motion_out_file="motion_a.dat", report_every=0, **dyn)
+    print("fortran driver: 520 steps with the dynamics, restart file at step 500", flush=True)
+    run_driver(exe, d, "a.nml")
+    raw = open(os.path.join(d, "saved_flow_feild.dat"), "rb").read()
+    p = case.params
+    assert np.frombuffer(raw[8 + 8 * 11 * p.n_wings:12 + 8 * 11 * p.n_wings], dtype=np.int32)[0] == 1  # movenum
+    write_inputs(d, case, nml="b.nml", restart_in="saved_flow_feild.dat", restart_out="unused.dat",
+                 force_file="force_b.dat", fields="b.bin", motion_out_file="motion_b.dat", report_every=0, **dyn)
+    out = run_driver(exe, d, "b.nml")
+    assert "resumed after step 500" in out
+    assert np.array_equal(np.fromfile(os.path.join(d, "a.bin")), np.fromfile(os.path.join(d, "b.bin")))
+    ra = open(os.path.join(d, "force_a.dat")).read().splitlines()
+    rb = open(os.path.join(d, "force_b.dat")).read().splitlines()
+    assert len(ra) == 520 and rb == ra[500:]
+    ma = open(os.path.join(d, "motion_a.dat")).read().splitlines()
+    mb = open(os.path.join(d, "motion_b.dat")).read().splitlines()
+    assert mb == ma[500:]
+    xr, yr = cases.read_boundary(os.path.join(d, "boundary.dat"), len(case.xb0))
+    case.xb0, case.yb0 = xr, yr
+    o = oracle_mod.Oracle(case, n_threads=8)
+    o.set_dynamics(cases.Dynamics(solid_den=200.0))
+    o.step(520)
+    assert np.max(np.abs(np.loadtxt(os.path.join(d, "force_a.dat")) - o.history())) <= 0.5e-8 + 1e-15
+    m = np.loadtxt(os.path.join(d, "motion_a.dat"))
+    assert m.shape == (520, 4) and np.max(np.abs(m - o.motion_rows())) <= 0.5e-8 + 1e-15
+    assert np.abs(m[:, 3]).max() > 0.01
