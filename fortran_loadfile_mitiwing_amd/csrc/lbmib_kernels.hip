@@ -1581,6 +1581,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     const double U = D.U;
     double* raw = D.raw ? D.raw + (long long)step * 3 * D.nw : nullptr;
     auto flush = [&](int w) {
+        if (tid != 0) return;
         if (raw) { raw[3 * w] = F0; raw[3 * w + 1] = F1; raw[3 * w + 2] = M; }
         hist[3 * w + 0] = 2.0 * F0 / (U * U * (1.0 / sc));
         hist[3 * w + 1] = 2.0 * F1 / (U * U * (1.0 / sc));
@@ -1592,35 +1593,51 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     for (int ch = 0; ch < nch; ++ch) {
         if (tid >= 64) {
             if (ch + 1 < nch) form(ch + 1, 64, 192);
-        } else if (tid == 0) {
-            // ascending node order; the terms of PF nodes are read ahead of
-            // their additions so that the LDS latency overlaps the sums
+        } else {
+            // wave 0 sums the chunk in ascending node order, every lane the
+            // same values: each run of one wing's nodes (wing ids never
+            // decrease along the list) is found with a ballot, then summed
+            // with no branch per node, the terms of PF nodes read ahead
             const int cnt = min(FCH, na - ch * FCH), b = ch & 1;
-            constexpr int PF = 8;
-            for (int q0 = 0; q0 < cnt; q0 += PF) {
-                double t0[PF], t1[PF], t2[PF], t3[PF];
-                int wq[PF];
-#pragma unroll
-                for (int u = 0; u < PF; ++u) {
-                    const int q = min(q0 + u, cnt - 1);
-                    t0[u] = terms[b][q][0];
-                    t1[u] = terms[b][q][1];
-                    t2[u] = terms[b][q][2];
-                    t3[u] = terms[b][q][3];
-                    wq[u] = wing[b][q];
+            int q = 0;
+            while (q < cnt) {
+                const int w = __builtin_amdgcn_readfirstlane(wing[b][q]);
+                if (w != curw) {
+                    flush(curw);
+                    for (int x = curw + 1; x < w; ++x) { F0 = F1 = M = 0.0; flush(x); }
+                    curw = w;
+                    F0 = F1 = M = 0.0;
                 }
-#pragma unroll
-                for (int u = 0; u < PF; ++u) {
-                    if (q0 + u >= cnt) break;
-                    if (wq[u] != curw) {
-                        flush(curw);
-                        for (int x = curw + 1; x < wq[u]; ++x) { F0 = F1 = M = 0.0; flush(x); }
-                        curw = wq[u];
-                        F0 = F1 = M = 0.0;
+                int e = cnt;
+                for (int base = q + 1; base < cnt; base += 64) {
+                    const int idx = base + tid;
+                    const unsigned long long other = __ballot(idx < cnt && wing[b][idx] != w);
+                    if (other) {
+                        e = base + __builtin_ctzll(other);
+                        break;
                     }
-                    F0 = F0 - t0[u];
-                    F1 = F1 - t1[u];
-                    M = M - t2[u] + t3[u];
+                }
+                constexpr int PF = 8;
+                for (; q + PF <= e; q += PF) {
+                    double t0[PF], t1[PF], t2[PF], t3[PF];
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) {
+                        t0[u] = terms[b][q + u][0];
+                        t1[u] = terms[b][q + u][1];
+                        t2[u] = terms[b][q + u][2];
+                        t3[u] = terms[b][q + u][3];
+                    }
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) {
+                        F0 = F0 - t0[u];
+                        F1 = F1 - t1[u];
+                        M = M - t2[u] + t3[u];
+                    }
+                }
+                for (; q < e; ++q) {
+                    F0 = F0 - terms[b][q][0];
+                    F1 = F1 - terms[b][q][1];
+                    M = M - terms[b][q][2] + terms[b][q][3];
                 }
             }
         }
