@@ -85,6 +85,16 @@ def test_pitch_plunge_small(oracle_mod, lbmib_mod):
     assert_parity(o, g)
 
 
+@pytest.mark.parametrize("index,steps", [(1, 18), (2, 3)])
+def test_full_size_workloads_first_steps(oracle_mod, lbmib_mod, index, steps):
+    """The timed workloads at their full sizes (4096x2048 with 1024 markers,
+    8192x4096 with 2048 markers): the first steps bit-identical to the
+    oracle, the 4096x2048 one across its first geometry batch boundary."""
+    case = cases.config(index, n_steps=steps + 8)
+    o, g = run_pair(case, steps, oracle_mod, lbmib_mod)
+    assert_parity(o, g)
+
+
 def test_peskin_kernel(oracle_mod, lbmib_mod):
     case = cases.make_case(160, 80, 16.0, 48, 200.0, 40, le=(40.0, 40.0), heave=0.25,
                            pitch_deg=10.0, smooth=0, name="peskin")
