@@ -236,9 +236,10 @@ int lbmib_surface_pressure(lbmib_handle* h, double* p, int32_t n);
 int lbmib_set_profiling(lbmib_handle* h, int32_t every);
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 
-/* Device memory the handle holds (all of it allocated by lbmib_create and
- * lbmib_set_motion_table), for admission by HBM footprint; and the device's
- * free and total memory.  No reference counterpart. */
+/* Device memory the handle holds (allocated by lbmib_create and
+ * lbmib_set_motion_table, plus the planes of the post-processing calls once
+ * used), for admission by HBM footprint; and the device's free and total
+ * memory.  No reference counterpart. */
 int lbmib_device_bytes(const lbmib_handle* h, int64_t* bytes);
 int lbmib_mem_info(int32_t device, int64_t* free_bytes, int64_t* total_bytes);
 
