@@ -1271,15 +1271,60 @@ static inline void meq_nee(double rh, double uu, double vv, double* t)
 }
 
 /* f_edge = M^-1 [ meq(edge) + (1-s) (M f0_nb - meq(nb)) ] */
-static inline void nee(const orc_case* c, double rhe, double ue, double ve,
-                       double rhn, double un, double vn, const double* f0nb, double* out)
+static inline void nee_s(const double* minv, const double* s, double rhe, double ue, double ve, double rhn,
+                         double un, double vn, const double* f0nb, double* out)
 {
     double t1[Q], t2[Q], t3[Q];
     meq_nee(rhe, ue, ve, t1);
     meq_nee(rhn, un, vn, t2);
     matmulvoc(t3, &MATM[0][0], f0nb);
-    for (int k = 0; k < Q; ++k) t3[k] = t1[k] + (1.0 - c->p.s[k]) * (t3[k] - t2[k]);
-    matmulvoc(out, c->minv, t3);
+    for (int k = 0; k < Q; ++k) t3[k] = t1[k] + (1.0 - s[k]) * (t3[k] - t2[k]);
+    matmulvoc(out, minv, t3);
+}
+static inline void nee(const orc_case* c, double rhe, double ue, double ve,
+                       double rhn, double un, double vn, const double* f0nb, double* out)
+{
+    nee_s(c->minv, c->p.s, rhe, ue, ve, rhn, un, vn, f0nb, out);
+}
+
+/* characteristic update of an inflow (side 0, boundary_condition.f90:418-440)
+ * or outflow (side 1, 500-521) edge node from its own state (tr, tu, tv) and
+ * the next two nodes inwards */
+static inline void char_update(int side, double cs, double tr, double tu, double tv, double r2, double u2,
+                               double v2, double r3, double u3, double v3, double* ro, double* uo, double* vo)
+{
+    double lx1, lx2, lx3;
+    if (side == 0) {
+        double rhox = (-3.0 * tr + 4.0 * r2 - r3) * 0.5;
+        double ux = (-3.0 * tu + 4.0 * u2 - u3) * 0.5;
+        double vx = (-3.0 * tv + 4.0 * v2 - v3) * 0.5;
+        lx1 = (tu - cs) * (cs * cs * rhox - cs * tr * ux);
+        lx2 = (tu < 0.0) ? tu * vx : 0.0;
+        lx3 = 0.0;
+    } else {
+        double rhox = (3.0 * tr - 4.0 * r2 + r3) * 0.5;
+        double ux = (3.0 * tu - 4.0 * u2 + u3) * 0.5;
+        double vx = (3.0 * tv - 4.0 * v2 + v3) * 0.5;
+        lx1 = 0.0;
+        lx2 = (tu >= 0.0) ? tu * vx : 0.0;
+        lx3 = (tu + cs) * (cs * cs * rhox + cs * tr * ux);
+    }
+    *ro = tr - (lx1 + lx3) * 1.5;
+    *uo = tu + (lx1 - lx3) / (2.0 * cs * *ro);
+    *vo = tv - lx2;
+}
+
+void orc_nee_node(const double s[9], double rhe, double ue, double ve, double rhn, double un, double vn,
+                  const double f0n[9], double out[9])
+{
+    double minv[Q * Q];
+    orc_minv(minv);
+    nee_s(minv, s, rhe, ue, ve, rhn, un, vn, f0n, out);
+}
+
+void orc_char_update(int side, double cs, const double in[9], double out[3])
+{
+    char_update(side, cs, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8], &out[0], &out[1], &out[2]);
 }
 
 /* boundary_condition1 (boundary_condition.f90:288-858) with periodical_bc=0,
@@ -1303,17 +1348,9 @@ static void boundary_condition1(orc_case* c)
 
     /* left inflow (413-483) */
     for (int j = 0; j < ny; ++j) { tr[j] = rho[IDX(0, j)]; tu[j] = u[IDX(0, j)]; tv[j] = v[IDX(0, j)]; }
-    for (int j = 1; j < ny - 1; ++j) {
-        double rhox = (-3.0 * tr[j] + 4.0 * rho[IDX(1, j)] - rho[IDX(2, j)]) * 0.5;
-        double ux = (-3.0 * tu[j] + 4.0 * u[IDX(1, j)] - u[IDX(2, j)]) * 0.5;
-        double vx = (-3.0 * tv[j] + 4.0 * v[IDX(1, j)] - v[IDX(2, j)]) * 0.5;
-        double lx1 = (tu[j] - cs) * (cs * cs * rhox - cs * tr[j] * ux);
-        double lx2 = (tu[j] < 0.0) ? tu[j] * vx : 0.0;
-        double lx3 = 0.0;
-        rho[IDX(0, j)] = rho[IDX(0, j)] - (lx1 + lx3) * 1.5;
-        u[IDX(0, j)] = u[IDX(0, j)] + (lx1 - lx3) / (2.0 * cs * rho[IDX(0, j)]);
-        v[IDX(0, j)] = v[IDX(0, j)] - lx2;
-    }
+    for (int j = 1; j < ny - 1; ++j)
+        char_update(0, cs, tr[j], tu[j], tv[j], rho[IDX(1, j)], u[IDX(1, j)], v[IDX(1, j)], rho[IDX(2, j)],
+                    u[IDX(2, j)], v[IDX(2, j)], &rho[IDX(0, j)], &u[IDX(0, j)], &v[IDX(0, j)]);
     for (int j = 1; j < ny - 1; ++j) {
         for (int k = 0; k < Q; ++k) f0n[k] = c->f0[k * N + IDX(1, j)];
         nee(c, rho[IDX(0, j)], u[IDX(0, j)], v[IDX(0, j)], rho[IDX(1, j)], u[IDX(1, j)], v[IDX(1, j)], f0n, out);
@@ -1321,17 +1358,10 @@ static void boundary_condition1(orc_case* c)
     }
     /* right outflow (487-569) */
     for (int j = 0; j < ny; ++j) { tr[j] = rho[IDX(nx - 1, j)]; tu[j] = u[IDX(nx - 1, j)]; tv[j] = v[IDX(nx - 1, j)]; }
-    for (int j = 1; j < ny - 1; ++j) {
-        double rhox = (3.0 * tr[j] - 4.0 * rho[IDX(nx - 2, j)] + rho[IDX(nx - 3, j)]) * 0.5;
-        double ux = (3.0 * tu[j] - 4.0 * u[IDX(nx - 2, j)] + u[IDX(nx - 3, j)]) * 0.5;
-        double vx = (3.0 * tv[j] - 4.0 * v[IDX(nx - 2, j)] + v[IDX(nx - 3, j)]) * 0.5;
-        double lx1 = 0.0;
-        double lx2 = (tu[j] >= 0.0) ? tu[j] * vx : 0.0;
-        double lx3 = (tu[j] + cs) * (cs * cs * rhox + cs * tr[j] * ux);
-        rho[IDX(nx - 1, j)] = rho[IDX(nx - 1, j)] - (lx1 + lx3) * 1.5;
-        u[IDX(nx - 1, j)] = u[IDX(nx - 1, j)] + (lx1 - lx3) / (2.0 * cs * rho[IDX(nx - 1, j)]);
-        v[IDX(nx - 1, j)] = v[IDX(nx - 1, j)] - lx2;
-    }
+    for (int j = 1; j < ny - 1; ++j)
+        char_update(1, cs, tr[j], tu[j], tv[j], rho[IDX(nx - 2, j)], u[IDX(nx - 2, j)], v[IDX(nx - 2, j)],
+                    rho[IDX(nx - 3, j)], u[IDX(nx - 3, j)], v[IDX(nx - 3, j)], &rho[IDX(nx - 1, j)],
+                    &u[IDX(nx - 1, j)], &v[IDX(nx - 1, j)]);
     for (int j = 1; j < ny - 1; ++j) {
         for (int k = 0; k < Q; ++k) f0n[k] = c->f0[k * N + IDX(nx - 2, j)];
         nee(c, rho[IDX(nx - 1, j)], u[IDX(nx - 1, j)], v[IDX(nx - 1, j)], rho[IDX(nx - 2, j)], u[IDX(nx - 2, j)],

@@ -20,15 +20,20 @@
  *   node set (brute force), marker positions and velocities (closed-form
  *   rotation, finite differences), the IB correction (D (u* + du) = U_B),
  *   dense vs sparse assembly (bit-identical), symmetric-airfoil zero lift,
- *   mirrored-plunge lift, the force-row layout and erranalysis.
+ *   mirrored-plunge lift, the force-row layout and erranalysis;
+ *   boundary_condition1's characteristic update against the textbook LODI
+ *   equations, its NEE nodes against the second-order equilibrium and the
+ *   per-moment (1 - s) scaling, and its wiring (walls, edge columns) on a
+ *   stepped case (tests/test_oracle_boundary.py); post-processing, body
+ *   dynamics and move_mesh (tests/test_oracle_post.py, test_oracle_dynamics.py).
  * PARITY UNPINNED:
- *   - boundary_condition1 (boundary_condition.f90:288-858): the characteristic
- *     + non-equilibrium-extrapolation outer boundary, its overwrites and the
- *     stale-ring / corner semantics are a line-by-line restatement; nothing
- *     but the reference itself fixes those ad-hoc formulas, and the reference
- *     does not build here (it needs mpi, cublas and cusolverDn modules).  The
- *     symmetric and mirrored whole-step tests exercise it only through the
- *     symmetries it must keep.
+ *   - boundary_condition1's corner nodes and the stale-ring semantics (which
+ *     f0 the edge state is taken from, and that the reference divides the
+ *     velocity update by the updated density) are a line-by-line restatement
+ *     that only the reference itself could fix, and the reference does not
+ *     build here (it needs mpi, cublas and cusolverDn modules).  The symmetric
+ *     and mirrored whole-step tests exercise them only through the
+ *     symmetries they must keep.
  *   - the round-off order of the reference's cuBLAS DGEMM/DGEMV and cuSOLVER
  *     DGETRF/DGETRS calls (NVHPC 23.11 / CUDA 12.3, closed source): solver=1
  *     restates them as textbook LAPACK; the GPU path and solver=0 use a banded
@@ -128,6 +133,14 @@ int orc_build_ib_set(const orc_case* c, const double* xb2, const double* yb2,
 int orc_set_dynamics(orc_case* c, const orc_dynamics* d);
 int orc_body_state(const orc_case* c, double* out);
 int orc_motion_rows(const orc_case* c, double* rows, int max_rows);
+/* Test hooks on boundary_condition1's pieces: one NEE edge node
+ * (boundary_condition.f90:444-483), f = M^-1 [meq(edge) + (1-s)(M f0n -
+ * meq(nb))], and the characteristic update of an inflow (side 0, 418-440) or
+ * outflow (side 1, 500-521) edge node; in = {rho, u, v} of the edge node, of
+ * the next node inwards and of the one after, out = the updated rho, u, v. */
+void orc_nee_node(const double s[9], double rhe, double ue, double ve, double rhn, double un, double vn,
+                  const double f0n[9], double out[9]);
+void orc_char_update(int side, double cs, const double in[9], double out[3]);
 /* Test hook: one move_mesh of the current state (dr = 1 or -1). */
 void orc_move_mesh(orc_case* c, int dr);
 

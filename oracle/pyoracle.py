@@ -78,6 +78,9 @@ def lib():
         L.orc_body_state.argtypes = [C.c_void_p, _dp]
         L.orc_motion_rows.argtypes = [C.c_void_p, _dp, C.c_int]
         L.orc_move_mesh.argtypes = [C.c_void_p, C.c_int]
+        L.orc_nee_node.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   _dp, _dp]
+        L.orc_char_update.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.orc_vorticity.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_vorticity_rate.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_surface_pressure.argtypes = [C.c_void_p, _dp, _dp]
@@ -136,6 +139,25 @@ def motion_step_table(motion: np.ndarray, dt: float) -> np.ndarray:
     out = np.zeros((n, ncols))
     rc = lib().orc_motion_step_table(_d(m), rows, ncols, dt, _d(out), n)
     assert rc == n, rc
+    return out
+
+
+def nee_node(s, edge, nb, f0n) -> np.ndarray:
+    """One NEE edge node: edge = (rho, u, v) of the edge, nb = that of the
+    neighbour whose populations are f0n."""
+    sv = np.ascontiguousarray(s, dtype=np.float64)
+    f0 = np.ascontiguousarray(f0n, dtype=np.float64)
+    out = np.zeros(9)
+    lib().orc_nee_node(_d(sv), edge[0], edge[1], edge[2], nb[0], nb[1], nb[2], _d(f0), _d(out))
+    return out
+
+
+def char_update(side: int, cs_: float, states) -> np.ndarray:
+    """Characteristic update of an edge node; states = 3 x (rho, u, v) from
+    the edge inwards; side 0 inflow, 1 outflow."""
+    a = np.ascontiguousarray(np.asarray(states, dtype=np.float64).reshape(9))
+    out = np.zeros(3)
+    lib().orc_char_update(side, cs_, _d(a), _d(out))
     return out
 
 
