@@ -690,7 +690,8 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
     double* colv = win + WIN * WIN;
     double* colt = colv + WIN;
     double* stage = colt + WIN;  // [2][CH][WIN]
-    int* pairs = (int*)(stage + 2 * CH * WIN);
+    double* piv = stage + 2 * CH * WIN;  // [2] d_k and [2] 1/d_k, alternating steps
+    int* pairs = (int*)(piv + 4);
     for (int a = tid; a < bw; a += T)
         for (int b = 0; b <= a; ++b) pairs[a * (a + 1) / 2 + b] = (a << 8) | b;
     for (int e = tid; e < min(W, n) * W; e += T) {
@@ -719,6 +720,21 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
     store_chunk(0);
     load_chunk(1);
     __syncthreads();
+    // each thread's update pairs (a, b), fixed for the whole factorisation
+    constexpr int PP = (PAIRMAX + 255) / 256;
+    int myab[PP];
+#pragma unroll
+    for (int r = 0; r < PP; ++r) {
+        const int t = tid + r * T;
+        myab[r] = t < bw * (bw + 1) / 2 ? pairs[t] : 0;
+    }
+    // the pivot of step k and its reciprocal are taken at the end of step
+    // k-1 by the thread that gives the pivot its last update
+    if (tid == 0 && n > 0) {
+        piv[0] = win[0];
+        piv[2] = 1.0 / piv[0];
+    }
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
         const int c = k / CH;
         if (k > 0 && k % CH == 0) {
@@ -727,8 +743,8 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
         }
         const int m = min(k + bw, n - 1) - k;
         const int sk = (k % WIN) * WIN;
-        const double akk = win[sk];
-        const double rk = 1.0 / akk;
+        const double akk = piv[k & 1];
+        const double rk = piv[2 + (k & 1)];
         if (tid < m) {
             int i = k + 1 + tid;
             int e = (i % WIN) * WIN + (i - k);
@@ -741,12 +757,20 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
         lds_barrier();
         if (tid == 0 && !(akk > 0.0)) atomicOr(err, ERR_SOLVE);
         const int npairs = m * (m + 1) / 2;
-        for (int t = tid; t < npairs; t += T) {
-            int ab = pairs[t];
-            int a = ab >> 8, b = ab & 255;
-            int i = k + 1 + a, j = k + 1 + b;
-            int e = (i % WIN) * WIN + (i - j);
-            win[e] = win[e] - colv[a] * colt[b];
+#pragma unroll
+        for (int r = 0; r < PP; ++r) {
+            const int t = tid + r * T;
+            if (t >= npairs) break;
+            const int ab = myab[r];
+            const int a = ab >> 8, b = ab & 255;
+            const int i = k + 1 + a, j = k + 1 + b;
+            const int e = (i % WIN) * WIN + (i - j);
+            const double v = win[e] - colv[a] * colt[b];
+            win[e] = v;
+            if (t == 0) {  // a(k+1, k+1) is final: the next pivot
+                piv[(k + 1) & 1] = v;
+                piv[2 + ((k + 1) & 1)] = 1.0 / v;
+            }
         }
         // solve coefficients of step k: column k (final now) and row k (final
         // since step k-1)
@@ -1452,7 +1476,7 @@ int solve2_threads() { return 64 * (2 + SV_NPROD); }
 int solve_unit(int npad) { return 3 * 8 * (size_t)npad + 2 * 64 * 64 * 16 <= 160 * 1024 ? 64 : 32; }
 int solve_threads() { return 64 * (1 + SV_NPROD); }
 
-size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + 2 * WIN + 2 * CH * WIN) + sizeof(int) * PAIRMAX; }
+size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + 2 * WIN + 2 * CH * WIN + 4) + sizeof(int) * PAIRMAX; }
 size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
 
 // Spread delta_u = sum_k D(k,a) y(k) in ascending k (the fixed-order gather in
