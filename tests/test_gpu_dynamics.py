@@ -66,6 +66,28 @@ def test_explicit_update_matches_the_oracle(oracle_mod, lbmib_mod):
     assert np.array_equal(g.body_state(), o.body_state())
 
 
+def test_two_wings_move_together(oracle_mod, lbmib_mod):
+    """wing_count = 2: the total x force of both wings drives the body, and
+    update_body_extra_from_rel gives wing 2 the running sum of the offsets
+    (mode.f90:285-307), so both wings move with wing 1."""
+    import os
+    from conftest import GOLDEN
+    xy = cases.read_boundary(os.path.join(GOLDEN, "boundary.dat"), 205)
+    case = cases.tandem_reference_case(xy, n_steps=60)
+    dyn = cases.Dynamics(solid_den=DEN)
+    o = oracle_mod.Oracle(case, n_threads=8)
+    o.set_dynamics(dyn)
+    g = lbmib_mod.LBMIB(case, dynamics=dyn)
+    o.step(60)
+    g.evolution(60)
+    assert_same(o, g, "two wings")
+    st = g.body_state()
+    assert np.array_equal(st, o.body_state())
+    m = g.motion_rows()
+    assert np.array_equal(m, o.motion_rows())
+    assert np.array_equal(m[:, 2], m[:, 5]) and np.array_equal(m[:, 3], m[:, 6]) and np.abs(m[:, 3]).max() > 0.0
+
+
 def test_infinite_mass_is_the_prescribed_run(lbmib_mod):
     case = case0(40)
     a = lbmib_mod.LBMIB(case)
