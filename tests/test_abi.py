@@ -77,6 +77,19 @@ def test_params_struct_layout(lbmib_mod):
     assert C.sizeof(P) == 8 + 17 * 8 + 12 * 4
 
 
+def test_dynamics_struct_layout(lbmib_mod):
+    """The ctypes mirror of lbmib_dynamics has the header's field order and
+    size: 4 int32 fields, then 5 doubles."""
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct lbmib_dynamics"):text.index("} lbmib_dynamics;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"(?:int32_t|double)\s+([\w, ]+);", body)
+    names = [n.strip() for group in names for n in group.split(",")]
+    D = lbmib_mod.DynamicsParams
+    assert [f[0] for f in D._fields_] == names
+    assert D.solid_den.offset == 16 and C.sizeof(D) == 16 + 5 * 8
+
+
 def test_version_and_last_error(lbmib_mod):
     lib = lbmib_mod.load()
     assert lbmib_mod.version().startswith("lbmib")
