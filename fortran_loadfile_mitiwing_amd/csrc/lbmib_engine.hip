@@ -207,11 +207,17 @@ int engine_prepare(Engine* E)
         if (rc) return rc;
         E->case_cap = nc;
         E->graph_valid = false;
+        E->tab_bytes.clear();
     }
     int rc = 0;
-    std::vector<DevCase> tab(nc);
-    for (size_t c = 0; c < nc; ++c) tab[c] = E->members[c]->D;
-    HIP_TRY(hipMemcpy(E->dcases, tab.data(), sizeof(DevCase) * nc, hipMemcpyHostToDevice));
+    // the case table goes to the device only when it changed since the
+    // last window (byte copies, so that padding compares equal too)
+    std::vector<unsigned char> tab(sizeof(DevCase) * nc);
+    for (size_t c = 0; c < nc; ++c) std::memcpy(&tab[c * sizeof(DevCase)], &E->members[c]->D, sizeof(DevCase));
+    if (tab != E->tab_bytes) {
+        HIP_TRY(hipMemcpy(E->dcases, tab.data(), tab.size(), hipMemcpyHostToDevice));
+        E->tab_bytes.swap(tab);
+    }
     if (!E->tiles_valid) {
         std::vector<int2> t;
         build_tiles(E, t);

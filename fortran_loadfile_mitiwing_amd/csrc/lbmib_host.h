@@ -109,6 +109,7 @@ struct Engine {
     int ntiles = 0;
     bool tiles_valid = false;
     std::vector<CaseClock> hclock;
+    std::vector<unsigned char> tab_bytes;  // the case table as last uploaded
     // captured window (graph mode)
     hipGraphExec_t gexec = nullptr;
     int graph_len = 0;
