@@ -131,6 +131,17 @@ def test_null_arguments_are_rejected(lbmib_mod):
     assert lib.lbmib_create(None, None, None, C.byref(h)) == -1
     assert lib.lbmib_step(None, 1) == -1
     assert lib.lbmib_destroy(None) in (0, -1)
+    # post-processing and body dynamics: a null handle is an argument error, checked before any device call
+    buf = (C.c_double * 64)()
+    n = C.c_int32()
+    assert lib.lbmib_vorticity(None, 0, None, None) == -1
+    assert lib.lbmib_surface_pressure(None, buf, 64) == -1
+    assert lib.lbmib_set_dynamics(None, None) == -1
+    assert lib.lbmib_get_body_state(None, buf, 64) == -1
+    assert lib.lbmib_set_body_state(None, buf, 12) == -1
+    assert lib.lbmib_get_motion_rows(None, buf, 1, C.byref(n)) == -1
+    assert lib.lbmib_get_edge_f0(None, buf, buf) == -1
+    assert lib.lbmib_set_edge_f0(None, buf, buf) == -1
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the behaviour without a GPU")
