@@ -24,7 +24,7 @@ the sequential rate, and per-case C_L / C_D errors of the batch against the
 FP64 runs.
 
 --form precision: the precision form's workload (8192x4096, chord 512 lu,
-2048 markers, Re 2000, plunge 0.5c, St 0.3), run in the FP32 deviation form
+2048 markers, Re 2000, the step workload's plunge + pitch), run in the FP32 deviation form
 and in FP64 from the same start.  value = FP32 MLUPS over the whole run; the
 JSON holds the FP64 rate, their ratio, the relative L2 differences of
 rho / u / v after the last step and the C_L / C_D errors of the FP32 run.
@@ -65,7 +65,8 @@ WORKLOADS = {
     0: "400x200 lattice, NACA0012 chord 40 lu, 128 markers, Re 100, plunge 0.5c, St 0.3, U 0.05",
     1: "4096x2048 lattice, NACA0012 chord 256 lu, 1024 markers, Re 1000, plunge 0.5c + pitch 30deg "
        "(90deg lead), St 0.3, U 0.05",
-    2: "8192x4096 lattice, NACA0012 chord 512 lu, 2048 markers, Re 2000, plunge 0.5c, St 0.3, U 0.05",
+    2: "8192x4096 lattice, NACA0012 chord 512 lu, 2048 markers, Re 2000, plunge 0.5c + pitch 30deg "
+       "(90deg lead), St 0.3, U 0.05",
 }
 PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every phase of it
 # HBM bytes per k_lattice_f64 launch on the 4096x2048 workload, from

@@ -251,7 +251,7 @@ def config(index: int, n_steps: Optional[int] = None, scale_down: int = 1) -> Ca
     0: 400x200, chord 40, 128 markers, Re 100, plunge 0.5c, St 0.3, 4000 steps.
     1: 4096x2048, chord 256, 1024 markers, Re 1000, plunge 0.5c + pitch 30 deg
        leading by 90 deg, St 0.3, 20000 steps.
-    2: 8192x4096, chord 512, 2048 markers, Re 2000, plunge 0.5c, St 0.3, 50000 steps.
+    2: 8192x4096, chord 512, 2048 markers, Re 2000, the kinematics of 1, 50000 steps.
     U = 0.05 throughout.  ``scale_down`` divides lattice, chord and markers
     (keeping Re) for reduced test versions.
     """
@@ -264,7 +264,7 @@ def config(index: int, n_steps: Optional[int] = None, scale_down: int = 1) -> Ca
                       heave=0.5, pitch_deg=30.0, phase_deg=90.0, name="cfg1_4096x2048")
     elif index == 2:
         c = make_case(8192 // d, 4096 // d, 512.0 / d, 2048 // d, 2000.0, n_steps or 50000,
-                      heave=0.5, pitch_deg=0.0, name="cfg2_8192x4096")
+                      heave=0.5, pitch_deg=30.0, phase_deg=90.0, name="cfg2_8192x4096")
     else:
         raise ValueError("unknown config %d" % index)
     return c
