@@ -98,42 +98,121 @@ def test_characteristic_update_is_lodi(oracle_mod, side):
 
 @pytest.fixture(scope="module")
 def stepped(oracle_mod):
+    """The 400x200 workload halved, after 7 steps, with the populations of
+    step 6 kept for the pull."""
     case = cases.config(0, n_steps=10, scale_down=2)
     o = oracle_mod.Oracle(case)
-    o.step(7)
-    return case, o
+    o.step(6)
+    prev = o.fields()["f"].copy()
+    o.step(1)
+    return case, o, prev
+
+
+# D2Q9feq (D2Q9model.f90:36-46) with the REAL(4) weights of D2Q9model.f90:24-32,
+# evaluated in the source's order
+W32 = np.array([np.float32(4.0) / np.float32(9.0)] + [np.float32(1.0) / np.float32(9.0)] * 4
+               + [np.float32(1.0) / np.float32(36.0)] * 4, dtype=np.float64)
+
+
+def feq_ref(rh, u1, v1):
+    out = np.zeros(9)
+    for k in range(9):
+        eu = EX[k] * u1 + EY[k] * v1
+        utwo = u1 * u1 + v1 * v1
+        out[k] = W32[k] * rh * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * utwo)
+    return out
+
+
+def pulled(prev, i, j):
+    """streaming (solution.f90:438-454): f0(i,j,k) = f(i-e_kx, j-e_ky, k)."""
+    return np.array([prev[k, j - int(EY[k]), i - int(EX[k])] for k in range(9)])
+
+
+def test_stale_ring_f0_stays_the_initial_equilibrium(stepped):
+    """streaming covers i = 2..Nx-1, j = 2..Ny-1 only (solution.f90:441-447 with
+    sp = 2, ep = Nx-1 on one rank, 300-311), and init sets f0 = f =
+    D2Q9feq(1, UU, 0) everywhere (solution.f90:34-59): after any number of
+    steps the ring's f0 is that closed form, bit for bit."""
+    case, o, _ = stepped
+    f0 = o.fields()["f0"]
+    want = feq_ref(1.0, case.params.U, 0.0)[:, None]
+    ny, nx = f0.shape[1:]
+    for name, ring in (("column 1", f0[:, :, 0]), ("column Nx", f0[:, :, nx - 1]),
+                       ("row 1", f0[:, 0, :]), ("row Ny", f0[:, ny - 1, :])):
+        assert np.array_equal(ring, np.broadcast_to(want, ring.shape)), name
+    # and the interior f0 is this step's pull of the previous populations
+    for i, j in ((1, 1), (2, ny // 2), (nx - 2, ny - 2), (nx // 2, 1)):
+        assert np.array_equal(f0[:, j, i], pulled(stepped[2], i, j)), (i, j)
 
 
 def test_walls_take_the_neighbour_density_at_rest(stepped, oracle_mod):
-    case, o = stepped
+    """Walls (boundary_condition.f90:658-661, 785-788): rho of the row next to
+    the wall, u = v = 0, NEE from that row's streamed populations.  Every
+    column, the four corners included: a corner's neighbour is an edge-column
+    node, whose rho/u/v is the left/right characteristic result and whose f0
+    is the stale ring's closed-form equilibrium."""
+    case, o, prev = stepped
     fl = o.fields()
     s = case.params.s
-    rho, u, v, f, f0 = fl["rho"], fl["u"], fl["v"], fl["f"], fl["f0"]
+    rho, u, v, f = fl["rho"], fl["u"], fl["v"], fl["f"]
     ny, nx = rho.shape
+    ring_f0 = feq_ref(1.0, case.params.U, 0.0)
     for je, jn in ((0, 1), (ny - 1, ny - 2)):
-        assert np.array_equal(rho[je, 1:-1], rho[jn, 1:-1])
-        assert np.all(u[je, 1:-1] == 0.0) and np.all(v[je, 1:-1] == 0.0)
-        for i in (1, nx // 3, nx - 2):
-            want = oracle_mod.nee_node(s, (rho[jn, i], 0.0, 0.0), (rho[jn, i], u[jn, i], v[jn, i]), f0[:, jn, i])
+        assert np.array_equal(rho[je, :], rho[jn, :])
+        assert np.all(u[je, :] == 0.0) and np.all(v[je, :] == 0.0)
+        for i in (0, 1, nx // 3, nx - 2, nx - 1):
+            f0n = ring_f0 if i in (0, nx - 1) else pulled(prev, i, jn)
+            want = oracle_mod.nee_node(s, (rho[jn, i], 0.0, 0.0), (rho[jn, i], u[jn, i], v[jn, i]), f0n)
             assert np.array_equal(f[:, je, i], want), (je, i)
 
 
-def test_edge_columns_take_the_characteristic_state(stepped, oracle_mod):
-    case, o = stepped
+def test_corners_follow_the_characteristic_state_of_their_column(stepped, oracle_mod):
+    """The corner nodes (i = 1, Nx on both walls): the wall assignment reads
+    the edge column at rows 2 and Ny-1 after its characteristic update.  That
+    state is recomputed here from the closed-form stale f0 of the edge and
+    the next two columns' moments (LODI, pinned above); the corner's
+    populations must be the NEE of that state at rest."""
+    case, o, _ = stepped
     fl = o.fields()
     s = case.params.s
-    rho, u, v, f, f0 = fl["rho"], fl["u"], fl["v"], fl["f"], fl["f0"]
+    rho, u, v, f = fl["rho"], fl["u"], fl["v"], fl["f"]
     ny, nx = rho.shape
     c = oracle_mod.cs()
+    g = feq_ref(1.0, case.params.U, 0.0)
+    r0 = g.sum()
+    u0, v0 = (EX * g).sum() / r0, (EY * g).sum() / r0
+    for je, jn in ((0, 1), (ny - 1, ny - 2)):
+        for side, ie, i1, i2 in ((0, 0, 1, 2), (1, nx - 1, nx - 2, nx - 3)):
+            st = [(r0, u0, v0), (rho[jn, i1], u[jn, i1], v[jn, i1]), (rho[jn, i2], u[jn, i2], v[jn, i2])]
+            got = np.array([rho[jn, ie], u[jn, ie], v[jn, ie]])
+            assert np.allclose(got, oracle_mod.char_update(side, c, st), rtol=1e-13, atol=1e-16)
+            want_lodi = lodi(side, c, st)
+            assert np.all(np.abs(got - want_lodi) <= 1e-2 * np.abs(want_lodi - np.array(st[0])) + 1e-15)
+            want = oracle_mod.nee_node(s, (rho[jn, ie], 0.0, 0.0), (rho[jn, ie], u[jn, ie], v[jn, ie]), g)
+            assert np.array_equal(f[:, je, ie], want), (je, ie)
+            assert rho[je, ie] == rho[jn, ie] and u[je, ie] == 0.0 and v[je, ie] == 0.0
+
+
+def test_edge_columns_take_the_characteristic_state(stepped, oracle_mod):
+    """Edge columns (boundary_condition.f90:413-569): the characteristic
+    update of the edge's own state -- the moments of its stale f0, which is
+    the closed-form initial equilibrium -- from the next two columns, then
+    the NEE from the neighbour column's streamed populations (the pull of
+    the previous step, recomputed here)."""
+    case, o, prev = stepped
+    fl = o.fields()
+    s = case.params.s
+    rho, u, v, f = fl["rho"], fl["u"], fl["v"], fl["f"]
+    ny, nx = rho.shape
+    c = oracle_mod.cs()
+    g = feq_ref(1.0, case.params.U, 0.0)
+    r0 = g.sum()
+    u0, v0 = (EX * g).sum() / r0, (EY * g).sum() / r0
     for side, ie, i1, i2 in ((0, 0, 1, 2), (1, nx - 1, nx - 2, nx - 3)):
-        for j in (1, ny // 2, ny - 2):
-            # the edge's own state before the update: moments of its stale f0
-            g = f0[:, j, ie]
-            r0 = g.sum()
-            u0, v0 = (EX * g).sum() / r0, (EY * g).sum() / r0
+        for j in (1, 2, ny // 2, ny - 3, ny - 2):
             st = [(r0, u0, v0), (rho[j, i1], u[j, i1], v[j, i1]), (rho[j, i2], u[j, i2], v[j, i2])]
             assert np.allclose([rho[j, ie], u[j, ie], v[j, ie]], oracle_mod.char_update(side, c, st),
                                rtol=1e-13, atol=1e-16)
             want = oracle_mod.nee_node(s, (rho[j, ie], u[j, ie], v[j, ie]), (rho[j, i1], u[j, i1], v[j, i1]),
-                                       f0[:, j, i1])
+                                       pulled(prev, i1, j))
             assert np.array_equal(f[:, j, ie], want), (side, j)
