@@ -57,7 +57,8 @@ def build_fortran(verbose: bool = False) -> str:
     out = os.path.join(HERE, "lbmib_driver")
     bdir = os.path.join(HERE, "_fbuild")
     os.makedirs(bdir, exist_ok=True)
-    srcs = [os.path.join(FSRC, "lbmib_mod.f90"), os.path.join(FSRC, "lbmib_driver.f90")]
+    srcs = [os.path.join(FSRC, "lbmib_mod.f90"), os.path.join(FSRC, "lbmib_restart.f90"),
+            os.path.join(FSRC, "lbmib_driver.f90")]
     _run([flang, "-O2", "-module-dir", bdir, "-o", out + ".tmp"] + srcs +
          ["-L" + HERE, "-llbmib", "-Wl,-rpath,$ORIGIN"], cwd=bdir)
     os.replace(out + ".tmp", out)

@@ -306,6 +306,31 @@ def tandem_reference_case(boundary_xy: Tuple[np.ndarray, np.ndarray], nx: int = 
 SWEEP_SEED = 20261203
 
 
+# Stability rule of the drawn cases: a draw is accepted only when the BGK
+# relaxation time tau = 3 nu + 1/2 is at least TAU_MIN and the fastest marker
+# moves at no more than MACH_MAX times the lattice sound speed 1/sqrt(3).
+TAU_MIN = 0.51
+MACH_MAX = 0.2
+
+
+def peak_marker_mach(xb: np.ndarray, yb: np.ndarray, U: float, heave: float, pitch_deg: float,
+                     phase_deg: float, freq: float, samples: int = 720) -> float:
+    """Largest speed of any polygon point over one period of the motion of
+    :func:`sinusoid_motion` (heave ``heave`` chords, pitch ``pitch_deg``
+    about the origin leading by ``phase_deg``, frequency ``freq`` per chord/U),
+    in lattice units (U times the chord-unit velocity) over the lattice sound
+    speed 1/sqrt(3)."""
+    t = np.linspace(0.0, 1.0 / freq, samples, endpoint=False)[:, None]
+    w, th0, ph = 2.0 * math.pi * freq, math.radians(pitch_deg), math.radians(phase_deg)
+    th = th0 * np.sin(w * t + ph)
+    thd = th0 * w * np.cos(w * t + ph)
+    hd = heave * w * np.cos(w * t)
+    x, y = np.asarray(xb)[None, :], np.asarray(yb)[None, :]
+    vx = thd * (-np.sin(th) * x + np.cos(th) * y)
+    vy = thd * (-np.cos(th) * x - np.sin(th) * y) + hd
+    return float(U * np.sqrt(vx * vx + vy * vy).max() * math.sqrt(3.0))
+
+
 def draw_case(rng: np.random.Generator, n_steps: int, nx_max: int = 2048, U: float = 0.05,
               name: str = "") -> Case:
     """One parameter-sweep case drawn from ``rng``:
@@ -318,14 +343,29 @@ def draw_case(rng: np.random.Generator, n_steps: int, nx_max: int = 2048, U: flo
       uniform in [0, 180] deg;
     * Strouhal number uniform in [0.1, 0.5], defined on twice the heave
       amplitude: f = St U / (2 h0).
+
+    Draws that break the stability rule (tau >= TAU_MIN and peak marker Mach
+    number <= MACH_MAX) are discarded and drawn again, all six parameters
+    together, so the accepted cases are the uniform draws conditioned on the
+    rule.  Small chords at high Re (tau near 1/2) and high Strouhal numbers
+    with large pitch (trailing-edge speeds near 0.25 c_s) are the draws it
+    removes.
     """
-    nx = int(rng.choice(np.arange(512, nx_max + 1, 128)))
-    Re = float(rng.uniform(100.0, 1000.0))
-    heave = float(rng.uniform(0.25, 1.0))
-    pitch = float(rng.uniform(0.0, 45.0))
-    phase = float(rng.uniform(0.0, 180.0))
-    st = float(rng.uniform(0.1, 0.5))
-    chord = nx / 16.0
+    while True:
+        nx = int(rng.choice(np.arange(512, nx_max + 1, 128)))
+        Re = float(rng.uniform(100.0, 1000.0))
+        heave = float(rng.uniform(0.25, 1.0))
+        pitch = float(rng.uniform(0.0, 45.0))
+        phase = float(rng.uniform(0.0, 180.0))
+        st = float(rng.uniform(0.1, 0.5))
+        chord = nx / 16.0
+        tau = 3.0 * U * chord / Re + 0.5
+        if tau < TAU_MIN:
+            continue
+        xb, yb = naca4_polygon(int(4 * chord))
+        if peak_marker_mach(xb, yb, U, heave, pitch, phase, st / (2.0 * heave)) > MACH_MAX:
+            continue
+        break
     return make_case(nx, nx // 2, chord, int(4 * chord), Re, n_steps, U=U, heave=heave, pitch_deg=pitch,
                      phase_deg=phase, strouhal=st,
                      name=name or "sweep_%dx%d_Re%.0f_h%.2f_p%.0f_ph%.0f_St%.2f" % (nx, nx // 2, Re, heave, pitch,

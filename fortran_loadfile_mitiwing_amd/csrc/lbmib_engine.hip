@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace lbmib {
 
@@ -38,14 +39,27 @@ __global__ void k_gather_err(const DevCase* __restrict__ C, int nc, int* __restr
 
 namespace {
 
+// Stream sets are shared: one set of NSTREAM streams per (device, reserved
+// CU count), made on first use and kept until the process ends.  A stream
+// made with a CU mask takes a hardware queue of its own, so engines that
+// made and destroyed their own sets (a process stepping many single cases,
+// the autotuner's trials) would create and release dozens of such queues;
+// with shared sets a process holds at most one set per CU split it uses.
+// Engines of one process run one at a time on a set: every engine call ends
+// with its streams synchronised, so no wait of one engine is pending when
+// another engine's work follows on the same streams.
+struct StreamSet {
+    int dev = -1, cus = -1;
+    hipStream_t s[Engine::NSTREAM] = {};
+};
+std::mutex g_sets_mu;
+std::vector<StreamSet*> g_sets;
+
 void destroy_streams(Engine* E)
 {
     for (hipStream_t& st : E->s) {
-        if (st) {
-            (void)hipStreamSynchronize(st);
-            (void)hipStreamDestroy(st);
-            st = nullptr;
-        }
+        if (st) (void)hipStreamSynchronize(st);
+        st = nullptr;
     }
     E->streams_cus = -1;
 }
@@ -61,19 +75,39 @@ int make_streams(Engine* E, int ib_cus)
     destroy_streams(E);
     ib_cus = std::max(0, ib_cus) / 8 * 8;
     const int ncu = E->ncu;
-    if (ib_cus > 0 && ib_cus < ncu / 2 && ncu % 8 == 0) {
-        const int nwords = (ncu + 31) / 32;
-        std::vector<uint32_t> lat(nwords, 0u), ib(nwords, 0u);
-        for (int c = 0; c < ncu; ++c) {
-            if (c >= ncu - ib_cus) ib[c / 32] |= 1u << (c % 32);
-            else lat[c / 32] |= 1u << (c % 32);
+    const bool split = ib_cus > 0 && ib_cus < ncu / 2 && ncu % 8 == 0;
+    const int key = split ? ib_cus : 0;
+    std::lock_guard<std::mutex> lock(g_sets_mu);
+    StreamSet* set = nullptr;
+    for (StreamSet* q : g_sets)
+        if (q->dev == E->dev && q->cus == key) set = q;
+    if (!set) {
+        set = new StreamSet();
+        set->dev = E->dev;
+        set->cus = key;
+        hipError_t e = hipSuccess;
+        if (split) {
+            const int nwords = (ncu + 31) / 32;
+            std::vector<uint32_t> lat(nwords, 0u), ib(nwords, 0u);
+            for (int c = 0; c < ncu; ++c) {
+                if (c >= ncu - ib_cus) ib[c / 32] |= 1u << (c % 32);
+                else lat[c / 32] |= 1u << (c % 32);
+            }
+            for (int i = 0; i < Engine::NSTREAM && e == hipSuccess; ++i)
+                e = hipExtStreamCreateWithCUMask(&set->s[i], nwords, i == 1 ? ib.data() : lat.data());
+        } else {
+            for (int i = 0; i < Engine::NSTREAM && e == hipSuccess; ++i)
+                e = hipStreamCreateWithFlags(&set->s[i], hipStreamNonBlocking);
         }
-        for (int i = 0; i < Engine::NSTREAM; ++i)
-            HIP_TRY(hipExtStreamCreateWithCUMask(&E->s[i], nwords, i == 1 ? ib.data() : lat.data()));
-    } else {
-        for (int i = 0; i < Engine::NSTREAM; ++i) HIP_TRY(hipStreamCreateWithFlags(&E->s[i], hipStreamNonBlocking));
-        ib_cus = 0;
+        if (e != hipSuccess) {
+            for (hipStream_t st : set->s)
+                if (st) (void)hipStreamDestroy(st);
+            delete set;
+            return fail(LBMIB_E_HIP, "stream creation failed: %s", hipGetErrorString(e));
+        }
+        g_sets.push_back(set);
     }
+    for (int i = 0; i < Engine::NSTREAM; ++i) E->s[i] = set->s[i];
     E->streams_cus = std::max(0, E->shape.ib_cus) / 8 * 8;
     return 0;
 }
@@ -670,7 +704,17 @@ int engine_run(Engine* E, int k)
             fresh = fresh || E->members[c]->geo_fresh;
         }
         if (kwin <= 0) break;
-        const int phase = E->members[0]->steps_done % B;
+        // the batch phase is that of the members that still step (members
+        // whose motion table has ended idle and may stand anywhere)
+        int phase = -1;
+        for (size_t c = 0; c < nc; ++c) {
+            if (final_end[c] <= E->members[c]->steps_done) continue;
+            const int ph = E->members[c]->steps_done % B;
+            if (phase < 0) phase = ph;
+            else if (ph != phase)
+                return fail(LBMIB_E_STATE, "members stand at different positions inside a geometry batch (%d, %d)",
+                            phase, ph);
+        }
         const bool graph = E->shape.graph && phase == 0 && !fresh && kwin >= 2 * B;
         int len = kwin;
         if (E->shape.graph && !graph) {
@@ -808,11 +852,17 @@ int lbmib_batch_add(lbmib_batch* b, lbmib_handle* h)
     if (h->dev != b->dev) return fail(LBMIB_E_ARG, "the case lives on device %d, the batch on %d", h->dev, b->dev);
     Engine* E = b->E;
     if (!E->members.empty()) {
-        const lbmib_handle* m = E->members[0];
-        if (m->D.prec != h->D.prec) return fail(LBMIB_E_ARG, "all cases of a batch must use the same precision");
-        if (m->steps_done % E->B != h->steps_done % E->B)
-            return fail(LBMIB_E_STATE, "the case stands at step %d, the batch at %d (mod %d): cases join at the "
-                        "same position inside a geometry batch", h->steps_done, m->steps_done, E->B);
+        if (E->members[0]->D.prec != h->D.prec)
+            return fail(LBMIB_E_ARG, "all cases of a batch must use the same precision");
+        // the position inside the geometry batch is set by the members that
+        // still have steps to take; finished members idle wherever they stand
+        for (const lbmib_handle* m : E->members) {
+            if (m->steps_done >= m->max_steps) continue;
+            if (m->steps_done % E->B != h->steps_done % E->B)
+                return fail(LBMIB_E_STATE, "the case stands at step %d, the batch at %d (mod %d): cases join at the "
+                            "same position inside a geometry batch", h->steps_done, m->steps_done, E->B);
+            break;
+        }
     }
     if (h->D.batch != E->B) return fail(LBMIB_E_STATE, "geometry batch length differs");
     E->members.push_back(h);

@@ -5,13 +5,15 @@
 !   force.dat                       one '(1000f15.8)' row per step (solution.f90:349-376)
 !   airfoil_motion.dat              one '(1000f15.8)' row per step (solution.f90:378-390)
 !   F<n>.dat                        Tecplot flow field and airfoil zones (output, solution.f90:129-238)
-!   saved_flow_feild.dat            binary restart state (saveflowfield / readflowfield,
-!                                   solution.f90:1604-1839)
+!   saved_flow_feild.dat            binary restart state and its copy
+!   saved_flow_feild_copy.dat       (saveflowfield / readflowfield, solution.f90:1604-1839;
+!                                   lbmib_restart.f90)
 ! and takes the parameters that the reference sets in ini_mesh / main from a
 ! namelist file (first argument, default lbmib.nml).
 program lbmib_driver
     use iso_c_binding
     use lbmib
+    use lbmib_restart
     implicit none
 
     integer :: nx = 0, ny = 0, smooth = 1, wing_count = 1, points_per_wing = 0
@@ -341,90 +343,47 @@ contains
     ! f(1:9), then the edge records: the coarse-to-fine copies (the initial
     ! equilibrium for one mesh) and the stale f0 of the edge columns (from
     ! the device) and rows (the initial equilibrium: no step reads them, so
-    ! the device does not keep them).  Raw stream, as form='binary' writes it.
+    ! the device does not keep them).  The primary file, then its copy
+    ! (saved_flow_feild_copy.dat beside it, solution.f90:1677-1706).
     subroutine write_restart(cycle)
         integer, intent(in) :: cycle
-        integer :: i, j, o, nb
-        integer(c_int32_t) :: i4
+        integer :: nb
+        character(len=512) :: path
         call lbmib_check(lbmib_download(h, c_loc(rho), c_loc(u), c_loc(v), c_loc(f)), 'lbmib_download')
         nb = 11 * wing_count
         call lbmib_check(lbmib_get_body_state(h, body_state, int(nb + 1, c_int32_t)), 'lbmib_get_body_state')
         call lbmib_check(lbmib_get_edge_f0(h, c_loc(edge_first), c_loc(edge_last)), 'lbmib_get_edge_f0')
-        open(newunit=o, file=trim(output_dir) // '/' // trim(restart_out), status='replace', access='stream', &
-             form='unformatted', action='write')
-        i4 = cycle
-        write(o) i4
-        i4 = wing_count
-        write(o) i4
-        write(o) body_state(1:nb)
-        i4 = nint(body_state(nb + 1))
-        write(o) i4
-        do j = 1, ny
-            do i = 1, nx
-                write(o) u(i, j), v(i, j), f(i, j, :)
-            enddo
-        enddo
-        do j = 2, ny - 1
-            write(o) feq0, edge_first(j, :), feq0, edge_last(j, :)
-        enddo
-        do j = 1, nx
-            if (j .eq. 1) then
-                write(o) feq0, edge_first(ny, :), feq0, edge_first(1, :)
-            else if (j .eq. nx) then
-                write(o) feq0, edge_last(ny, :), feq0, edge_last(1, :)
-            else
-                write(o) feq0, feq0, feq0, feq0
-            endif
-        enddo
-        close(o)
+        path = trim(output_dir) // '/' // trim(restart_out)
+        call write_flow_field(trim(path), cycle, wing_count, body_state(1:nb), nint(body_state(nb + 1)), nx, ny, &
+                              u, v, f, feq0, edge_first, edge_last)
+        call write_flow_field(restart_copy_name(trim(path)), cycle, wing_count, body_state(1:nb), &
+                              nint(body_state(nb + 1)), nx, ny, u, v, f, feq0, edge_first, edge_last)
     end subroutine
 
     ! readflowfield (solution.f90:1709-1839): the step, the rigid-body state,
     ! the populations and the stale f0 of the edge columns; the velocity
-    ! fields are recomputed by the next step.
+    ! fields are recomputed by the next step.  A primary file that cannot be
+    ! opened or read to its end gives way to its copy (solution.f90:1737-1802).
     subroutine read_restart(fname, cycle)
         character(len=*), intent(in) :: fname
         integer, intent(out) :: cycle
-        integer :: i, j, o, rios, nb
-        integer(c_int32_t) :: i4, nwf
-        real(c_double) :: uu1, vv1, skip(9)
-        open(newunit=o, file=fname, status='old', access='stream', form='unformatted', action='read', iostat=rios)
-        if (rios .ne. 0) then
-            write(*, *) 'cannot open restart file: ', fname
-            stop 1
+        integer :: nb, st, nwf, mv
+        nb = 11 * wing_count
+        call read_flow_field(fname, cycle, wing_count, body_state(1:nb), mv, nx, ny, f, edge_first, edge_last, &
+                             st, nwf)
+        if (st .eq. RESTART_UNREADABLE) then
+            write(*, *) 'restart file unreadable, reading its copy: ', restart_copy_name(fname)
+            call read_flow_field(restart_copy_name(fname), cycle, wing_count, body_state(1:nb), mv, nx, ny, f, &
+                                 edge_first, edge_last, st, nwf)
+            if (st .eq. RESTART_UNREADABLE) then
+                write(*, *) 'cannot read the restart file or its copy: ', fname
+                stop 1
+            endif
         endif
-        read(o) i4
-        cycle = i4
-        read(o) nwf
-        if (nwf .ne. wing_count) then
+        if (st .eq. RESTART_WING_COUNT) then
             write(*, *) 'restart file wing count ', nwf, ' differs from wing_count ', wing_count
             stop 1
         endif
-        nb = 11 * wing_count
-        read(o) body_state(1:nb)
-        read(o) i4
-        body_state(nb + 1) = dble(i4)
-        do j = 1, ny
-            do i = 1, nx
-                read(o, iostat=rios) uu1, vv1, f(i, j, :)
-                if (rios .ne. 0) then
-                    write(*, *) 'restart file too short for a ', nx, ' x ', ny, ' lattice: ', fname
-                    stop 1
-                endif
-            enddo
-        enddo
-        do j = 2, ny - 1
-            read(o) skip, edge_first(j, :), skip, edge_last(j, :)
-        enddo
-        do j = 1, nx
-            if (j .eq. 1) then
-                read(o) skip, edge_first(ny, :), skip, edge_first(1, :)
-            else if (j .eq. nx) then
-                read(o) skip, edge_last(ny, :), skip, edge_last(1, :)
-            else
-                read(o) skip, skip, skip, skip
-            endif
-        enddo
-        close(o)
+        body_state(nb + 1) = dble(mv)
     end subroutine
 end program lbmib_driver

@@ -94,3 +94,26 @@ def test_standard_workloads(index, nx, ny, markers, chord):
     # the leading edge sits on the stated lattice node at zero pitch
     le_x = (np.min(c.xb0) - p.ox) / p.scale
     assert le_x == pytest.approx(nx / 4 if index else 100, abs=1e-5)
+
+
+def test_drawn_cases_keep_the_stability_rule():
+    """Every draw of the sweep and of the serving workload has tau >= 0.51 and
+    a peak marker Mach number <= 0.2; the rule is computed independently here
+    from the motion table the case carries (finite differences of the
+    polygon's positions over the table's rows)."""
+    cs = cases.sweep_cases(96, n_steps=4000)
+    reqs = cases.serving_requests(40)
+    for c in cs + [r.case for r in reqs[:10]]:
+        p = c.params
+        tau = 1.0 / p.s[7]
+        assert tau >= cases.TAU_MIN - 1e-12, c.name
+        m = c.motion
+        xb, yb = np.asarray(c.xb0), np.asarray(c.yb0)
+        ang, sa, sb = m[:, 1:2], m[:, 4:5], m[:, 7:8]
+        X = np.cos(ang) * xb + np.sin(ang) * yb + sa
+        Y = -np.sin(ang) * xb + np.cos(ang) * yb + sb
+        dt = p.scale * p.U
+        v = np.hypot(np.diff(X, axis=0), np.diff(Y, axis=0)) / dt  # chord units per chord/U
+        assert p.U * v.max() * np.sqrt(3.0) <= cases.MACH_MAX * 1.01, c.name
+    # the rule removes draws: the unconditioned corner c = 32 at Re 1000 has tau 0.5048
+    assert 3 * 0.05 * 32 / 1000.0 + 0.5 < cases.TAU_MIN

@@ -110,6 +110,16 @@ def test_driver_restart_file_continues_the_run(tmp_path):
     ra = open(os.path.join(d, "force_a.dat")).read().splitlines()
     rb = open(os.path.join(d, "force_b.dat")).read().splitlines()
     assert len(ra) == 30 and rb == ra[20:]
+    # saveflowfield's copy beside the primary (solution.f90:1677-1706); a
+    # primary cut short gives way to it (solution.f90:1737-1802)
+    assert open(os.path.join(d, "saved_flow_feild_copy.dat"), "rb").read() == raw
+    with open(os.path.join(d, "saved_flow_feild.dat"), "wb") as fh:
+        fh.write(raw[:len(raw) // 2])
+    write_inputs(d, case, nml="c.nml", restart_in="saved_flow_feild.dat", restart_out="unused.dat",
+                 force_file="force_c.dat", fields="c.bin")
+    out = run_driver(exe, d, "c.nml")
+    assert "reading its copy" in out and "resumed after step 20" in out
+    assert np.array_equal(np.fromfile(os.path.join(d, "c.bin")), fa)
 
 
 @pytest.mark.gpu

@@ -122,6 +122,37 @@ def test_members_with_shorter_tables_idle(lbmib_mod):
     same(sims[0], alone(lbmib_mod, cs[0], 32, prec))
 
 
+def test_finished_first_member_does_not_set_the_batch_phase(lbmib_mod):
+    """The batch's position inside its geometry batch comes from the members
+    that still step: here the first member's table ends at step 20, so after
+    the first call it idles at 20 (4 steps into a geometry batch) while the
+    others stand at 32 (a batch boundary).  Every member must still equal the
+    same case run alone, over a second call of 28 steps."""
+    short = cases.make_case(160, 80, 16.0, 48, 300.0, 20, heave=0.4, name="short_first")
+    cs = mixed_cases(80)[:2]
+    prec = 1
+    sims = [lbmib_mod.LBMIB(short, precision=prec)] + [lbmib_mod.LBMIB(c, precision=prec) for c in cs]
+    b = lbmib_mod.Batch()
+    for s in sims:
+        b.add(s)
+    b.step(32)
+    b.step(28)
+    assert [s.steps_done for s in sims] == [20, 60, 60]
+    same(sims[0], alone(lbmib_mod, short, 20, prec))
+    for c, s in zip(cs, sims[1:]):
+        same(s, alone(lbmib_mod, c, 60, prec))
+    # a joining case must match the stepping members' position (60 mod 16 =
+    # 12), not the idle member's (20 mod 16 = 4)
+    late = lbmib_mod.LBMIB(cases.config(0, n_steps=40, scale_down=4), precision=prec)
+    late.evolution(4)
+    with pytest.raises(lbmib_mod.LBMIBError) as e:
+        b.add(late)
+    assert e.value.code == -6
+    late.evolution(8)
+    b.add(late)
+    b.remove(late)
+
+
 GRAPH_OFF = ("graph replay is switched off: its first GPU run (this three-case batch replaying captured "
              "16-step windows) ended the host process with a segmentation fault of unknown cause")
 SHAPES = [
