@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,8 @@ int geometric_bandwidth(const lbmib_handle* h)
 
 }  // namespace
 
+static int scratch_on(int dev, double** out);  // staging buffer of lbmib_surface_pressure_noalloc
+
 extern "C" {
 
 const char* lbmib_last_error(void) { return g_err.c_str(); }
@@ -260,6 +263,10 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
         }                                                                         \
     } while (0)
     HTRY(hipSetDevice(h->dev));
+    {
+        double* sbuf = nullptr;  // the device's staging buffer, recorded once per device
+        TRY(scratch_on(h->dev, &sbuf));
+    }
     TRY(engine_create(h->dev, &h->own));
     h->own->members.push_back(h);
     h->in_xb0.assign(xb0, xb0 + (size_t)p->n_wings * p->pts_per_wing);
@@ -1100,6 +1107,65 @@ int lbmib_surface_pressure(lbmib_handle* h, double* p, int32_t n)
                        h->surf);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(p, h->surf, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"
+
+// Staging buffer of lbmib_surface_pressure_noalloc: one value per marker
+// segment, LBMIB_SCRATCH_SEGMENTS of them (16 KiB), in the code object's
+// global memory, so each device holds its own copy from the library's first
+// use on it until the process ends.  Host code reaches a device's copy only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name is a host-side stand-in); lbmib_create records it.
+static __device__ __align__(256) double scratch[LBMIB_SCRATCH_SEGMENTS];
+static std::mutex g_scratch_mu;
+static constexpr int SCRATCH_DEVS = 64;
+static double* g_scratch_ptr[SCRATCH_DEVS] = {};
+
+// The device copy of the staging buffer on `dev`, looked up once per device
+// with that device current.
+static int scratch_on(int dev, double** out)
+{
+    *out = nullptr;
+    if (dev < 0 || dev >= SCRATCH_DEVS) return fail(LBMIB_E_ARG, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        HIP_TRY(hipSetDevice(dev));
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(LBMIB_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" {
+
+int lbmib_surface_pressure_noalloc(lbmib_handle* h, double* p, int32_t n)
+{
+    if (!h || !p) return fail(LBMIB_E_ARG, "null argument");
+    const int ns = h->D.nseg;
+    if (ns > LBMIB_SCRATCH_SEGMENTS)
+        return fail(LBMIB_E_ARG, "the case has %d marker segments; the allocation-free call stages at most %d", ns,
+                    LBMIB_SCRATCH_SEGMENTS);
+    if (h->steps_done < 1 || h->geo_fresh) return fail(LBMIB_E_STATE, "no step taken yet");
+    if (h->p.smooth != 1)
+        return fail(LBMIB_E_ARG, "cal_sur_cp interpolates with the smoothed kernel (smooth = 1), the case uses smooth = 0");
+    if (n < ns) return fail(LBMIB_E_ARG, "room for %d markers needed (got %d)", ns, n);
+    double* buf = nullptr;
+    if (int rc = scratch_on(h->dev, &buf)) return rc;
+    HIP_TRY(hipSetDevice(h->dev));
+    HIP_TRY(hipDeviceSynchronize());
+    // every one of the ns values is written by the kernel: nothing to clear
+    const Slot S = slot_of(h->D, (h->steps_done - 1) % h->D.nslot);
+    hipLaunchKernelGGL(k_surface_pressure, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, nullptr,
+                       (const int*)S.dcnt, (const int*)S.didx, (const double*)S.dval, (const double*)S.rhoadj, ns,
+                       buf);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(p, buf, sizeof(double) * ns, hipMemcpyDeviceToHost));
     return 0;
 }
 

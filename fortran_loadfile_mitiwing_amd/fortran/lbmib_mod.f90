@@ -59,7 +59,7 @@ module lbmib
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
     public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings, lbmib_set_step
-    public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure
+    public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure, lbmib_surface_pressure_noalloc
     ! rigid-body dynamics
     public :: lbmib_dynamics, lbmib_set_dynamics, lbmib_get_body_state, lbmib_set_body_state
     public :: lbmib_get_motion_rows, lbmib_get_edge_f0, lbmib_set_edge_f0
@@ -181,6 +181,15 @@ module lbmib
         end function
 
         integer(c_int) function lbmib_surface_pressure(h, p, n) bind(C, name="lbmib_surface_pressure")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: p(*)
+            integer(c_int32_t), value :: n
+        end function
+
+        ! no counterpart in the reference: the same, staged through the
+        ! library's fixed per-device buffer (for callers that cannot allocate)
+        integer(c_int) function lbmib_surface_pressure_noalloc(h, p, n) bind(C, name="lbmib_surface_pressure_noalloc")
             import :: c_int, c_int32_t, c_double, c_ptr
             type(c_ptr), value :: h
             real(c_double), intent(out) :: p(*)

@@ -91,7 +91,8 @@ _lib = None
 
 EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_done",
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
-           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure", "lbmib_set_dynamics", "lbmib_get_body_state",
+           "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure",
+           "lbmib_surface_pressure_noalloc", "lbmib_set_dynamics", "lbmib_get_body_state",
            "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_get_edge_f0", "lbmib_set_edge_f0", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
@@ -128,6 +129,7 @@ def load(path: str = LIB_PATH):
     L.lbmib_set_edge_f0.argtypes = [vp, _dp, _dp]
     L.lbmib_vorticity.argtypes = [vp, C.c_int32, _dp, _dp]
     L.lbmib_surface_pressure.argtypes = [vp, _dp, C.c_int32]
+    L.lbmib_surface_pressure_noalloc.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_residual.argtypes = [vp, _dp]
     L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
@@ -328,11 +330,14 @@ class LBMIB:
                                       _d(dv) if dv is not None else None))
         return vor, dv
 
-    def surface_pressure(self) -> np.ndarray:
-        """cal_sur_cp: pressure at each marker midpoint of the last step."""
+    def surface_pressure(self, noalloc: bool = False) -> np.ndarray:
+        """cal_sur_cp: pressure at each marker midpoint of the last step.
+        With ``noalloc`` through lbmib_surface_pressure_noalloc, which stages
+        the values in the library's fixed per-device buffer."""
         n = self.p.n_wings * (self.p.pts_per_wing - 1)
         out = np.zeros(n)
-        _check(load().lbmib_surface_pressure(self.h, _d(out), n))
+        fn = load().lbmib_surface_pressure_noalloc if noalloc else load().lbmib_surface_pressure
+        _check(fn(self.h, _d(out), n))
         return out
 
     def ib_nodes(self):

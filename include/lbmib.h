@@ -216,6 +216,17 @@ int lbmib_vorticity(lbmib_handle* h, int32_t rate, double* vor, double* dvordt);
  * count, n_wings * (pts_per_wing - 1)).  Needs smooth = 1. */
 int lbmib_surface_pressure(lbmib_handle* h, double* p, int32_t n);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lbmib_surface_pressure with the same arguments and
+ * results, staged through a fixed per-device buffer of the library instead
+ * of a per-handle one allocated on first use; it calls no allocating
+ * routine.  At most LBMIB_SCRATCH_SEGMENTS marker segments (LBMIB_E_ARG
+ * beyond).  The buffer is shared by every call on one device: calls that
+ * overlap on one device race. */
+#define LBMIB_SCRATCH_SEGMENTS 2048
+int lbmib_surface_pressure_noalloc(lbmib_handle* h, double* p, int32_t n);
+
 /* Device timing of the hot path (the reference's cpu_time table,
  * main.f90:338-369, 420-433), measured with HIP events on the streams the
  * kernels run on.  lbmib_set_profiling(h, every) resets the counters and,
