@@ -488,6 +488,128 @@ def bench_post(args):
     print(json.dumps(out))
 
 
+def precision_summary(steps=200):
+    """FP32 deviation form against FP64 on the 8192x4096 workload over a fixed
+    window from rest: MLUPS of both (whole steps, after one geometry batch of
+    warm-up), their ratio, and the FP32 run's C_L / C_D and field errors."""
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    warm = 16
+    case = cases.config(2, n_steps=warm + steps + 8)
+    p = case.params
+    runs = {}
+    for name, prec in (("fp32", lbmib.PREC_FP32DEV), ("fp64", lbmib.PREC_FP64)):
+        g = lbmib.LBMIB(case, device=0, precision=prec)
+        g.evolution(warm)
+        t0 = time.perf_counter()
+        g.evolution(steps)
+        dt = time.perf_counter() - t0
+        runs[name] = (p.nx * p.ny * steps / dt / 1e6, g.history(), g.fields(with_f=False))
+        g.close()
+        print("[summary] precision %s %.0f MLUPS" % (name, runs[name][0]), file=sys.stderr, flush=True)
+    (m32, h32, f32), (m64, h64, f64) = runs["fp32"], runs["fp64"]
+    ce = coefficient_errors(h32, h64)
+    return {"workload": "%s, %d timed steps after %d" % (WORKLOADS[2], steps, warm),
+            "fp32_MLUPS": round(m32, 1), "fp64_MLUPS": round(m64, 1), "fp32_over_fp64": round(m32 / m64, 3),
+            "CL_CD_errors_last_half": {k: float("%.3g" % v) for k, v in ce.items() if not k.startswith("_")},
+            "field_rel_L2": {k: float("%.3g" % (np.linalg.norm(f32[k] - f64[k]) / np.linalg.norm(f64[k])))
+                             for k in ("rho", "u", "v")},
+            "notes": "mean C_L / C_D relative to their r.m.s. over the last half of the run, mean_CD_rel to "
+                     "|mean C_D|, peak |C_L| relative"}
+
+
+def sweep_summary(n_cases=96, steps=400):
+    """The seeded sweep's cases cut to `steps` steps: all of them as one
+    batched FP32-deviation run, then each alone as a fresh plain FP64 handle
+    (the sequential baseline, measured, not projected; it also gives the FP64
+    histories the batch is checked against).  Times include creating the
+    cases.  A case whose history is not finite counts as failed, not as
+    throughput."""
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    cs = cases.sweep_cases(n_cases, n_steps=steps)
+    nodes = sum(c.params.nx * c.params.ny for c in cs)
+    t0 = time.perf_counter()
+    sims = [lbmib.LBMIB(c, precision=lbmib.PREC_FP32DEV) for c in cs]
+    b = lbmib.Batch()
+    b.set_shape(ib_collide_shared=1)
+    for s_ in sims:
+        b.add(s_)
+    b.step(steps)
+    t_batch = time.perf_counter() - t0
+    h32 = [s_.history() for s_ in sims]
+    b.close()
+    for s_ in sims:
+        s_.close()
+    print("[summary] sweep batch %.2f s" % t_batch, file=sys.stderr, flush=True)
+    t_seq, h64 = 0.0, []
+    for i, c in enumerate(cs):
+        t1 = time.perf_counter()
+        g = lbmib.LBMIB(c)
+        g.evolution(steps)
+        h64.append(g.history())
+        g.close()
+        t_seq += time.perf_counter() - t1
+        if i % 32 == 31:
+            print("[summary] sweep sequential %d cases %.2f s" % (i + 1, t_seq), file=sys.stderr, flush=True)
+    ok = [i for i in range(len(cs)) if np.all(np.isfinite(h32[i][:, 1:4])) and np.all(np.isfinite(h64[i][:, 1:4]))]
+    errs = [coefficient_errors(h32[i], h64[i]) for i in ok]
+    keys = ("mean_CD", "mean_CL", "peak_CL")
+    within = [i for i, e in zip(ok, errs) if all(e[k] <= 0.005 for k in keys)]
+    return {"workload": "%d seeded sweep cases (nx 512..2048, tau >= %.2f, marker Mach <= %.1f), %d steps each"
+                        % (len(cs), cases.TAU_MIN, cases.MACH_MAX, steps),
+            "lattice_nodes": nodes,
+            "batched_fp32_cases_per_hr": round(len(within) / t_batch * 3600.0, 1),
+            "sequential_fp64_cases_per_hr": round(len(cs) / t_seq * 3600.0, 1),
+            "speedup": round(t_seq / t_batch * len(within) / len(cs), 3),
+            "batched_aggregate_MLUPS": round(nodes * steps / t_batch / 1e6, 1),
+            "cases_failed": len(cs) - len(ok), "cases_within_0.5pct": len(within),
+            "worst_CL_CD_errors": {k: float("%.3g" % max(e[k] for e in errs)) for k in keys + ("mean_CD_rel",)},
+            "notes": "batched cases/hr counts only cases that stay finite and meet the 0.5% bar against their "
+                     "FP64 run; times include creating the cases; errors over the last half of each run"}
+
+
+def serve_summary(n_req=48, load=0.8, window=32):
+    """A short serving run: requests of the serving distribution with step
+    counts cut tenfold (200..2000, log-uniform), Poisson arrivals offering
+    `load` of the capacity a 16-case probe batch measures, continuous
+    batching in the FP32 deviation form; p50/p99 time-to-result."""
+    from fortran_loadfile_mitiwing_amd import cases, lbmib
+    from fortran_loadfile_mitiwing_amd.serve import GPUBackend, Server
+    probe = [r.case for r in cases.serving_requests(16, rate_per_s=1.0, seed=cases.SWEEP_SEED + 2,
+                                                    steps_lo=200, steps_hi=2000)]
+    sims = [lbmib.LBMIB(c, precision=lbmib.PREC_FP32DEV) for c in probe]
+    b = lbmib.Batch()
+    b.set_shape(ib_collide_shared=1)
+    for s_ in sims:
+        b.add(s_)
+    b.step(16)
+    t0 = time.perf_counter()
+    b.step(64)
+    cap = sum(c.params.nx * c.params.ny for c in probe) * 64 / (time.perf_counter() - t0) / 1e6
+    b.close()
+    for s_ in sims:
+        s_.close()
+    reqs = cases.serving_requests(n_req, rate_per_s=1.0, steps_lo=200, steps_hi=2000)
+    work = np.array([r.case.params.nx * r.case.params.ny * r.case.n_steps for r in reqs], dtype=np.float64)
+    rate = load * cap * 1e6 / work.mean()
+    for r in reqs:
+        r.arrival = r.arrival / rate
+    free, _ = lbmib.mem_info(0)
+    srv = Server(GPUBackend(precision=lbmib.PREC_FP32DEV, shape={"ib_collide_shared": 1}),
+                 hbm_budget=int(free * 0.85), window=window)
+    srv.run(reqs)
+    st = srv.stats()
+    srv.close()
+    print("[summary] serve p50 %.3f s p99 %.3f s" % (st["p50_time_to_result_s"], st["p99_time_to_result_s"]),
+          file=sys.stderr, flush=True)
+    return {"workload": "%d requests (nx 512..4096, 200..2000 steps), Poisson arrivals at %.2f /s = %.0f%% of the "
+                        "%.0f MLUPS capacity, windows of %d steps" % (n_req, rate, 100 * load, cap, window),
+            "p50_time_to_result_s": round(st["p50_time_to_result_s"], 3),
+            "p99_time_to_result_s": round(st["p99_time_to_result_s"], 3),
+            "aggregate_MLUPS": round(st["aggregate_MLUPS"], 1),
+            "mean_running_cases": round(st["mean_running_cases"], 2),
+            "peak_running_cases": st["peak_running_cases"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -495,6 +617,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-summaries", action="store_true",
+                    help="step form: leave out the precision, sweep and serving summaries of the default line")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="fp64: the drop-in path (default); fp32: the FP32 deviation form")
     ap.add_argument("--shape", default="",
@@ -554,9 +678,9 @@ def main():
         g.set_shape(**parse_shape(args.shape))
     if W:
         g.evolution(W)
-    # phase events on every PROFILE_EVERY-th timed step: they time the lattice
-    # kernel live inside the timed region at well under 1% of its cost
-    g.set_profiling(PROFILE_EVERY)
+    # phase events inside the timed region: on every step of a short run
+    # (up to 64 steps), else on every PROFILE_EVERY-th step
+    g.set_profiling(1 if K <= 64 else PROFILE_EVERY)
     t0 = time.perf_counter()
     g.evolution(K)  # synchronises the device before returning
     dt = time.perf_counter() - t0
@@ -595,6 +719,10 @@ def main():
     g.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case)
+    if not args.no_summaries and not fp32 and args.config == 1:
+        out["precision"] = precision_summary()
+        out["sweep"] = sweep_summary()
+        out["serve"] = serve_summary()
     print(json.dumps(out))
 
 

@@ -163,12 +163,19 @@ int dalloc(lbmib_handle* h, T** ptr, size_t count)
     return 0;
 }
 
-int fill_i32(int* d, int v, size_t n)
+// Device fills on the null stream, which every later engine call and copy
+// of the handle follows.
+template <typename T>
+int fill_dev(T* d, T v, size_t n)
 {
-    std::vector<int> hv(n, v);
-    HIP_TRY(hipMemcpy(d, hv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (n == 0) return 0;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, nullptr, d, (long long)n, v);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
+
+int fill_i32(int* d, int v, size_t n) { return fill_dev<int>(d, v, n); }
 
 // Largest zig-zag distance between two segments whose +-2.5 lattice-unit
 // boxes can share a node under any rigid motion: such segments are within
@@ -349,34 +356,25 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     }
     TRY(dalloc(h, &D.macro[0], 3 * (size_t)N));
     TRY(dalloc(h, &D.macro[1], 3 * (size_t)N));
-    // init (solution.f90:34-59): f = f0 = feq(1, U, 0); rho = 1, u = U, v = 0
+    // init (solution.f90:34-59): f = f0 = feq(1, U, 0); rho = 1, u = U, v = 0;
+    // filled on the device, plane by plane
     {
         double feq0[9];
         for (int k = 0; k < 9; ++k) feq0[k] = feq_ref(1.0, p->U, 0.0, k);
-        std::vector<double> plane((size_t)N);
-        std::vector<float> plane32(D.prec == LBMIB_PREC_FP64 ? 0 : (size_t)N);
         for (int k = 0; k < 9; ++k) {
             if (D.prec == LBMIB_PREC_FP64) {
-                std::fill(plane.begin(), plane.end(), feq0[k]);
-                HTRY(hipMemcpy(D.f[0] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-                HTRY(hipMemcpy(D.f[1] + (size_t)k * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+                TRY(fill_dev<double>(D.f[0] + (size_t)k * N, feq0[k], (size_t)N));
+                TRY(fill_dev<double>(D.f[1] + (size_t)k * N, feq0[k], (size_t)N));
             } else {
-                std::fill(plane32.begin(), plane32.end(), (float)(feq0[k] - D.W[k]));
-                HTRY(hipMemcpy(D.g[0] + (size_t)k * N, plane32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-                HTRY(hipMemcpy(D.g[1] + (size_t)k * N, plane32.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+                TRY(fill_dev<float>(D.g[0] + (size_t)k * N, (float)(feq0[k] - D.W[k]), (size_t)N));
+                TRY(fill_dev<float>(D.g[1] + (size_t)k * N, (float)(feq0[k] - D.W[k]), (size_t)N));
             }
+            TRY(fill_dev<double>(D.f0L + (size_t)k * ny, feq0[k], (size_t)ny));
+            TRY(fill_dev<double>(D.f0R + (size_t)k * ny, feq0[k], (size_t)ny));
         }
-        std::vector<double> col(9 * (size_t)ny);
-        for (int k = 0; k < 9; ++k)
-            for (int j = 0; j < ny; ++j) col[(size_t)k * ny + j] = feq0[k];
-        HTRY(hipMemcpy(D.f0L, col.data(), sizeof(double) * col.size(), hipMemcpyHostToDevice));
-        HTRY(hipMemcpy(D.f0R, col.data(), sizeof(double) * col.size(), hipMemcpyHostToDevice));
-        std::fill(plane.begin(), plane.end(), 1.0);
-        HTRY(hipMemcpy(D.macro[0], plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-        std::fill(plane.begin(), plane.end(), p->U);
-        HTRY(hipMemcpy(D.macro[0] + N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-        std::fill(plane.begin(), plane.end(), 0.0);
-        HTRY(hipMemcpy(D.macro[0] + 2 * N, plane.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+        TRY(fill_dev<double>(D.macro[0], 1.0, (size_t)N));
+        TRY(fill_dev<double>(D.macro[0] + N, p->U, (size_t)N));
+        TRY(fill_dev<double>(D.macro[0] + 2 * N, 0.0, (size_t)N));
     }
     // markers
     double *dxb0, *dyb0;
@@ -398,6 +396,8 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     TRY(fill_i32(D.err, 0, 2));
     D.bw = geometric_bandwidth(h);
     if (D.bw >= ns) D.bw = ns - 1;
+    // the fills ran on the null stream; the engine's streams do not wait for it
+    HTRY(hipStreamSynchronize(nullptr));
     *out = h;
     return 0;
 #undef TRY
@@ -582,6 +582,8 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
         dd.rhoadj = o; o += al(cap);
         dd.glx = o; o += al(cap);
         dd.gly = o; o += al(cap);
+        dd.ust = o; o += al(cap);
+        dd.vst = o; o += al(cap);
         dd.total = o;
         if (dalloc(h, &D.ipool, (size_t)D.nslot * io.total)) return LBMIB_E_HIP;
         if (dalloc(h, &D.dpool, (size_t)D.nslot * dd.total)) return LBMIB_E_HIP;
@@ -593,6 +595,8 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
             if (fill_i32(S.nodeidx, -1, (size_t)grid)) return LBMIB_E_HIP;
             if (fill_i32(S.nadj, 0, 1)) return LBMIB_E_HIP;
         }
+        // the fills ran on the null stream; the engine's streams do not wait for it
+        HIP_TRY(hipStreamSynchronize(nullptr));
     }
     h->have_motion = true;
     return 0;

@@ -354,8 +354,16 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (wnext) HIP_TRY(hipStreamWaitEvent(sr, wnext, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[2], sr));
     const bool f64 = d.prec == LBMIB_PREC_FP64;
-    const int rhs_x = (d.maxns + 6) / 7;
-    hipLaunchKernelGGL(f64 ? k_crit_rhs<double> : k_crit_rhs<float>, dim3(rhs_x, d.nc), dim3(256), 0, sr, C, K, off);
+    if (d.nc == 1) {
+        // one case: the single pass (the shorter chain of dependent loads)
+        hipLaunchKernelGGL(f64 ? k_crit_rhs1<double> : k_crit_rhs1<float>, dim3((d.maxns + 6) / 7, d.nc), dim3(256),
+                           0, sr, C, K, off);
+    } else {
+        // several cases: u* once per IB node, then the per-marker sums
+        hipLaunchKernelGGL(f64 ? k_crit_ustar<double> : k_crit_ustar<float>, dim3((d.maxcap + 255) / 256, d.nc),
+                           dim3(256), 0, sr, C, K, off);
+        hipLaunchKernelGGL(k_crit_rhs, dim3((d.maxns + 255) / 256, d.nc), dim3(256), 0, sr, C, K, off);
+    }
     if (!rr) {
         HIP_TRY(hipEventRecord(E->ev_rhs[gs % R], s3));
         HIP_TRY(hipStreamWaitEvent(s1, E->ev_rhs[gs % R], 0));

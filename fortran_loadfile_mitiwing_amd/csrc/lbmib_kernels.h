@@ -64,6 +64,7 @@ struct SlotDoubles {
     long long zsep;     // separator step: M [bw][3bw] (x_S = M v), then N [2bw][bw] (tail corrections = N x_S)
     long long rhoadj;   // [cap] rho at the nodes
     long long glx, gly; // [cap] velocity corrections
+    long long ust, vst; // [cap] u*, v* at the nodes (moments of the post-stream populations)
     long long total;
 };
 
@@ -143,7 +144,7 @@ enum GeoMode { GEO_FIRST = 0, GEO_NEXT = 1 };
 struct Slot {
     int *claim, *nodeidx, *box, *draw_idx, *nraw_s, *adj, *nadj, *dcnt, *didx, *nmkcnt, *nmk;
     double *xb2, *yb2, *ub, *vb, *phi, *draw_val, *nraw_w, *dval, *nmkv, *G, *coef, *dinv, *rhoadj, *glx, *gly;
-    double *coef2, *dinv2, *ltail2, *zsep;
+    double *coef2, *dinv2, *ltail2, *zsep, *ust, *vst;
 };
 
 __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
@@ -161,6 +162,7 @@ __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
     S.dinv = db + D.dof.dinv; S.rhoadj = db + D.dof.rhoadj; S.glx = db + D.dof.glx; S.gly = db + D.dof.gly;
     S.coef2 = db + D.dof.coef2; S.dinv2 = db + D.dof.dinv2; S.ltail2 = db + D.dof.ltail2;
     S.zsep = db + D.dof.zsep;
+    S.ust = db + D.dof.ust; S.vst = db + D.dof.vst;
     return S;
 }
 
@@ -172,6 +174,7 @@ __host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny 
 // first node of the interior tiles: row 1 rounded down to a multiple of 64 nodes
 __host__ __device__ inline int lattice_base(int nx) { return nx / 64 * 64; }
 int lattice_tiles(int nx, int ny, int npt);  // ring blocks + interior tiles of one case
+template <typename T> __global__ void k_fill(T* __restrict__ p, long long n, T v);
 __global__ void k_geo_clear(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_markers(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_fill(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
@@ -184,7 +187,10 @@ __global__ void k_geo_factor(const DevCase* __restrict__, const CaseClock* __res
 __global__ void k_geo_factor2(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_schur(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 template <typename T>
+__global__ void k_crit_ustar(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 __global__ void k_crit_rhs(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+template <typename T>
+__global__ void k_crit_rhs1(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 template <int U, bool CMP>
 __global__ void k_crit_solve(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 int solve_unit(int npad);

@@ -303,6 +303,17 @@ int lattice_tiles(int nx, int ny, int npt)
     return lattice_ring_blocks(nx, ny) + (int)((span + 256 * npt - 1) / (256 * npt));
 }
 
+// Fills n values of p with v (grid-stride; set-up of planes and tables).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, long long n, T v)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+template __global__ void k_fill<double>(double*, long long, double);
+template __global__ void k_fill<float>(float*, long long, float);
+template __global__ void k_fill<int>(int*, long long, int);
+
 // Box search of move_boundary (airfoil.f90:290-321) for one axis: the 1-based
 // index range of nodes with |(o + (i-1)*sc - X)/sc| <= 2.5.
 __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, int& lo, int& hi)
@@ -330,7 +341,7 @@ __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, i
 //   k_geo_clear -> k_geo_markers -> k_geo_fill -> k_geo_phi -> k_geo_pairs ->
 //   k_geo_dsort -> k_geo_nsort -> k_geo_assemble -> k_geo_factor
 // Per-step solve, on the critical path beside the lattice kernel:
-//   k_crit_rhs -> k_crit_solve -> k_band (beside the lattice kernel)
+//   k_crit_ustar -> k_crit_rhs -> k_crit_solve -> k_band (beside the lattice kernel)
 // Per-wing forces of a batch of steps: k_forces.
 // ---------------------------------------------------------------------------
 
@@ -969,14 +980,73 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// u* at the step's IB nodes: the moments of the nodes' post-stream
+// populations (physical_variables, solution.f90:504-545), once per node, so
+// that the interpolation below reads each node's value instead of pulling
+// the node again for every marker box that holds it (about 11 boxes per
+// node at the standard marker spacing).
+template <typename T>
+__global__ __launch_bounds__(256) void k_crit_ustar(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                    int off)
+{
+    __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
+    const DevCase& D = C[blockIdx.y];
+    const int nstep = K[blockIdx.y].n0 + off;
+    if (nstep > K[blockIdx.y].n_end || (int)blockIdx.x * 256 >= D.cap) return;
+    const Slot S = slot_of(D, (nstep - 1) % D.nslot);
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= *S.nadj) return;
+    const long long c = (long long)S.adj[3 * a + 1] * D.nx + S.adj[3 * a];
+    double f[9], r, u, v;
+    pull9(pops<T>(D, (nstep & 1) ^ 1), D.N, D.nx, c, f, D.W);
+    moments(f, r, u, v);
+    S.ust[a] = u;
+    S.vst[a] = v;
+}
+template __global__ void k_crit_ustar<double>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_ustar<float>(const DevCase*, const CaseClock*, int);
+
 // Interpolation D u* (the DGEMVs, boundary_condition.f90:1620-1621) and the
-// right-hand side U_B - D u* (1624-1629) in zig-zag order.  Each thread
-// forms one product xx1(s,node) * u*(node) with u* taken from the moments of
-// the node's post-stream populations (physical_variables there); the
-// segment's products are then summed in ascending node order.
+// right-hand side U_B - D u* (1624-1629) in zig-zag order: one thread per
+// marker segment sums xx1(s,node) * u*(node) over its box's nodes in
+// ascending node order, each product rounded before it is added.  This
+// two-kernel form does a third of the memory work of k_crit_rhs1 and is
+// the batched form's; a single case takes k_crit_rhs1, whose one pass has
+// the shorter dependency chain.
+__global__ __launch_bounds__(256) void k_crit_rhs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                  int off)
+{
+    __builtin_amdgcn_s_setprio(3);
+    const DevCase& D = C[blockIdx.y];
+    const int nstep = K[blockIdx.y].n0 + off;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (nstep > K[blockIdx.y].n_end || s >= D.nseg) return;
+    const Slot S = slot_of(D, (nstep - 1) % D.nslot);
+    const int n = S.dcnt[s];
+    const int* __restrict__ ix = S.didx + s * BOXMAX;
+    const double* __restrict__ w = S.dval + s * BOXMAX;
+    double su = 0.0, sv = 0.0;
+    for (int q = 0; q < n; ++q) {
+        const int node = ix[q];
+        const double pu = w[q] * S.ust[node];
+        const double pv = w[q] * S.vst[node];
+        su = su + pu;
+        sv = sv + pv;
+    }
+    const int p = D.iperm[s];
+    D.rhs[p] = S.ub[s] - su;
+    D.rhs[D.nseg + p] = S.vb[s] - sv;
+}
+
+// The same right-hand side in one pass, for an engine of one case, whose IB
+// chain is latency-bound: each thread forms one product xx1(s,node) *
+// u*(node) with u* taken from the moments of the node's post-stream
+// populations, pulled by the thread itself; the segment's products are then
+// summed in ascending node order.  Bit-identical to k_crit_ustar +
+// k_crit_rhs (the same products, summed in the same order).
 constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
 template <typename T>
-__global__ __launch_bounds__(256) void k_crit_rhs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+__global__ __launch_bounds__(256) void k_crit_rhs1(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                                                   int off)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
@@ -1009,8 +1079,8 @@ __global__ __launch_bounds__(256) void k_crit_rhs(const DevCase* __restrict__ C,
         D.rhs[D.nseg + p] = S.vb[s] - sv;
     }
 }
-template __global__ void k_crit_rhs<double>(const DevCase*, const CaseClock*, int);
-template __global__ void k_crit_rhs<float>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_rhs1<double>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_rhs1<float>(const DevCase*, const CaseClock*, int);
 
 // Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
 // right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
