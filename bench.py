@@ -9,9 +9,13 @@ outer boundaries.  The lattice starts resident in HBM; the timed region is
 exactly K steps between two device synchronisations.
 
 value = lattice updates per second of whole steps (MLUPS, nx*ny per step).
-roofline = the fused stream+collide kernel (the dominant one): 144 algorithmic
-bytes per interior node (9 FP64 populations read + 9 written) over its mean
-launch time, timed with HIP events on the stream it runs on.
+roofline = the dominant kernel.  Steps are taken in two-step passes (the
+default launch shape): the far-field kernel k_far2 advances every node
+outside the lattice window that holds the body by two updates per launch, so
+its algorithmic bytes are 2 x 144 per far-field node (9 FP64 populations read
++ 9 written per update) over its mean launch time, timed with HIP events on
+the stream it runs on.  With single steps (fuse_steps=0 in --shape) it is the
+fused stream+collide kernel k_lattice, 144 bytes per interior node.
 cpu_baseline = the CPU oracle (C restatement of the reference step, OpenMP
 over the host cores) on the first steps of the same workload.
 
@@ -73,6 +77,8 @@ PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every pha
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
 # device copy measured the same way (the lattice traffic summary in profiles/)
 LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
+# the same for a k_far2<double> launch (two updates of the far field)
+FAR_TRAFFIC_BYTES = {}
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
@@ -688,8 +694,21 @@ def main():
     cells = p.nx * p.ny
     interior = (p.nx - 2) * (p.ny - 2)
     bpn = BYTES_PER_NODE_F32 if fp32 else BYTES_PER_NODE_F64
-    lat_s = tm["lattice_ms"] * 1e-3
-    achieved = interior * bpn / lat_s / 1e9
+    wi0, wj0, ww, wh = g.lattice_window()
+    if tm["far_passes"] > 0:
+        # two-step passes: the far-field kernel advances every node outside
+        # the lattice window by two updates per launch
+        n_far = cells - ww * wh
+        kern = "k_far2<%s>" % ("float" if fp32 else "double")
+        bytes_launch = 2 * n_far * bpn
+        lat_s = tm["far_pass_ms"] * 1e-3
+        traffic = None if fp32 else FAR_TRAFFIC_BYTES.get((p.nx, p.ny))
+    else:
+        kern = "k_lattice<%s>" % ("float" if fp32 else "double")
+        bytes_launch = interior * bpn
+        lat_s = tm["lattice_ms"] * 1e-3
+        traffic = None if fp32 else LATTICE_TRAFFIC_BYTES.get((p.nx, p.ny))
+    achieved = bytes_launch / lat_s / 1e9
     value = cells * K / dt / 1e6
     out = {
         "metric": "MLUPS (whole %s timestep incl. immersed boundary)" % ("FP32-deviation" if fp32 else "FP64"),
@@ -708,10 +727,9 @@ def main():
                                                 "FP32 deviation form" if fp32 else "FP64 drop-in"),
                    "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if fp32 else LATTICE_TRAFFIC_BYTES.get((p.nx, p.ny)),
-                     "kernel": "k_lattice<%s>" % ("float" if fp32 else "double"),
-                     "bytes_per_launch": interior * bpn, "mean_launch_ms": round(lat_s * 1e3, 5)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
+                     "bytes_per_launch": bytes_launch, "mean_launch_ms": round(lat_s * 1e3, 5)},
+        "lattice_window": {"i0": wi0, "j0": wj0, "width": ww, "height": wh},
         "whole_step_hbm_frac": round(value * 1e6 * bpn / (HBM_PEAK_GBS * 1e9), 4),
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},
         "launch_shape": g.shape(),

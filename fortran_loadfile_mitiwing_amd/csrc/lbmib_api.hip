@@ -606,6 +606,18 @@ int lbmib_set_motion_table(lbmib_handle* h, const double* rows, int32_t nrows, i
 
 namespace lbmib {
 
+int ensure_mid_plane(lbmib_handle* h)
+{
+    DevCase& D = h->D;
+    const size_t n = 9 * (size_t)D.N;
+    if (D.prec == LBMIB_PREC_FP64) {
+        if (!D.f[2]) return dalloc(h, &D.f[2], n);
+    } else {
+        if (!D.g[2]) return dalloc(h, &D.g[2], n);
+    }
+    return 0;
+}
+
 int nan_guard_check(lbmib_handle* h)
 {
     if (!h->p.nan_guard || h->steps_done == 0) return 0;
@@ -614,10 +626,10 @@ int nan_guard_check(lbmib_handle* h)
     hipStream_t st = nullptr;  // every engine call has synchronised its own streams
     if (h->D.prec == LBMIB_PREC_FP64)
         hipLaunchKernelGGL(k_nan_check<double>, dim3(1024), dim3(256), 0, st,
-                           (const double*)h->D.f[h->steps_done & 1], 9 * h->D.N, flag);
+                           (const double*)h->D.f[state_plane(h)], 9 * h->D.N, flag);
     else
         hipLaunchKernelGGL(k_nan_check<float>, dim3(1024), dim3(256), 0, st,
-                           (const float*)h->D.g[h->steps_done & 1], 9 * h->D.N, flag);
+                           (const float*)h->D.g[state_plane(h)], 9 * h->D.N, flag);
     int f = 0;
     HIP_TRY(hipMemcpyAsync(&f, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -726,7 +738,7 @@ static int move_domain(lbmib_handle* h, int dr)
 {
     DevCase& D = h->D;
     const long long N = D.N;
-    const int nx = D.nx, ny = D.ny, slot = h->steps_done & 1;
+    const int nx = D.nx, ny = D.ny, slot = state_plane(h), mslot = h->steps_done & 1;
     RingW w;
     for (int k = 0; k < 9; ++k) w.w[k] = D.W[k];
     const dim3 gp((unsigned)((9 * N + 255) / 256)), gr((unsigned)((9 * ny + 255) / 256)), b(256);
@@ -742,10 +754,10 @@ static int move_domain(lbmib_handle* h, int dr)
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(D.g[slot], D.g[slot ^ 1], sizeof(float) * 9 * N, hipMemcpyDeviceToDevice, nullptr));
     }
-    if (h->last_flag_step[slot] == h->steps_done) {
+    if (h->last_flag_step[mslot] == h->steps_done) {
         const dim3 gm((unsigned)((3 * N + 255) / 256));
-        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[slot], D.macro[slot ^ 1], nx, N, dr, 0);
-        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[slot], D.macro[slot ^ 1], nx, N, dr, 1);
+        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[mslot], D.macro[mslot ^ 1], nx, N, dr, 0);
+        hipLaunchKernelGGL(k_move_macro, gm, b, 0, nullptr, D.macro[mslot], D.macro[mslot ^ 1], nx, N, dr, 1);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipDeviceSynchronize());
@@ -959,7 +971,7 @@ int lbmib_download(lbmib_handle* h, double* rho, double* u, double* v, double* f
     HIP_TRY(hipSetDevice(h->dev));
     HIP_TRY(hipDeviceSynchronize());
     const long long N = h->D.N;
-    const int slot = h->steps_done & 1;
+    const int slot = h->steps_done & 1, plane = state_plane(h);
     if (h->last_flag_step[slot] != h->steps_done)
         return fail(LBMIB_E_STATE, "macroscopic fields of step %d were not recorded", h->steps_done);
     const double* m = h->D.macro[slot];
@@ -967,11 +979,11 @@ int lbmib_download(lbmib_handle* h, double* rho, double* u, double* v, double* f
     if (u) HIP_TRY(hipMemcpy(u, m + N, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (v) HIP_TRY(hipMemcpy(v, m + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (f && h->D.prec == LBMIB_PREC_FP64)
-        HIP_TRY(hipMemcpy(f, h->D.f[slot], sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(f, h->D.f[plane], sizeof(double) * 9 * N, hipMemcpyDeviceToHost));
     if (f && h->D.prec != LBMIB_PREC_FP64) {
         std::vector<float> g((size_t)N);
         for (int k = 0; k < 9; ++k) {
-            HIP_TRY(hipMemcpy(g.data(), h->D.g[slot] + (size_t)k * N, sizeof(float) * N, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(g.data(), h->D.g[plane] + (size_t)k * N, sizeof(float) * N, hipMemcpyDeviceToHost));
             for (long long c = 0; c < N; ++c) f[(size_t)k * N + c] = h->D.W[k] + (double)g[c];
         }
     }
@@ -984,14 +996,14 @@ int lbmib_upload_state(lbmib_handle* h, const double* f)
     HIP_TRY(hipSetDevice(h->dev));
     HIP_TRY(hipDeviceSynchronize());
     const long long N = h->D.N;
-    const int slot = h->steps_done & 1;
+    const int plane = state_plane(h);
     if (h->D.prec == LBMIB_PREC_FP64) {
-        HIP_TRY(hipMemcpy(h->D.f[slot], f, sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->D.f[plane], f, sizeof(double) * 9 * N, hipMemcpyHostToDevice));
     } else {
         std::vector<float> g((size_t)N);
         for (int k = 0; k < 9; ++k) {
             for (long long c = 0; c < N; ++c) g[c] = (float)(f[(size_t)k * N + c] - h->D.W[k]);
-            HIP_TRY(hipMemcpy(h->D.g[slot] + (size_t)k * N, g.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(h->D.g[plane] + (size_t)k * N, g.data(), sizeof(float) * N, hipMemcpyHostToDevice));
         }
     }
     return 0;
@@ -1196,6 +1208,17 @@ int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_
     }
     if (glx) HIP_TRY(hipMemcpy(glx, S.glx, sizeof(double) * na, hipMemcpyDeviceToHost));
     if (gly) HIP_TRY(hipMemcpy(gly, S.gly, sizeof(double) * na, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbmib_lattice_window(const lbmib_handle* h, int32_t* i0, int32_t* j0, int32_t* w, int32_t* hgt)
+{
+    if (!h || !i0 || !j0 || !w || !hgt) return fail(LBMIB_E_ARG, "null argument");
+    if (!h->have_motion) return fail(LBMIB_E_STATE, "no motion table");
+    *i0 = h->D.gi0;
+    *j0 = h->D.gj0;
+    *w = h->D.gw;
+    *hgt = h->D.gh;
     return 0;
 }
 

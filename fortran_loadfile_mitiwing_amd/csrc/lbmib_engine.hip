@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 
 namespace lbmib {
@@ -217,6 +218,17 @@ Dims dims_of(const Engine* E)
     return d;
 }
 
+// Whether an engine takes two-step passes: one case, prescribed motion (the
+// dynamics step one at a time), and the lattice window of the run at least
+// 4 nodes away from every edge (the window's single steps then never touch
+// the outer ring).
+bool fuse_ok(const Engine* E)
+{
+    if (!E->shape.fuse || E->shape.graph || E->per_step || E->members.size() != 1) return false;
+    const DevCase& D = E->members[0]->D;
+    return D.gi0 >= 4 && D.gj0 >= 4 && D.gi0 + D.gw <= D.nx - 4 && D.gj0 + D.gh <= D.ny - 4;
+}
+
 int engine_prepare(Engine* E)
 {
     HIP_TRY(hipSetDevice(E->dev));
@@ -244,6 +256,22 @@ int engine_prepare(Engine* E)
         E->tab_bytes.clear();
     }
     int rc = 0;
+    if (fuse_ok(E)) {
+        // the middle plane, and the far-field grid: strips of at most FAR_OUT
+        // columns, segments of about 10 rows (FP64) or 8 (FP32).  Many short
+        // segments beat one workgroup per CU slot with tall ones (4096x2048,
+        // FP64: 0.338 ms per pass at 224 segments against 0.488 ms at 28):
+        // the workgroups in flight, taken in launch order, then cover one band
+        // of consecutive rows, and the last ones balance the load; shorter
+        // segments than these repeat more of phase 1.  LBMIB_FAR_SEGS overrides.
+        if ((rc = ensure_mid_plane(E->members[0]))) return rc;
+        const DevCase& D = E->members[0]->D;
+        E->far_strips = (D.nx + FAR_OUT - 1) / FAR_OUT;
+        E->far_segs = std::max(1, D.ny / (D.prec == LBMIB_PREC_FP64 ? 10 : 8));
+        if (const char* e = getenv("LBMIB_FAR_SEGS")) E->far_segs = std::max(1, std::min(D.ny / 4, atoi(e)));
+        HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)far2_smem_bytes(true)));
+    }
     // the case table goes to the device only when it changed since the
     // last window (byte copies, so that padding compares equal too)
     std::vector<unsigned char> tab(sizeof(DevCase) * nc);
@@ -324,6 +352,63 @@ int launch_factor(Engine* E, const Dims& d, int off, int mode, hipEvent_t* gev)
     return 0;
 }
 
+// Host check of the planes a launch names: each index one of the case's
+// allocated planes (0, 1, or the middle plane 2).
+bool planes_ok(const DevCase& D, std::initializer_list<int> planes)
+{
+    for (int p : planes) {
+        if (p < 0 || p > 2) return false;
+        if (D.prec == LBMIB_PREC_FP64 ? D.f[p] == nullptr : D.g[p] == nullptr) return false;
+    }
+    return true;
+}
+
+// Right-hand sides of the IB systems of the step at window offset off on
+// stream st: one case takes the single pass (the shorter chain of dependent
+// loads), several cases u* once per IB node and then the per-marker sums.
+int launch_rhs(Engine* E, const Dims& d, hipStream_t st, int off, int psel)
+{
+    const DevCase* C = E->dcases;
+    const CaseClock* K = E->dclock;
+    const bool f64 = d.prec == LBMIB_PREC_FP64;
+    if (d.nc == 1) {
+        hipLaunchKernelGGL(f64 ? k_crit_rhs1<double> : k_crit_rhs1<float>, dim3((d.maxns + 6) / 7, d.nc), dim3(256),
+                           0, st, C, K, off, psel);
+    } else {
+        hipLaunchKernelGGL(f64 ? k_crit_ustar<double> : k_crit_ustar<float>, dim3((d.maxcap + 255) / 256, d.nc),
+                           dim3(256), 0, st, C, K, off, psel);
+        hipLaunchKernelGGL(k_crit_rhs, dim3((d.maxns + 255) / 256, d.nc), dim3(256), 0, st, C, K, off);
+    }
+    return 0;
+}
+
+// Triangular solves of the step at window offset off on stream st: chain
+// members of each coefficient layout and separator-form members (a kernel
+// returns at once for the members of the other kinds).
+int launch_solve(Engine* E, const Dims& d, hipStream_t st, int off)
+{
+    const DevCase* C = E->dcases;
+    const CaseClock* K = E->dclock;
+    const size_t smc = solve_smem_bytes(d.maxnpad);
+    const bool u64 = chain_unit(E, d) == 64;
+    if (d.chain > d.plain) {
+        if (u64) hipLaunchKernelGGL((k_crit_solve<64, true>), dim3(d.nc), dim3(solve_threads()), smc, st, C, K, off);
+        else hipLaunchKernelGGL((k_crit_solve<32, true>), dim3(d.nc), dim3(solve_threads()), smc, st, C, K, off);
+    }
+    if (d.plain) {
+        if (u64) hipLaunchKernelGGL((k_crit_solve<64, false>), dim3(d.nc), dim3(solve_threads()), smc, st, C, K, off);
+        else hipLaunchKernelGGL((k_crit_solve<32, false>), dim3(d.nc), dim3(solve_threads()), smc, st, C, K, off);
+    }
+    if (d.sep) {
+        const size_t sm = solve2_smem_bytes(d.maxnp2);
+        if (sep_unit(E, d) == 32)
+            hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, st, C, K, off);
+        else
+            hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, st, C, K, off);
+    }
+    return 0;
+}
+
 // Step at window offset off.  ev (when this step is sampled): [0]/[1]
 // around the lattice update, [2] before the right-hand sides, [3] after the
 // solves, [4] after the IB-node collides.
@@ -334,10 +419,12 @@ int launch_factor(Engine* E, const Dims& d, int off, int mode, hipEvent_t* gev)
 constexpr int LAG = Engine::RING / 2;
 
 int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
-                hipEvent_t wnodes, hipEvent_t* ev)
+                hipEvent_t wnodes, hipEvent_t* ev, int psel)
 {
     const int R = Engine::RING;
     const long long gs = E->gstep;
+    if (psel >= 0 && !planes_ok(E->members[0]->D, {psel & 3, (psel >> 2) & 3}))
+        return fail(LBMIB_E_STATE, "step planes %d -> %d not allocated", psel & 3, (psel >> 2) & 3);
     if (off >= LAG) HIP_TRY(hipEventSynchronize(E->ev_step[(gs - LAG) % R]));
     const DevCase* C = E->dcases;
     const CaseClock* K = E->dclock;
@@ -354,39 +441,12 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (wnext) HIP_TRY(hipStreamWaitEvent(sr, wnext, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[2], sr));
     const bool f64 = d.prec == LBMIB_PREC_FP64;
-    if (d.nc == 1) {
-        // one case: the single pass (the shorter chain of dependent loads)
-        hipLaunchKernelGGL(f64 ? k_crit_rhs1<double> : k_crit_rhs1<float>, dim3((d.maxns + 6) / 7, d.nc), dim3(256),
-                           0, sr, C, K, off);
-    } else {
-        // several cases: u* once per IB node, then the per-marker sums
-        hipLaunchKernelGGL(f64 ? k_crit_ustar<double> : k_crit_ustar<float>, dim3((d.maxcap + 255) / 256, d.nc),
-                           dim3(256), 0, sr, C, K, off);
-        hipLaunchKernelGGL(k_crit_rhs, dim3((d.maxns + 255) / 256, d.nc), dim3(256), 0, sr, C, K, off);
-    }
+    if (int rc = launch_rhs(E, d, sr, off, psel)) return rc;
     if (!rr) {
         HIP_TRY(hipEventRecord(E->ev_rhs[gs % R], s3));
         HIP_TRY(hipStreamWaitEvent(s1, E->ev_rhs[gs % R], 0));
     }
-    // chain members of each coefficient layout (a kernel returns at once for
-    // the other layout's and the separator form's members)
-    const size_t smc = solve_smem_bytes(d.maxnpad);
-    const bool u64 = chain_unit(E, d) == 64;
-    if (d.chain > d.plain) {
-        if (u64) hipLaunchKernelGGL((k_crit_solve<64, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
-        else hipLaunchKernelGGL((k_crit_solve<32, true>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
-    }
-    if (d.plain) {
-        if (u64) hipLaunchKernelGGL((k_crit_solve<64, false>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
-        else hipLaunchKernelGGL((k_crit_solve<32, false>), dim3(d.nc), dim3(solve_threads()), smc, s1, C, K, off);
-    }
-    if (d.sep) {
-        const size_t sm = solve2_smem_bytes(d.maxnp2);
-        if (sep_unit(E, d) == 32)
-            hipLaunchKernelGGL(k_crit_solve2<32>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
-        else
-            hipLaunchKernelGGL(k_crit_solve2<16>, dim3(d.nc), dim3(solve2_threads()), sm, s1, C, K, off);
-    }
+    if (int rc = launch_solve(E, d, s1, off)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
     // the forced collide of the IB nodes follows the solves on their reserved
     // CUs, or (shape.band) on the shared CUs of the right-hand-side stream
@@ -397,7 +457,7 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
         sb = s3;
     }
     const int band_x = std::max(1, (3 * d.maxns + 255) / 256);
-    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, sb, C, K, off);
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, sb, C, K, off, psel);
     if (ev) HIP_TRY(hipEventRecord(ev[4], sb));
     HIP_TRY(hipEventRecord(E->ev_ib[gs % R], sb));
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
@@ -410,11 +470,11 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     const int2* tiles = E->dtiles;
     const int nt = E->ntiles;
     if (f64) {
-        if (npt == 2) hipLaunchKernelGGL((k_lattice<double, 2>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off);
-        else hipLaunchKernelGGL((k_lattice<double, 1>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off);
+        if (npt == 2) hipLaunchKernelGGL((k_lattice<double, 2>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off, psel);
+        else hipLaunchKernelGGL((k_lattice<double, 1>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off, psel);
     } else {
-        if (npt == 2) hipLaunchKernelGGL((k_lattice<float, 2>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off);
-        else hipLaunchKernelGGL((k_lattice<float, 1>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off);
+        if (npt == 2) hipLaunchKernelGGL((k_lattice<float, 2>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off, psel);
+        else hipLaunchKernelGGL((k_lattice<float, 1>), dim3(grid), dim3(256), 0, s0, C, K, tiles, nt, off, psel);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s0));
@@ -422,16 +482,112 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     return 0;
 }
 
+// Steps n, n+1 at window offsets off, off+1 of an engine of one case as a
+// two-step pass (k_far2 on s0) over the far field beside the window's single
+// steps (k_window on s3) and the IB chain of both steps (s1).  Planes: the
+// state of step n-1 in X, of step n+1 written to Y = X ^ 1, step n in the
+// middle plane 2 where the window's second step and the IB chain of step
+// n+1 read it.  Order (every wait an event of the step before or of this
+// pass):
+//   IB(n):   after the window/lattice and the IB collide of step n-1
+//   W(n):    after the update and IB collide of step n-1 (plane X complete)
+//   F:       after the same (it reads X everywhere and writes Y and the
+//            middle plane's layer around W, whose readers of the step
+//            before are done)
+//   IB(n+1): after W(n) (and IB(n) in its own stream)
+//   W(n+1):  after F in F's stream (its layer around W), W(n) and the IB
+//            collide of step n
+// The ring's step n (k_ring) goes first into F's stream: it needs only the
+// far pass or lattice update before it there.
+// ev as in launch_step for step n, plus [5]/[6] around W(n); [0]/[1] around F.
+int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
+                hipEvent_t wnodes, hipEvent_t* ev, int X)
+{
+    const int R = Engine::RING;
+    const long long gs = E->gstep;
+    if (off >= LAG) HIP_TRY(hipEventSynchronize(E->ev_step[(gs - LAG) % R]));
+    if (off + 1 >= LAG) HIP_TRY(hipEventSynchronize(E->ev_step[(gs + 1 - LAG) % R]));
+    const DevCase* C = E->dcases;
+    const CaseClock* K = E->dclock;
+    hipStream_t s0 = E->s[0], s1 = E->s[1], s3 = E->s[3];
+    const bool f64 = d.prec == LBMIB_PREC_FP64;
+    const int Y = X ^ 1, M = 2;
+    const int pa = make_psel(X, M), pb = make_psel(M, Y);
+    if (!planes_ok(E->members[0]->D, {X, Y, M}))
+        return fail(LBMIB_E_STATE, "two-step pass: planes %d, %d, %d not all allocated", X, Y, M);
+    // IB chain of step n on the solves' stream
+    if (after_prev) HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[(gs - 1) % R], 0));
+    if (wfirst) HIP_TRY(hipStreamWaitEvent(s1, wfirst, 0));
+    if (wnext) HIP_TRY(hipStreamWaitEvent(s1, wnext, 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s1));
+    if (int rc = launch_rhs(E, d, s1, off, pa)) return rc;
+    if (int rc = launch_solve(E, d, s1, off)) return rc;
+    if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
+    const int band_x = std::max(1, (3 * d.maxns + 255) / 256);
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, 1), dim3(256), 0, s1, C, K, off, pa);
+    if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
+    HIP_TRY(hipEventRecord(E->ev_ib[gs % R], s1));
+    // the ring's step n into the middle plane: it reads only the ring's
+    // neighbourhood of plane X, which the far pass (or lattice update) before
+    // it on this stream wrote, so it runs beside the window step n-1
+    const DevCase& D = E->members[0]->D;
+    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3((unsigned)lattice_ring_blocks(D.nx, D.ny)),
+                       dim3(256), 0, s0, C, K, off, pa);
+    // window step n on the shared CUs
+    const dim3 wg((unsigned)(((long long)D.gw * D.gh + 255) / 256));
+    if (after_prev) {
+        HIP_TRY(hipStreamWaitEvent(s3, E->ev_step[(gs - 1) % R], 0));
+        HIP_TRY(hipStreamWaitEvent(s3, E->ev_ib[(gs - 1) % R], 0));
+    }
+    if (wfirst) HIP_TRY(hipStreamWaitEvent(s3, wnodes ? wnodes : wfirst, 0));
+    if (wnext) HIP_TRY(hipStreamWaitEvent(s3, wnext, 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[5], s3));
+    hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s3, C, K, off, pa);
+    if (ev) HIP_TRY(hipEventRecord(ev[6], s3));
+    HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
+    // the far field, both steps
+    if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
+    const dim3 fg((unsigned)E->far_strips, (unsigned)E->far_segs);
+    const size_t fsm = far2_smem_bytes(f64);
+    const int pf = X | (Y << 2) | (M << 4);  // k_far2's planes: src, dst, middle
+    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, off, pf);
+    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, off, pf);
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
+    // IB chain of step n+1
+    HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[gs % R], 0));
+    if (int rc = launch_rhs(E, d, s1, off + 1, pb)) return rc;
+    if (int rc = launch_solve(E, d, s1, off + 1)) return rc;
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, 1), dim3(256), 0, s1, C, K, off + 1, pb);
+    HIP_TRY(hipEventRecord(E->ev_ib[(gs + 1) % R], s1));
+    // window step n+1 right after the far pass in its stream (no hand-over on
+    // the path from one pass to the next); W(n) and the IB collide of step n
+    // have long finished by then
+    HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[gs % R], 0));
+    hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s0, C, K, off + 1, pb);
+    HIP_TRY(hipEventRecord(E->ev_step[(gs + 1) % R], s0));
+    E->gstep += 2;
+    return 0;
+}
+
 // Enqueue `len` steps from window offset 0; phase = position of offset 0
 // inside its geometry batch, fresh = some case takes its first step at
 // offset 0.  With prof, samples every prof_every-th step.
-int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof)
+//
+// An engine of one case that takes two-step passes (par >= 0) names the
+// planes of every step itself: the state of step m lies in plane
+// (m + par) & 1, and each pass flips par.  Passes start at even positions of
+// the geometry batch, so they never straddle a batch boundary.
+int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par)
 {
     const int B = E->B, BR = Engine::BRING;
     int fupto = 0;
     hipEvent_t prev_next = nullptr;
     const int every = prof ? E->prof_every : 0;
-    const int Eper = 5;
+    const int Eper = Engine::EPER;
+    const int n0 = par ? E->members[0]->steps_done + 1 : 0;
+    if (every > 0) E->samp_kind.assign((size_t)(len + every - 1) / every, 2);
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
         if ((phase + off) % B == 0 || (fresh && off == 0)) {
@@ -480,17 +636,32 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
                 HIP_TRY(hipEventRecord(gev[2], st));
             }
         }
+        const bool pair = par && (phase + off) % 2 == 0 && off + 1 < len;
         hipEvent_t* ev = nullptr;
-        if (every > 0 && off % every == 0) {
-            const size_t need = (size_t)Eper * (off / every + 1);
+        int q = -1;
+        if (every > 0 && off % every == 0) q = off / every;
+        else if (every > 0 && pair && (off + 1) % every == 0) q = (off + 1) / every;
+        if (q >= 0) {
+            const size_t need = (size_t)Eper * (q + 1);
             while (E->evpool.size() < need + 2) {
                 hipEvent_t e;
                 HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
                 E->evpool.push_back(e);
             }
-            ev = &E->evpool[2 + (size_t)Eper * (off / every)];
+            ev = &E->evpool[2 + (size_t)Eper * q];
+            E->samp_kind[q] = pair ? 1 : 0;
         }
-        int rc = launch_step(E, d, off, off > 0, wfirst, wnext, wnodes, ev);
+        int rc;
+        if (pair) {
+            const int X = (n0 + off - 1 + *par) & 1;
+            rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X);
+            *par ^= 1;
+            ++off;
+        } else {
+            const int n = n0 + off;
+            const int psel = par ? make_psel((n - 1 + *par) & 1, (n + *par) & 1) : -1;
+            rc = launch_step(E, d, off, off > 0, wfirst, wnext, wnodes, ev, psel);
+        }
         if (rc) return rc;
     }
     if (len > fupto) return enqueue_forces(E, d, fupto, len - 1, E->gstep - 1);
@@ -569,7 +740,7 @@ int capture_graph(Engine* E, const Dims& d)
     for (int i = 1; i < Engine::NSTREAM; ++i)
         if (hipStreamWaitEvent(E->s[i], E->ev_fork, 0) != hipSuccess)
             return abort_capture(fail(LBMIB_E_HIP, "graph fork wait failed"));
-    int rc = launch_window(E, d, len, 0, false, false);
+    int rc = launch_window(E, d, len, 0, false, false, nullptr);
     if (rc) return abort_capture(rc);
     for (int i = 1; i < Engine::NSTREAM; ++i) {
         if (hipEventRecord(E->ev_join[i], E->s[i]) != hipSuccess || hipStreamWaitEvent(s0, E->ev_join[i], 0) != hipSuccess)
@@ -665,6 +836,7 @@ int engine_set_shape(Engine* E, const Shape& s)
     if (s.rhs != 0 && s.rhs != 1) return fail(LBMIB_E_ARG, "ib_rhs_reserved must be 0 or 1");
     if (s.unit != 0 && s.unit != 16 && s.unit != 32 && s.unit != 64)
         return fail(LBMIB_E_ARG, "solve_unit must be 0, 16, 32 or 64");
+    if (s.fuse != 0 && s.fuse != 1) return fail(LBMIB_E_ARG, "fuse_steps must be 0 or 1");
     // Graph replay is switched off: its first run on the GPU (a three-case
     // batch, replaying captured 16-step windows on CU-masked streams) ended
     // the host process with a segmentation fault whose cause is not known.
@@ -690,7 +862,8 @@ int engine_run(Engine* E, int k)
     }
     const int B = E->B;
     const int every = E->prof_every;
-    const int Eper = 5;
+    const int Eper = Engine::EPER;
+    const bool fuse = fuse_ok(E);
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (every > 0) {
         while (E->evpool.size() < 2) {
@@ -740,18 +913,32 @@ int engine_run(Engine* E, int k)
             for (int r = 0; r < len / E->graph_len; ++r) HIP_TRY(hipGraphLaunch(E->gexec, E->s[0]));
         } else {
             if (every > 0) E->gev_used = 0;
-            if ((rc = launch_window(E, d, len, phase, fresh, every > 0))) return rc;
+            int par = fuse ? E->members[0]->D.pofs : 0;
+            if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr))) return rc;
+            if (fuse) E->members[0]->D.pofs = par;
             if (every > 0) {
                 const int nsamp = (len + every - 1) / every;
                 if ((rc = sync_all(E))) return rc;
                 for (int q = 0; q < nsamp; ++q) {
+                    const int kind = E->samp_kind[q];
+                    if (kind == 2) continue;
                     hipEvent_t* ev = &E->evpool[2 + (size_t)Eper * q];
                     float ms;
-                    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); E->tms[3] += ms;
+                    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                    if (kind == 0) {
+                        E->tms[3] += ms;
+                        E->tms[15] += 1.0;
+                    } else {
+                        E->tms[11] += ms;
+                        E->tms[12] += 1.0;
+                        HIP_TRY(hipEventElapsedTime(&ms, ev[5], ev[6]));
+                        E->tms[13] += ms;
+                        E->tms[14] += 1.0;
+                    }
                     HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); E->tms[4] += ms;
                     HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4])); E->tms[5] += ms;
+                    ++sampled;
                 }
-                sampled += nsamp;
                 for (int q = 0; q < E->gev_used; ++q) {
                     hipEvent_t* ev = &E->gevpool[(size_t)3 * q];
                     float ms;
@@ -800,6 +987,7 @@ Shape shape_in(const lbmib_launch_shape* s)
     o.band = s->ib_collide_shared;
     o.rhs = s->ib_rhs_reserved;
     o.unit = s->solve_unit;
+    o.fuse = s->fuse_steps;
     return o;
 }
 
@@ -814,6 +1002,7 @@ void shape_out(const Shape& s, lbmib_launch_shape* o)
     o->ib_collide_shared = s.band;
     o->ib_rhs_reserved = s.rhs;
     o->solve_unit = s.unit;
+    o->fuse_steps = s.fuse;
 }
 
 }  // namespace

@@ -73,6 +73,7 @@ struct Shape {
     int band = 0;
     int rhs = 1;
     int unit = 0;
+    int fuse = 1;  // an engine of one case takes two-step passes over the far field
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in
@@ -114,7 +115,12 @@ struct Engine {
     hipGraphExec_t gexec = nullptr;
     int graph_len = 0;
     bool graph_valid = false;
-    // profiling
+    // two-step passes (an engine of one case): the far kernel's grid
+    int far_strips = 0, far_segs = 0;
+    // profiling: EPER events per sampled step or pass (see launch_pair), the
+    // kind of each sample (0 single step, 1 pass, 2 unused)
+    static constexpr int EPER = 7;
+    std::vector<char> samp_kind;
     int prof_every = 0;
     std::vector<hipEvent_t> evpool, gevpool;
     int gev_used = 0;
@@ -129,5 +135,9 @@ int engine_set_shape(Engine* E, const Shape& s);
 int shape_from_env(Shape& s);
 // NaN guard of one case after a call (no-op unless params.nan_guard)
 int nan_guard_check(lbmib_handle* h);
+// The middle plane of two-step passes, allocated on first use.
+int ensure_mid_plane(lbmib_handle* h);
+// Plane holding the state after the case's last step.
+inline int state_plane(const lbmib_handle* h) { return (h->steps_done + h->D.pofs) & 1; }
 
 }  // namespace lbmib

@@ -96,8 +96,9 @@ struct DevCase {
     // lattice (population-planar, x fastest): FP64 populations f, or in the
     // FP32 deviation form g = f - W
     int prec;               // LBMIB_PREC_FP64 / LBMIB_PREC_FP32DEV
-    double* f[2];
-    float* g[2];
+    double* f[3];           // [0], [1]: the state planes; [2]: the middle step of two-step passes (or null)
+    float* g[3];
+    int pofs;               // plane parity: the state after step m lies in plane (m + pofs) & 1
     double W[9];            // the reference's REAL(4) weights, as doubles
     double rho0;            // sum of W
     DevConst K;             // FP32 collision constants
@@ -166,10 +167,36 @@ __host__ __device__ inline Slot slot_of(const DevCase& D, int g)
     return S;
 }
 
+// Planes a step reads and writes.  psel < 0: the case's own parity (the
+// state of step m in plane (m + pofs) & 1); otherwise the launch names them,
+// src = psel & 3 and dst = psel >> 2 (an engine of one case that runs two-step
+// passes through the middle plane 2).
+__host__ __device__ inline void step_planes(const DevCase& D, int n, int psel, int& src, int& dst)
+{
+    if (psel < 0) {
+        dst = (n + D.pofs) & 1;
+        src = dst ^ 1;
+    } else {
+        src = psel & 3;
+        dst = (psel >> 2) & 3;
+    }
+}
+__host__ __device__ inline int make_psel(int src, int dst) { return src | (dst << 2); }
+
 // ---- kernels (lbmib_kernels.hip); C: case table, K: clocks of the window ----
 template <typename T, int NPT>
 __global__ void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                          const int2* __restrict__ tiles, int ntiles, int off);
+                          const int2* __restrict__ tiles, int ntiles, int off, int psel);
+// two-step far-field pass and the window's single steps (an engine of one case)
+constexpr int FAR_LANES = 256;             // phase-1 columns per strip
+constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip
+template <typename T>
+__global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+template <typename T>
+__global__ void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+template <typename T>
+__global__ void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+size_t far2_smem_bytes(bool f64);
 __host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
 // first node of the interior tiles: row 1 rounded down to a multiple of 64 nodes
 __host__ __device__ inline int lattice_base(int nx) { return nx / 64 * 64; }
@@ -187,10 +214,10 @@ __global__ void k_geo_factor(const DevCase* __restrict__, const CaseClock* __res
 __global__ void k_geo_factor2(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_geo_schur(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 template <typename T>
-__global__ void k_crit_ustar(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+__global__ void k_crit_ustar(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_crit_rhs(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 template <typename T>
-__global__ void k_crit_rhs1(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+__global__ void k_crit_rhs1(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 template <int U, bool CMP>
 __global__ void k_crit_solve(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 int solve_unit(int npad);
@@ -201,7 +228,7 @@ int solve2_unit(int np2);  // 0 when the values leave no room for the rings
 int solve2_threads();
 size_t solve2_smem_bytes(int np2);
 template <typename T>
-__global__ void k_band(const DevCase* __restrict__, const CaseClock* __restrict__, int);
+__global__ void k_band(const DevCase* __restrict__, const CaseClock* __restrict__, int, int);
 __global__ void k_forces(const DevCase* __restrict__, const CaseClock* __restrict__, int);
 __global__ void k_clock_advance(CaseClock* __restrict__, int ncases, int steps);
 size_t factor_smem_bytes();

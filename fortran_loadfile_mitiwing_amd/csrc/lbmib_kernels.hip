@@ -27,6 +27,13 @@ namespace lbmib {
 // population access a global (not flat) memory instruction.
 #define GAS __attribute__((address_space(1)))
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, not for its outstanding global stores.
+__device__ __forceinline__ void lds_barrier()
+{
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Stored population -> FP64 population: FP64 storage holds f itself, the
 // FP32 deviation form holds g = f - W.
 __device__ __forceinline__ double lift(double v, double) { return v; }
@@ -100,24 +107,40 @@ __device__ __forceinline__ void right_char(double cs, double tr, double tu, doub
     vo = tv - lx2;
 }
 
+// The steps below read the previous step's populations through a pull
+// accessor: pull(i, j, f) gives the nine populations streamed into node
+// (i, j), f[k] = prev(i - e_kx, j - e_ky, k), in FP64.  GlobalPull reads a
+// population plane; the two-step pass reads its LDS ring instead.  Both
+// feed the same arithmetic.
+template <typename T>
+struct GlobalPull {
+    const GAS T* src;
+    long long N;
+    int nx;
+    const double* W;
+    __device__ __forceinline__ void operator()(int i, int j, double* f) const
+    {
+        pull9(src, N, nx, (long long)j * nx + i, f, W);
+    }
+};
+
 // Edge state of column `col` (0: left, 1: right) at interior row j: the
 // characteristic-updated rho/u/v and the neighbour's f0 and moments.
-template <typename T>
-__device__ __forceinline__ void edge_column_state(const DevCase& D, const GAS T* src, double cs, int col, int j,
-                                  double& ro, double& uo, double& vo, double* f0n, double& rn,
-                                  double& un, double& vn)
+template <typename PULL>
+__device__ __forceinline__ void edge_column_state(const DevCase& D, const PULL& pull, double cs, int col, int j,
+                                                  double& ro, double& uo, double& vo, double* f0n, double& rn,
+                                                  double& un, double& vn)
 {
     const int nx = D.nx;
-    const long long N = D.N;
     const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
     double tr, tu, tv;
     ring_moments(f0c, D.ny, j, tr, tu, tv);
     int i2 = col == 0 ? 1 : nx - 2;
     int i3 = col == 0 ? 2 : nx - 3;
     double f3[9], r3, u3, v3;
-    pull9(src, N, nx, (long long)j * nx + i2, f0n, D.W);
+    pull(i2, j, f0n);
     moments(f0n, rn, un, vn);
-    pull9(src, N, nx, (long long)j * nx + i3, f3, D.W);
+    pull(i3, j, f3);
     moments(f3, r3, u3, v3);
     if (col == 0)
         left_char(cs, tr, tu, tv, rn, un, vn, r3, u3, v3, ro, uo, vo);
@@ -126,120 +149,150 @@ __device__ __forceinline__ void edge_column_state(const DevCase& D, const GAS T*
 }
 
 // boundary_condition1 (boundary_condition.f90:288-858) with no periodic
-// edges, for ring thread t.  Threads 0..ny-3 take the left and right columns
-// at rows 1..ny-2; threads ny-2..ny-2+nx-1 take the top and bottom rows at
-// column x.  The top and bottom characteristic updates and corner states of
-// the reference are overwritten by rho = rho(.,Ny-1), u = v = 0 before use
-// (658-661, 785-788), so only that effective state is computed.  Corner
-// columns of the walls read the left/right results at rows 1 and ny-2,
-// recomputed here.  Reads only the previous populations and the stale ring
-// f0, so it runs beside the interior update.
+// edges at ring node (i, j): the left and right columns at rows 1..ny-2 take
+// the characteristic state and the NEE from the next column; the top and
+// bottom rows take rho of the next row at rest (the reference's top and
+// bottom characteristic updates and corner states are overwritten by rho =
+// rho(.,Ny-1), u = v = 0 before use, 658-661 and 785-788, so only that
+// effective state is computed), the corners reading the left/right results
+// at rows 1 and ny-2, recomputed here.  Reads only the previous populations
+// and the stale ring f0.
+template <typename PULL>
+__device__ __forceinline__ void ring_node(const DevCase& D, const PULL& pull, int i, int j, double* out, double& ro,
+                                          double& uo, double& vo)
+{
+    const int nx = D.nx, ny = D.ny;
+    if (j >= 1 && j <= ny - 2) {
+        double f0n[9], rn, un, vn;
+        edge_column_state(D, pull, D.cs, i == 0 ? 0 : 1, j, ro, uo, vo, f0n, rn, un, vn);
+        nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
+        return;
+    }
+    // top wall (row ny-1, neighbour ny-2) or bottom wall (row 0, neighbour 1)
+    const int jn = j == 0 ? 1 : ny - 2;
+    double rn, un, vn, f0n[9];
+    if (i == 0 || i == nx - 1) {
+        const int col = i == 0 ? 0 : 1;
+        double fdummy[9], r2, u2, v2;
+        edge_column_state(D, pull, D.cs, col, jn, rn, un, vn, fdummy, r2, u2, v2);
+        const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f0n[k] = f0c[k * ny + jn];
+    } else {
+        pull(i, jn, f0n);
+        moments(f0n, rn, un, vn);
+    }
+    ro = rn;
+    uo = 0.0;
+    vo = 0.0;
+    nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
+}
+
+// Ring thread t of a lattice update: threads 0..ny-3 take the left and right
+// columns at rows 1..ny-2, threads ny-2..ny-2+nx-1 the top and bottom rows at
+// column t-(ny-2).  Runs beside the interior update.
 template <typename T>
-__device__ __forceinline__ void ring_thread(const DevCase& D, int cur, double cs, double* __restrict__ macro, int t)
+__device__ __forceinline__ void ring_thread(const DevCase& D, int src, int dst, double* __restrict__ macro, int t)
 {
     const int nx = D.nx, ny = D.ny;
     const long long N = D.N;
-    const GAS T* src = pops<T>(D, cur ^ 1);
-    GAS T* dst = pops<T>(D, cur);
+    const GlobalPull<T> pull{pops<T>(D, src), N, nx, D.W};
+    GAS T* out_p = pops<T>(D, dst);
+    int ii[2], jj[2];
     if (t < ny - 2) {
-        int j = t + 1;
-        for (int col = 0; col < 2; ++col) {
-            double ro, uo, vo, f0n[9], rn, un, vn, out[9];
-            edge_column_state(D, src, cs, col, j, ro, uo, vo, f0n, rn, un, vn);
-            nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
-            int i = col == 0 ? 0 : nx - 1;
-            long long c = (long long)j * nx + i;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], dst + k * N + c);
-            if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
-        }
-        return;
+        ii[0] = 0; ii[1] = nx - 1; jj[0] = jj[1] = t + 1;
+    } else {
+        t -= ny - 2;
+        if (t >= nx) return;
+        ii[0] = ii[1] = t; jj[0] = ny - 1; jj[1] = 0;
     }
-    t -= ny - 2;
-    if (t >= nx) return;
-    int x = t;
-    for (int wall = 0; wall < 2; ++wall) {
-        // wall 0: top (row ny-1, neighbour ny-2); wall 1: bottom (row 0, neighbour 1)
-        int jn = wall == 0 ? ny - 2 : 1;
-        int je = wall == 0 ? ny - 1 : 0;
-        double rn, un, vn, f0n[9];
-        if (x == 0 || x == nx - 1) {
-            int col = x == 0 ? 0 : 1;
-            double fdummy[9], r2, u2, v2;
-            edge_column_state(D, src, cs, col, jn, rn, un, vn, fdummy, r2, u2, v2);
-            const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
+    for (int q = 0; q < 2; ++q) {
+        double out[9], ro, uo, vo;
+        ring_node(D, pull, ii[q], jj[q], out, ro, uo, vo);
+        long long c = (long long)jj[q] * nx + ii[q];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) f0n[k] = f0c[k * ny + jn];
-        } else {
-            pull9(src, N, nx, (long long)jn * nx + x, f0n, D.W);
-            moments(f0n, rn, un, vn);
-        }
-        double re = rn, ue = 0.0, ve = 0.0, out[9];
-        nee_node(re, ue, ve, rn, un, vn, f0n, D.R, out);
-        long long c = (long long)je * nx + x;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], dst + k * N + c);
-        if (macro) { macro[c] = re; macro[N + c] = ue; macro[2 * N + c] = ve; }
+        for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], out_p + k * N + c);
+        if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
     }
 }
 
-// One interior node (i, j) of the lattice update prev -> cur: fused
-// streaming + physical_variables + colission (solution.f90:428-643).  The IB
-// nodes of the step (slot g) are skipped: k_band collides them with their
-// forcing, concurrently.  Everywhere else gforce is zero and the force
-// moments vanish exactly.  Optionally records rho/u/v (download / residual
+// Fused streaming + physical_variables + colission (solution.f90:428-643) of
+// one interior node without forcing, from its pulled populations p (the
+// stored type: FP64 populations, or FP32 deviations g = f - W) into out;
+// rho/u/v of the step as well.  gforce is zero away from the IB nodes, so
+// the force moments vanish exactly.
+__device__ __forceinline__ void interior_step(const DevCase& D, const double* p, double* out, double& rho,
+                                              double& u, double& v)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k] = p[k];
+    moments(out, rho, u, v);
+    collide_node(out, rho, u, v, D.R, false, 0.0, 0.0);
+}
+__device__ __forceinline__ void interior_step(const DevCase& D, const float* p, float* out, double& rho, double& u,
+                                              double& v)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k] = p[k];
+    float drho, jx, jy;
+    collide_dev_f32(out, D.K, drho, jx, jy);
+    rho = D.rho0 + (double)drho;
+    u = (double)jx / rho;
+    v = (double)jy / rho;
+}
+
+// The nine stored populations streamed into node c = j*nx + i of plane src.
+template <typename T>
+__device__ __forceinline__ void pull_raw(const GAS T* __restrict__ src, long long N, int nx, long long c, T* g)
+{
+    g[0] = src[c];
+    g[1] = src[1 * N + c - 1];
+    g[2] = src[2 * N + c - nx];
+    g[3] = src[3 * N + c + 1];
+    g[4] = src[4 * N + c + nx];
+    g[5] = src[5 * N + c - nx - 1];
+    g[6] = src[6 * N + c - nx + 1];
+    g[7] = src[7 * N + c + nx + 1];
+    g[8] = src[8 * N + c + nx - 1];
+}
+
+// Is (i, j) one of step slot g's IB nodes?  (Only nodes inside the lattice
+// window of the run can be.)
+__device__ __forceinline__ bool is_ib_node(const DevCase& D, int g, int i, int j)
+{
+    const unsigned wi = (unsigned)(i - D.gi0), wj = (unsigned)(j - D.gj0);
+    if (wi < (unsigned)D.gw && wj < (unsigned)D.gh) {
+        const GAS int* nodeidx = (const GAS int*)(D.ipool + (long long)g * D.io.total + D.io.nodeidx);
+        return nodeidx[(long long)wj * D.gw + wi] >= 0;
+    }
+    return false;
+}
+
+// One interior node (i, j) of the lattice update src -> dst.  The IB nodes
+// of the step (slot g) are skipped: k_band collides them with their
+// forcing, concurrently.  Optionally records rho/u/v (download / residual
 // support).
 template <typename T>
-__device__ __forceinline__ void lattice_node(const DevCase& D, int g, int cur, double* __restrict__ macro,
+__device__ __forceinline__ void lattice_node(const DevCase& D, int g, int src, int dst, double* __restrict__ macro,
                                              int i, int j)
 {
     const int nx = D.nx;
     const long long N = D.N;
     if (i < 1 || i > nx - 2) return;
-    const unsigned wi = (unsigned)(i - D.gi0), wj = (unsigned)(j - D.gj0);
-    if (wi < (unsigned)D.gw && wj < (unsigned)D.gh) {
-        const GAS int* nodeidx = (const GAS int*)(D.ipool + (long long)g * D.io.total + D.io.nodeidx);
-        if (nodeidx[(long long)wj * D.gw + wi] >= 0) return;
-    }
+    if (is_ib_node(D, g, i, j)) return;
     const long long c = (long long)j * nx + i;
-    const GAS T* __restrict__ src = pops<T>(D, cur ^ 1);
-    GAS T* __restrict__ dst = pops<T>(D, cur);
-    if constexpr (sizeof(T) == 8) {
-        double f[9];
-        pull9(src, N, nx, c, f, D.W);
-        double rho, u, v;
-        moments(f, rho, u, v);
-        if (macro) {
-            macro[c] = rho;
-            macro[N + c] = u;
-            macro[2 * N + c] = v;
-        }
-        collide_node(f, rho, u, v, D.R, false, 0.0, 0.0);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(f[k], dst + k * N + c);
-    } else {
-        // FP32 deviation form: pull, collide and store g = f - W in FP32
-        float g[9];
-        g[0] = src[c];
-        g[1] = src[1 * N + c - 1];
-        g[2] = src[2 * N + c - nx];
-        g[3] = src[3 * N + c + 1];
-        g[4] = src[4 * N + c + nx];
-        g[5] = src[5 * N + c - nx - 1];
-        g[6] = src[6 * N + c - nx + 1];
-        g[7] = src[7 * N + c + nx + 1];
-        g[8] = src[8 * N + c + nx - 1];
-        float drho, jx, jy;
-        collide_dev_f32(g, D.K, drho, jx, jy);
-        if (macro) {
-            const double rho = D.rho0 + (double)drho;
-            macro[c] = rho;
-            macro[N + c] = (double)jx / rho;
-            macro[2 * N + c] = (double)jy / rho;
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(g[k], dst + k * N + c);
+    T p[9], out[9];
+    pull_raw(pops<T>(D, src), N, nx, c, p);
+    double rho, u, v;
+    interior_step(D, p, out, rho, u, v);
+    if (macro) {
+        macro[c] = rho;
+        macro[N + c] = u;
+        macro[2 * N + c] = v;
     }
+    GAS T* __restrict__ o = pops<T>(D, dst);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], o + k * N + c);
 }
 
 // One lattice update of every case of the window: the interior nodes and
@@ -257,7 +310,7 @@ __device__ __forceinline__ void lattice_node(const DevCase& D, int g, int cur, d
 // launch shape's choice; the result does not depend on it.
 template <typename T, int NPT>
 __global__ __launch_bounds__(256) void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                 const int2* __restrict__ tiles, int ntiles, int off)
+                                                 const int2* __restrict__ tiles, int ntiles, int off, int psel)
 {
     for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
         // the tile's case and block are uniform over the workgroup: keep them
@@ -269,11 +322,12 @@ __global__ __launch_bounds__(256) void k_lattice(const DevCase* __restrict__ C, 
         const CaseClock kc = K[cix];
         const int n = kc.n0 + off;
         if (n > kc.n_end) continue;
-        const int cur = n & 1;
-        double* macro = n >= kc.flag_from ? D.macro[cur] : nullptr;
+        int src, dst;
+        step_planes(D, n, psel, src, dst);
+        double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
         const int rb = lattice_ring_blocks(D.nx, D.ny);
         if (blk < rb) {
-            ring_thread<T>(D, cur, D.cs, macro, blk * 256 + threadIdx.x);
+            ring_thread<T>(D, src, dst, macro, blk * 256 + threadIdx.x);
             continue;
         }
         const int q = blk - rb;
@@ -288,14 +342,230 @@ __global__ __launch_bounds__(256) void k_lattice(const DevCase* __restrict__ C, 
                 i -= nx;
                 ++j;
             }
-            if (j >= 1 && j <= D.ny - 2) lattice_node<T>(D, g, cur, macro, i, j);
+            if (j >= 1 && j <= D.ny - 2) lattice_node<T>(D, g, src, dst, macro, i, j);
         }
     }
 }
-template __global__ void k_lattice<double, 1>(const DevCase*, const CaseClock*, const int2*, int, int);
-template __global__ void k_lattice<double, 2>(const DevCase*, const CaseClock*, const int2*, int, int);
-template __global__ void k_lattice<float, 1>(const DevCase*, const CaseClock*, const int2*, int, int);
-template __global__ void k_lattice<float, 2>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_lattice<double, 1>(const DevCase*, const CaseClock*, const int2*, int, int, int);
+template __global__ void k_lattice<double, 2>(const DevCase*, const CaseClock*, const int2*, int, int, int);
+template __global__ void k_lattice<float, 1>(const DevCase*, const CaseClock*, const int2*, int, int, int);
+template __global__ void k_lattice<float, 2>(const DevCase*, const CaseClock*, const int2*, int, int, int);
+
+// One step of the nodes of the lattice window W (the rectangle gi0.. x
+// gj0.. holding every IB node of the run) but its IB nodes, for an engine of
+// one case whose far field takes two steps per pass (k_far2).  Thread t of
+// the grid takes node t of W in row order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                int off, int psel)
+{
+    const DevCase& D = C[0];
+    const CaseClock kc = K[0];
+    const int n = kc.n0 + off;
+    if (n > kc.n_end) return;
+    int src, dst;
+    step_planes(D, n, psel, src, dst);
+    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)D.gw * D.gh) return;
+    const int i = D.gi0 + (int)(t % D.gw);
+    const int j = D.gj0 + (int)(t / D.gw);
+    lattice_node<T>(D, (n - 1) % D.nslot, src, dst, macro, i, j);
+}
+template __global__ void k_window<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_window<float>(const DevCase*, const CaseClock*, int, int);
+
+// Step n of the outer ring alone, src -> dst (the middle plane of a two-step
+// pass, which then reads the ring's step n there instead of recomputing it
+// in phase 1 one row at a time); records rho/u/v when step n is recorded.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
+                                             int psel)
+{
+    const DevCase& D = C[0];
+    const CaseClock kc = K[0];
+    const int n = kc.n0 + off;
+    if (n > kc.n_end) return;
+    int src, dst;
+    step_planes(D, n, psel, src, dst);
+    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
+    ring_thread<T>(D, src, dst, macro, (int)(blockIdx.x * 256 + threadIdx.x));
+}
+template __global__ void k_ring<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_ring<float>(const DevCase*, const CaseClock*, int, int);
+
+// ---------------------------------------------------------------------------
+// Two steps of the far field in one pass
+//
+// An engine of one case may take its steps in pairs (n, n+1): the nodes of
+// the window W (every IB node of the run lies at least 3 nodes inside it)
+// advance one step at a time with the IB chain between the steps (k_window,
+// k_band), and every other node -- the far field F, the outer ring included
+// -- takes both steps in one pass here, reading each population of step n-1
+// once and writing each of step n+1 once: 144 (FP64) or 72 (FP32) bytes per
+// node for two steps.
+//
+// A workgroup of FAR_LANES lanes owns a strip of output columns [a, b)
+// (b - a <= FAR_OUT) and a segment of output rows [j0, j1]; lane L stands at
+// column i = a - 1 + L.  It sweeps the rows upwards.  At row r it forms step
+// n at (i, r) -- phase 1: pull from plane src, moments, collide, or the
+// ring's update -- into slot r & 3 of a 4-row LDS ring, then, after one
+// workgroup barrier, step n+1 at (i, r-1) from the ring (phase 2) for the
+// lanes of the strip's columns.  The bottom wall row pulls rows 0..2, so it
+// takes its step one row later, with row 1; the top wall row pulls rows
+// ny-3..ny-1, all in the ring when it is reached.  A ring slot is written
+// again four rows later, after every lane has passed the barrier of the
+// row in between, so one barrier per row orders all LDS traffic.
+//
+// Every node keeps the single-step arithmetic (the same functions, the same
+// operations in the same order), so two steps here equal two single steps
+// bit for bit.  Phase 1 skips W's nodes but its outer layer (phase 2 pulls
+// only that layer of W); it writes step n of F's layer around W into the
+// middle plane (the window's second step pulls it there), and both phases
+// record rho/u/v of their step when it is recorded.  Each of these writes
+// comes from the workgroup whose output range holds the node.
+//
+// psel packs the planes: src = psel & 3, dst = (psel >> 2) & 3, middle =
+// (psel >> 4) & 3.
+constexpr int FAR_SLOTS = 4;
+size_t far2_smem_bytes(bool f64) { return (size_t)FAR_SLOTS * 9 * FAR_LANES * (f64 ? 8 : 4); }
+
+template <typename T>
+struct LdsPull {
+    const T* ring;   // [FAR_SLOTS][9][FAR_LANES]
+    int c0;          // column of lane 0
+    const double* W;
+    __device__ __forceinline__ void operator()(int i, int j, double* f) const
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int ex = k == 1 || k == 5 || k == 8 ? 1 : (k == 3 || k == 6 || k == 7 ? -1 : 0);
+            const int ey = k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 || k == 8 ? -1 : 0);
+            const int sl = (j - ey) & (FAR_SLOTS - 1);
+            f[k] = lift(ring[(sl * 9 + k) * FAR_LANES + (i - ex - c0)], W[k]);
+        }
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                    int off, int psel)
+{
+    extern __shared__ __align__(16) unsigned char far_smem[];
+    T* ring = reinterpret_cast<T*>(far_smem);
+    const DevCase& D = C[0];
+    const CaseClock kc = K[0];
+    const int n = kc.n0 + off;  // the pair's first step
+    if (n + 1 > kc.n_end) return;
+    const int src = psel & 3, dst = (psel >> 2) & 3, mid = (psel >> 4) & 3;
+    const int nx = D.nx, ny = D.ny;
+    const long long N = D.N;
+    const int a = (int)(((long long)blockIdx.x * nx) / gridDim.x);
+    const int b = (int)(((long long)(blockIdx.x + 1) * nx) / gridDim.x);
+    const int j0 = (int)(((long long)blockIdx.y * ny) / gridDim.y);
+    const int j1 = (int)(((long long)(blockIdx.y + 1) * ny) / gridDim.y) - 1;
+    const int L = (int)threadIdx.x;
+    const int c0 = a - 1;
+    const int i = c0 + L;
+    const int wi0 = D.gi0, wi1 = D.gi0 + D.gw - 1, wj0 = D.gj0, wj1 = D.gj0 + D.gh - 1;
+    const bool col_in = i >= 0 && i < nx;
+    const bool own_col = i >= a && i < b;
+    double* mac1 = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
+    double* mac2 = n + 1 >= kc.flag_from ? D.macro[(n + 1) & 1] : nullptr;
+    const LdsPull<T> lpull{ring, c0, D.W};
+    GAS T* __restrict__ out2 = pops<T>(D, dst);
+    GAS T* __restrict__ outm = pops<T>(D, mid);
+
+    // phase 2 at output row q for this lane (a strip column)
+    auto phase2 = [&](int q) {
+        if (!own_col) return;
+        if (i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1) return;
+        const long long c = (long long)q * nx + i;
+        double rho, u, v;
+        if (i == 0 || i == nx - 1 || q == 0 || q == ny - 1) {
+            double out[9];
+            ring_node(D, lpull, i, q, out, rho, u, v);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], out2 + k * N + c);
+        } else {
+            T p[9], out[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int ex = k == 1 || k == 5 || k == 8 ? 1 : (k == 3 || k == 6 || k == 7 ? -1 : 0);
+                const int ey = k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 || k == 8 ? -1 : 0);
+                p[k] = ring[(((q - ey) & (FAR_SLOTS - 1)) * 9 + k) * FAR_LANES + (L - ex)];
+            }
+            interior_step(D, p, out, rho, u, v);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], out2 + k * N + c);
+        }
+        if (mac2) { mac2[c] = rho; mac2[N + c] = u; mac2[2 * N + c] = v; }
+    };
+
+    // Phase 1 of row r at this lane: 0 nothing, 1 an interior node, 2 a ring
+    // node, whose step n k_ring has put into the middle plane.  The loads of
+    // both are issued one row ahead, so they overlap the barrier and phase 2
+    // of the row before.
+    auto kind1 = [&](int r) -> int {
+        if (r < 0 || r >= ny || !col_in) return 0;
+        const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
+        const bool w_edge = in_w && (i == wi0 || i == wi1 || r == wj0 || r == wj1);
+        if (in_w && !w_edge) return 0;
+        return (i == 0 || i == nx - 1 || r == 0 || r == ny - 1) ? 2 : 1;
+    };
+    const GAS T* __restrict__ srcp = pops<T>(D, src);
+    T pre[9];
+    auto load1 = [&](int kind, int r) {
+        const long long c = (long long)r * nx + i;
+        if (kind == 1) {
+            pull_raw(srcp, N, nx, c, pre);
+        } else if (kind == 2) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pre[k] = outm[k * N + c];
+        }
+    };
+    int kpre = kind1(j0 - 1);
+    load1(kpre, j0 - 1);
+    for (int r = j0 - 1; r <= j1 + 1; ++r) {
+        const int kr = kpre;
+        T p[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) p[k] = pre[k];
+        kpre = r + 1 <= j1 + 1 ? kind1(r + 1) : 0;
+        load1(kpre, r + 1);
+        // phase 1: step n at (i, r)
+        if (kr != 0) {
+            const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
+            {
+                const long long c = (long long)r * nx + i;
+                T* slot = ring + (size_t)(r & (FAR_SLOTS - 1)) * 9 * FAR_LANES + L;
+                T st[9];
+                double rho = 0.0, u = 0.0, v = 0.0;
+                if (kr == 2) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) st[k] = p[k];
+                } else {
+                    interior_step(D, p, st, rho, u, v);
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) slot[k * FAR_LANES] = st[k];
+                const bool own = own_col && r >= j0 && r <= j1 && !in_w && kr == 1;
+                if (own && i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // F's layer around W, step n
+                }
+                if (own && mac1) { mac1[c] = rho; mac1[N + c] = u; mac1[2 * N + c] = v; }
+            }
+        }
+        lds_barrier();  // the next row's pulls stay in flight across it
+        // phase 2: step n+1 at row r-1 (the bottom wall row with row 1)
+        const int q = r - 1;
+        if (q >= j0 && q <= j1 && q != 0) phase2(q);
+        if (q == 1 && j0 == 0) phase2(0);
+    }
+}
+template __global__ void k_far2<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_far2<float>(const DevCase*, const CaseClock*, int, int);
 
 int lattice_tiles(int nx, int ny, int npt)
 {
@@ -345,12 +615,6 @@ __device__ __forceinline__ void axis_box(double X, double o, double sc, int n, i
 // Per-wing forces of a batch of steps: k_forces.
 // ---------------------------------------------------------------------------
 
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS
-// operations, not for its outstanding global stores.
-__device__ __forceinline__ void lds_barrier()
-{
-    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 __device__ __forceinline__ bool box_cell(const int* b, int q, int& i, int& j)
 {
@@ -987,7 +1251,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 // node at the standard marker spacing).
 template <typename T>
 __global__ __launch_bounds__(256) void k_crit_ustar(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                    int off)
+                                                    int off, int psel)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const DevCase& D = C[blockIdx.y];
@@ -998,13 +1262,15 @@ __global__ __launch_bounds__(256) void k_crit_ustar(const DevCase* __restrict__ 
     if (a >= *S.nadj) return;
     const long long c = (long long)S.adj[3 * a + 1] * D.nx + S.adj[3 * a];
     double f[9], r, u, v;
-    pull9(pops<T>(D, (nstep & 1) ^ 1), D.N, D.nx, c, f, D.W);
+    int src, dst;
+    step_planes(D, nstep, psel, src, dst);
+    pull9(pops<T>(D, src), D.N, D.nx, c, f, D.W);
     moments(f, r, u, v);
     S.ust[a] = u;
     S.vst[a] = v;
 }
-template __global__ void k_crit_ustar<double>(const DevCase*, const CaseClock*, int);
-template __global__ void k_crit_ustar<float>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_ustar<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_crit_ustar<float>(const DevCase*, const CaseClock*, int, int);
 
 // Interpolation D u* (the DGEMVs, boundary_condition.f90:1620-1621) and the
 // right-hand side U_B - D u* (1624-1629) in zig-zag order: one thread per
@@ -1047,13 +1313,14 @@ __global__ __launch_bounds__(256) void k_crit_rhs(const DevCase* __restrict__ C,
 constexpr int RHS_SEG = 7;  // segments per 256-thread block (7 * 36 = 252)
 template <typename T>
 __global__ __launch_bounds__(256) void k_crit_rhs1(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                  int off)
+                                                  int off, int psel)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const DevCase& D = C[blockIdx.y];
     const int nstep = K[blockIdx.y].n0 + off;
     if (nstep > K[blockIdx.y].n_end || (int)blockIdx.x * RHS_SEG >= D.nseg) return;
-    const int cur = nstep & 1;
+    int src, dst;
+    step_planes(D, nstep, psel, src, dst);
     const Slot S = slot_of(D, (nstep - 1) % D.nslot);
     __shared__ double pu[RHS_SEG][BOXMAX], pv[RHS_SEG][BOXMAX];
     const int sl = threadIdx.x / BOXMAX, q = threadIdx.x % BOXMAX;
@@ -1065,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_crit_rhs1(const DevCase* __restrict__ C
         double w = S.dval[s * BOXMAX + q];
         long long c = (long long)S.adj[3 * node + 1] * D.nx + S.adj[3 * node];
         double f[9], r, u, v;
-        pull9(pops<T>(D, cur ^ 1), D.N, D.nx, c, f, D.W);
+        pull9(pops<T>(D, src), D.N, D.nx, c, f, D.W);
         moments(f, r, u, v);
         pu[sl][q] = w * u;
         pv[sl][q] = w * v;
@@ -1079,8 +1346,8 @@ __global__ __launch_bounds__(256) void k_crit_rhs1(const DevCase* __restrict__ C
         D.rhs[D.nseg + p] = S.vb[s] - sv;
     }
 }
-template __global__ void k_crit_rhs1<double>(const DevCase*, const CaseClock*, int);
-template __global__ void k_crit_rhs1<float>(const DevCase*, const CaseClock*, int);
+template __global__ void k_crit_rhs1<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_crit_rhs1<float>(const DevCase*, const CaseClock*, int, int);
 
 // Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
 // right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
@@ -1555,7 +1822,8 @@ size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
 // (solution.f90:598-638) of the IB nodes, which k_lattice leaves out;
 // the two run concurrently.
 template <typename T>
-__device__ __forceinline__ void band_node(const DevCase& D, const Slot& S, int a, int cur, double* __restrict__ macro)
+__device__ __forceinline__ void band_node(const DevCase& D, const Slot& S, int a, int src, int dst,
+                                          double* __restrict__ macro)
 {
     const int cnt = min(S.nmkcnt[a], NCAP);
     const int* km = S.nmk + (long long)a * NCAP;
@@ -1564,7 +1832,7 @@ __device__ __forceinline__ void band_node(const DevCase& D, const Slot& S, int a
     const int* ad = S.adj + 3 * a;
     long long c = (long long)ad[1] * D.nx + ad[0];
     double f[9], r, us, vs;
-    pull9(pops<T>(D, cur ^ 1), D.N, D.nx, c, f, D.W);
+    pull9(pops<T>(D, src), D.N, D.nx, c, f, D.W);
     double gx = 0.0, gy = 0.0;
     for (int q = 0; q < cnt; ++q) {
         int s = km[q];
@@ -1582,29 +1850,31 @@ __device__ __forceinline__ void band_node(const DevCase& D, const Slot& S, int a
     double gfy = 2.0 * r * gy;
     bool forced = (gfx != 0.0) || (gfy != 0.0);
     collide_node(f, r, u, v, D.R, forced, gfx, gfy);
-    GAS T* dst = pops<T>(D, cur);
+    GAS T* o = pops<T>(D, dst);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) lower(f[k], D.W[k], dst + k * D.N + c);
+    for (int k = 0; k < 9; ++k) lower(f[k], D.W[k], o + k * D.N + c);
     if (macro) { macro[c] = r; macro[D.N + c] = u; macro[2 * D.N + c] = v; }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_band(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off)
+__global__ __launch_bounds__(256) void k_band(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
+                                              int psel)
 {
     __builtin_amdgcn_s_setprio(3);  // critical path: ahead of co-resident lattice waves
     const DevCase& D = C[blockIdx.y];
     const CaseClock kc = K[blockIdx.y];
     const int n = kc.n0 + off;
     if (n > kc.n_end) return;
-    const int cur = n & 1;
-    double* macro = n >= kc.flag_from ? D.macro[cur] : nullptr;
+    int src, dst;
+    step_planes(D, n, psel, src, dst);
+    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
     const Slot S = slot_of(D, (n - 1) % D.nslot);
     const int na = *S.nadj;
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x)
-        band_node<T>(D, S, a, cur, macro);
+        band_node<T>(D, S, a, src, dst, macro);
 }
-template __global__ void k_band<double>(const DevCase*, const CaseClock*, int);
-template __global__ void k_band<float>(const DevCase*, const CaseClock*, int);
+template __global__ void k_band<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_band<float>(const DevCase*, const CaseClock*, int, int);
 
 // Per-wing force and moment, accumulated in node order
 // (boundary_condition.f90:1662-1675), and their normalisation

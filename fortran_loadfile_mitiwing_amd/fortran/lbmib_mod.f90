@@ -22,7 +22,7 @@ module lbmib
     integer(c_int32_t), parameter, public :: LBMIB_E_NAN = -7
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP64 = 0
     integer(c_int32_t), parameter, public :: LBMIB_PREC_FP32DEV = 1
-    integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 11
+    integer(c_int32_t), parameter, public :: LBMIB_NTIMINGS = 16
     integer(c_int32_t), parameter, public :: LBMIB_GEOMETRY_BATCH = 16
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_AUTO = 0
     integer(c_int32_t), parameter, public :: LBMIB_SOLVER_CHAIN = 1
@@ -53,13 +53,14 @@ module lbmib
         integer(c_int32_t) :: nodes_per_thread = 1, tile_order = 0, persistent = 0
         integer(c_int32_t) :: ib_cus = 16, graph = 0, ib_collide_shared = 0, ib_rhs_reserved = 1
         integer(c_int32_t) :: solve_unit = 0
-        integer(c_int32_t) :: reserved(1) = 0
+        integer(c_int32_t) :: fuse_steps = 1
     end type
 
     public :: lbmib_create, lbmib_set_motion_table, lbmib_step, lbmib_steps_done, lbmib_max_steps
     public :: lbmib_get_forces, lbmib_get_history, lbmib_download, lbmib_upload_state
     public :: lbmib_residual, lbmib_ib_nodes, lbmib_set_profiling, lbmib_get_timings, lbmib_set_step
     public :: lbmib_markers, lbmib_vorticity, lbmib_surface_pressure, lbmib_surface_pressure_noalloc
+    public :: lbmib_lattice_window
     ! rigid-body dynamics
     public :: lbmib_dynamics, lbmib_set_dynamics, lbmib_get_body_state, lbmib_set_body_state
     public :: lbmib_get_motion_rows, lbmib_get_edge_f0, lbmib_set_edge_f0
@@ -189,6 +190,12 @@ module lbmib
 
         ! no counterpart in the reference: the same, staged through the
         ! library's fixed per-device buffer (for callers that cannot allocate)
+        integer(c_int) function lbmib_lattice_window(h, i0, j0, w, hgt) bind(C, name="lbmib_lattice_window")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(out) :: i0, j0, w, hgt
+        end function
+
         integer(c_int) function lbmib_surface_pressure_noalloc(h, p, n) bind(C, name="lbmib_surface_pressure_noalloc")
             import :: c_int, c_int32_t, c_double, c_ptr
             type(c_ptr), value :: h

@@ -42,7 +42,7 @@ GEOMETRY_BATCH = 16  # steps per geometry batch (LBMIB_GEOMETRY_BATCH)
 SOLVER_AUTO = 0       # LBMIB_SOLVER_*: chain for FP64, separator form for FP32
 SOLVER_CHAIN = 1
 SOLVER_SEPARATOR = 2
-NTIMINGS = 11  # LBMIB_NTIMINGS
+NTIMINGS = 16  # LBMIB_NTIMINGS
 
 
 class LBMIBError(RuntimeError):
@@ -69,19 +69,20 @@ class DynamicsParams(C.Structure):
 class LaunchShape(C.Structure):
     _fields_ = [("nodes_per_thread", C.c_int32), ("tile_order", C.c_int32), ("persistent", C.c_int32),
                 ("ib_cus", C.c_int32), ("graph", C.c_int32), ("ib_collide_shared", C.c_int32),
-                ("ib_rhs_reserved", C.c_int32), ("solve_unit", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("ib_rhs_reserved", C.c_int32), ("solve_unit", C.c_int32), ("fuse_steps", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k in ("nodes_per_thread", "tile_order", "persistent", "ib_cus", "graph",
-                                                    "ib_collide_shared", "ib_rhs_reserved", "solve_unit")}
+                                                    "ib_collide_shared", "ib_rhs_reserved", "solve_unit",
+                                                    "fuse_steps")}
 
 
 def make_shape(nodes_per_thread=1, tile_order=0, persistent=0, ib_cus=16, graph=0, ib_collide_shared=0,
-               ib_rhs_reserved=1, solve_unit=0) -> LaunchShape:
+               ib_rhs_reserved=1, solve_unit=0, fuse_steps=1) -> LaunchShape:
     s = LaunchShape()
     s.nodes_per_thread, s.tile_order, s.persistent = nodes_per_thread, tile_order, persistent
     s.ib_cus, s.graph, s.ib_collide_shared = ib_cus, graph, ib_collide_shared
-    s.ib_rhs_reserved, s.solve_unit = ib_rhs_reserved, solve_unit
+    s.ib_rhs_reserved, s.solve_unit, s.fuse_steps = ib_rhs_reserved, solve_unit, fuse_steps
     return s
 
 
@@ -93,7 +94,7 @@ EXPORTS = ["lbmib_create", "lbmib_set_motion_table", "lbmib_step", "lbmib_steps_
            "lbmib_max_steps", "lbmib_get_forces", "lbmib_get_history", "lbmib_download",
            "lbmib_upload_state", "lbmib_set_step", "lbmib_markers", "lbmib_vorticity", "lbmib_surface_pressure",
            "lbmib_surface_pressure_noalloc", "lbmib_set_dynamics", "lbmib_get_body_state",
-           "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_get_edge_f0", "lbmib_set_edge_f0", "lbmib_residual", "lbmib_ib_nodes", "lbmib_set_profiling",
+           "lbmib_set_body_state", "lbmib_get_motion_rows", "lbmib_get_edge_f0", "lbmib_set_edge_f0", "lbmib_residual", "lbmib_ib_nodes", "lbmib_lattice_window", "lbmib_set_profiling",
            "lbmib_get_timings", "lbmib_destroy", "lbmib_last_error", "lbmib_version",
            "lbmib_set_shape", "lbmib_get_shape", "lbmib_batch_create", "lbmib_batch_add",
            "lbmib_batch_remove", "lbmib_batch_size", "lbmib_batch_step", "lbmib_batch_set_shape",
@@ -130,6 +131,7 @@ def load(path: str = LIB_PATH):
     L.lbmib_vorticity.argtypes = [vp, C.c_int32, _dp, _dp]
     L.lbmib_surface_pressure.argtypes = [vp, _dp, C.c_int32]
     L.lbmib_surface_pressure_noalloc.argtypes = [vp, _dp, C.c_int32]
+    L.lbmib_lattice_window.argtypes = [vp, _ip, _ip, _ip, _ip]
     L.lbmib_residual.argtypes = [vp, _dp]
     L.lbmib_ib_nodes.argtypes = [vp, _ip, _ip, _ip, _ip, _dp, _dp]
     L.lbmib_set_profiling.argtypes = [vp, C.c_int32]
@@ -351,6 +353,13 @@ class LBMIB:
                                 w.ctypes.data_as(_ip), _d(gx), _d(gy)))
         return dict(i=pi, j=pj, wing=w, glx=gx, gly=gy)
 
+    def lattice_window(self):
+        """(i0, j0, width, height) of the lattice window holding every IB node
+        of the run (0-based node indices)."""
+        v = [C.c_int32() for _ in range(4)]
+        _check(load().lbmib_lattice_window(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def set_profiling(self, every: int = 1):
         """Record phase timings on every ``every``-th step (0: off); resets them."""
         _check(load().lbmib_set_profiling(self.h, int(every)))
@@ -379,10 +388,12 @@ class LBMIB:
 
 def _timings(t) -> dict:
     ns, nb = max(t[2], 1.0), max(t[8], 1.0)
-    return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / ns, ib_solve_ms=t[4] / ns,
+    return dict(step_ms=t[0] / max(t[1], 1.0), lattice_ms=t[3] / max(t[15], 1.0), ib_solve_ms=t[4] / ns,
                 ib_collide_ms=t[5] / ns, geometry_batch_ms=t[6] / nb, factor_batch_ms=t[7] / nb,
                 steps=int(t[1]), sampled_steps=int(t[2]), batches=int(t[8]),
-                vorticity_ms=t[9] / max(t[10], 1.0), vorticity_calls=int(t[10]))
+                vorticity_ms=t[9] / max(t[10], 1.0), vorticity_calls=int(t[10]),
+                far_pass_ms=t[11] / max(t[12], 1.0), far_passes=int(t[12]),
+                window_step_ms=t[13] / max(t[14], 1.0), lattice_steps=int(t[15]))
 
 
 class Batch:

@@ -149,6 +149,13 @@ int lbmib_residual(lbmib_handle* h, double* err);
 int lbmib_ib_nodes(lbmib_handle* h, int32_t* n, int32_t* pi, int32_t* pj, int32_t* wing,
                    double* glx, double* gly);
 
+/* The lattice window of the run: the rectangle of nodes (0-based origin i0,
+ * j0; width w, height hgt) that holds every IB node of every step of the
+ * motion table (move_boundary's node sets, airfoil.f90:276-335, with a margin
+ * of 6 nodes).  Its nodes step one step at a time in two-step passes; the
+ * rest, the far field, two at a time.  Needs the motion table. */
+int lbmib_lattice_window(const lbmib_handle* h, int32_t* i0, int32_t* j0, int32_t* w, int32_t* hgt);
+
 /* Marker polygon of every wing at the last step taken, n_wings *
  * pts_per_wing points (move_boundary's xb1/yb1, airfoil.f90:256-263; the
  * points of step 1 before any step), for the airfoil zone of the flow-field
@@ -242,8 +249,12 @@ int lbmib_surface_pressure_noalloc(lbmib_handle* h, double* p, int32_t n);
  *   [6] ms of geometry batches before the factorisation, [7] ms of their
  *       factorisations, [8] batches timed (geometry overlaps the steps)
  *   [9] ms in the vorticity kernel of lbmib_vorticity, [10] its calls timed
- *       (a handle's own counters; zero for a batch). */
-#define LBMIB_NTIMINGS 11
+ *       (a handle's own counters; zero for a batch)
+ *   [11] ms in two-step far-field passes, [12] passes timed
+ *   [13] ms in the lattice window's single steps of those passes, [14] timed
+ *   [15] single-step lattice updates timed ([3] is summed over them; [4] and
+ *        [5] over all sampled steps, [2]). */
+#define LBMIB_NTIMINGS 16
 int lbmib_set_profiling(lbmib_handle* h, int32_t every);
 int lbmib_get_timings(lbmib_handle* h, double* t, int32_t n);
 
@@ -283,11 +294,14 @@ typedef struct lbmib_launch_shape {
                                  their reserved CUs), not beside the lattice update */
     int32_t solve_unit;       /* cap on the steps the solves stage per LDS ring unit: 0 (none: the
                                  largest that fits), 16, 32 or 64 */
-    int32_t reserved[1];
+    int32_t fuse_steps;       /* 1: an engine of one case (a handle stepped alone, or a batch of one)
+                                 with prescribed motion takes its steps in pairs: the nodes outside
+                                 the body's lattice window take both steps in one pass, reading and
+                                 writing each population once; 0: one step at a time */
 } lbmib_launch_shape;
 
-/* Launch shape of a single handle's steps (default 1, 0, 0, 16, 0, 0, 1, 0; the
- * environment variable LBMIB_IB_CUS overrides ib_cus). */
+/* Launch shape of a single handle's steps (default 1, 0, 0, 16, 0, 0, 1, 0, 1;
+ * the environment variable LBMIB_IB_CUS overrides ib_cus). */
 int lbmib_set_shape(lbmib_handle* h, const lbmib_launch_shape* s);
 int lbmib_get_shape(const lbmib_handle* h, lbmib_launch_shape* s);
 

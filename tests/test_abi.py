@@ -137,6 +137,7 @@ def test_null_arguments_are_rejected(lbmib_mod):
     assert lib.lbmib_vorticity(None, 0, None, None) == -1
     assert lib.lbmib_surface_pressure(None, buf, 64) == -1
     assert lib.lbmib_surface_pressure_noalloc(None, buf, 64) == -1
+    assert lib.lbmib_lattice_window(None, C.byref(n), C.byref(n), C.byref(n), C.byref(n)) == -1
     assert lib.lbmib_set_dynamics(None, None) == -1
     assert lib.lbmib_get_body_state(None, buf, 64) == -1
     assert lib.lbmib_set_body_state(None, buf, 12) == -1

@@ -1,0 +1,105 @@
+"""Two-step passes of an engine of one case (launch shape fuse_steps = 1, the
+default): the nodes outside the body's lattice window take steps n and n+1
+in one pass over the far field, the window one step at a time with the IB
+chain between.  Every node keeps the single-step arithmetic, so the bar is
+bit-for-bit equality with the same case stepped one step at a time
+(fuse_steps = 0) -- populations, rho/u/v, force history, residual -- for
+call lengths that start passes at every position (odd chunks leave single
+steps), in both precisions, across a restart, and for a case that runs
+passes alone and then joins a batch (its state then lies in the other
+plane).  Parity with the oracle is covered by the single-case parity tests,
+which run with the default shape.
+"""
+import numpy as np
+import pytest
+
+from fortran_loadfile_mitiwing_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+CHUNKS = [5, 16, 3, 1, 27, 12]
+
+
+def same(a, b):
+    fa, fb = a.fields(), b.fields()
+    for k in ("rho", "u", "v", "f"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert np.array_equal(a.history(), b.history())
+
+
+def run(lbmib_mod, case, prec, fuse, chunks):
+    g = lbmib_mod.LBMIB(case, precision=prec)
+    g.set_shape(fuse_steps=fuse)
+    for c in chunks:
+        g.evolution(c)
+    return g
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_passes_equal_single_steps(lbmib_mod, prec):
+    case = cases.config(1, n_steps=80, scale_down=4)  # 1024x512, 256 markers, plunge + pitch
+    print("  %s fuse on / off, chunks %s" % (case.name, CHUNKS), flush=True)
+    a = run(lbmib_mod, case, prec, 1, CHUNKS)
+    b = run(lbmib_mod, case, prec, 0, CHUNKS)
+    same(a, b)
+    assert a.erranalysis() == b.erranalysis()
+    a.close()
+    b.close()
+
+
+def test_passes_are_taken(lbmib_mod):
+    case = cases.config(1, n_steps=40, scale_down=4)
+    g = lbmib_mod.LBMIB(case)
+    g.set_profiling(1)
+    g.evolution(32)
+    t = g.timings()
+    print("  timings:", {k: round(v, 4) if isinstance(v, float) else v for k, v in t.items()}, flush=True)
+    assert t["far_passes"] == 16 and t["lattice_steps"] == 0
+    g.set_shape(fuse_steps=0)
+    g.set_profiling(1)
+    g.evolution(8)
+    t = g.timings()
+    assert t["far_passes"] == 0 and t["lattice_steps"] == 8
+
+
+def test_full_size_first_steps_equal_single_steps(lbmib_mod):
+    case = cases.config(1, n_steps=40)  # 4096x2048, 1024 markers
+    a = run(lbmib_mod, case, 0, 1, [7, 18])
+    b = run(lbmib_mod, case, 0, 0, [7, 18])
+    same(a, b)
+
+
+def test_restart_after_passes(lbmib_mod):
+    case = cases.config(0, n_steps=60, scale_down=2)
+    whole = run(lbmib_mod, case, 0, 1, [21, 30])
+    first = run(lbmib_mod, case, 0, 1, [21])
+    f = first.fields()["f"]
+    resumed = lbmib_mod.LBMIB(case)
+    resumed.upload(f, step=21)
+    resumed.evolution(30)
+    fw, fr = whole.fields(), resumed.fields()
+    for k in ("rho", "u", "v", "f"):
+        assert np.array_equal(fw[k], fr[k]), k
+
+
+def test_case_after_passes_joins_a_batch(lbmib_mod):
+    """Two cases take 14 steps alone (7 passes each, so their states lie in
+    the other plane), then 18 together in a batch (one step at a time); the
+    first then takes 9 more alone.  Every result equals the same case stepped
+    alone one step at a time."""
+    ca = cases.config(1, n_steps=64, scale_down=8)
+    cb = cases.config(0, n_steps=64, scale_down=2)
+    prec = 1
+    a = lbmib_mod.LBMIB(ca, precision=prec)
+    a.evolution(14)
+    bb = lbmib_mod.LBMIB(cb, precision=prec)
+    bb.evolution(14)
+    b = lbmib_mod.Batch()
+    b.add(a)
+    b.add(bb)
+    b.step(18)
+    b.remove(a)
+    b.remove(bb)
+    a.evolution(9)
+    same(a, run(lbmib_mod, ca, prec, 0, [41]))
+    same(bb, run(lbmib_mod, cb, prec, 0, [32]))
