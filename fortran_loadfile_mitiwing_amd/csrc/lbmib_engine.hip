@@ -498,10 +498,13 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 //   W(n+1):  after F in F's stream (its layer around W), W(n) and the IB
 //            collide of step n
 // The ring's step n (k_ring) goes first into F's stream: it needs only the
-// far pass or lattice update before it there.
+// far pass or lattice update before it there.  When another pass follows
+// (next_pair), its ring step shares one launch with W(n+1) (k_window_ring),
+// so the next pass finds it done (ring_ready) and the path from one far
+// pass to the next holds a single launch.
 // ev as in launch_step for step n, plus [5]/[6] around W(n); [0]/[1] around F.
 int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
-                hipEvent_t wnodes, hipEvent_t* ev, int X)
+                hipEvent_t wnodes, hipEvent_t* ev, int X, bool ring_ready, bool next_pair)
 {
     const int R = Engine::RING;
     const long long gs = E->gstep;
@@ -529,12 +532,14 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     HIP_TRY(hipEventRecord(E->ev_ib[gs % R], s1));
     // the ring's step n into the middle plane: it reads only the ring's
     // neighbourhood of plane X, which the far pass (or lattice update) before
-    // it on this stream wrote, so it runs beside the window step n-1
+    // it on this stream wrote
     const DevCase& D = E->members[0]->D;
-    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3((unsigned)lattice_ring_blocks(D.nx, D.ny)),
-                       dim3(256), 0, s0, C, K, off, pa);
+    const unsigned rblocks = (unsigned)ring_blocks1(D.nx, D.ny);
+    if (!ring_ready)
+        hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(rblocks), dim3(256), 0, s0, C, K, off, pa);
     // window step n on the shared CUs
-    const dim3 wg((unsigned)(((long long)D.gw * D.gh + 255) / 256));
+    const unsigned wblocks = (unsigned)(((long long)D.gw * D.gh + 255) / 256);
+    const dim3 wg(wblocks);
     if (after_prev) {
         HIP_TRY(hipStreamWaitEvent(s3, E->ev_step[(gs - 1) % R], 0));
         HIP_TRY(hipStreamWaitEvent(s3, E->ev_ib[(gs - 1) % R], 0));
@@ -562,10 +567,15 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     HIP_TRY(hipEventRecord(E->ev_ib[(gs + 1) % R], s1));
     // window step n+1 right after the far pass in its stream (no hand-over on
     // the path from one pass to the next); W(n) and the IB collide of step n
-    // have long finished by then
+    // have long finished by then.  The next pass's ring step n+2 (planes Y
+    // -> middle) rides along.
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[gs % R], 0));
-    hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s0, C, K, off + 1, pb);
+    if (next_pair)
+        hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(wblocks + rblocks), dim3(256), 0,
+                           s0, C, K, off + 1, pb, make_psel(Y, M), (int)wblocks);
+    else
+        hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s0, C, K, off + 1, pb);
     HIP_TRY(hipEventRecord(E->ev_step[(gs + 1) % R], s0));
     E->gstep += 2;
     return 0;
@@ -588,6 +598,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
     const int Eper = Engine::EPER;
     const int n0 = par ? E->members[0]->steps_done + 1 : 0;
     if (every > 0) E->samp_kind.assign((size_t)(len + every - 1) / every, 2);
+    bool ring_ready = false;  // the ring step of the pass at off is already enqueued
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
         if ((phase + off) % B == 0 || (fresh && off == 0)) {
@@ -654,7 +665,11 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         int rc;
         if (pair) {
             const int X = (n0 + off - 1 + *par) & 1;
-            rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X);
+            // the pass after this one starts at off + 2 (even position) when
+            // its second step is inside the call
+            const bool next_pair = off + 3 < len;
+            rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X, ring_ready, next_pair);
+            ring_ready = next_pair;
             *par ^= 1;
             ++off;
         } else {

@@ -356,43 +356,96 @@ template __global__ void k_lattice<float, 2>(const DevCase*, const CaseClock*, c
 // one case whose far field takes two steps per pass (k_far2).  Thread t of
 // the grid takes node t of W in row order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                int off, int psel)
+__device__ __forceinline__ void window_thread(const DevCase& D, const CaseClock& kc, int off, int psel, long long t)
 {
-    const DevCase& D = C[0];
-    const CaseClock kc = K[0];
     const int n = kc.n0 + off;
     if (n > kc.n_end) return;
+    if (t >= (long long)D.gw * D.gh) return;
     int src, dst;
     step_planes(D, n, psel, src, dst);
     double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
-    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (long long)D.gw * D.gh) return;
     const int i = D.gi0 + (int)(t % D.gw);
     const int j = D.gj0 + (int)(t / D.gw);
     lattice_node<T>(D, (n - 1) % D.nslot, src, dst, macro, i, j);
+}
+
+// Step n of one node of the outer ring, src -> dst, thread t: threads
+// 0..ny-3 the left column at rows 1..ny-2, the next ny-2 the right column,
+// then nx the top row and nx the bottom row.  Records rho/u/v when step n is
+// recorded.
+template <typename T>
+__device__ __forceinline__ void ring_thread1(const DevCase& D, const CaseClock& kc, int off, int psel, int t)
+{
+    const int n = kc.n0 + off;
+    if (n > kc.n_end) return;
+    const int nx = D.nx, ny = D.ny;
+    int i, j;
+    if (t < 2 * (ny - 2)) {
+        const bool right = t >= ny - 2;
+        i = right ? nx - 1 : 0;
+        j = 1 + (right ? t - (ny - 2) : t);
+    } else {
+        t -= 2 * (ny - 2);
+        if (t >= 2 * nx) return;
+        const bool bottom = t >= nx;
+        i = bottom ? t - nx : t;
+        j = bottom ? 0 : ny - 1;
+    }
+    int src, dst;
+    step_planes(D, n, psel, src, dst);
+    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
+    const long long N = D.N;
+    const GlobalPull<T> pull{pops<T>(D, src), N, nx, D.W};
+    double out[9], ro, uo, vo;
+    ring_node(D, pull, i, j, out, ro, uo, vo);
+    const long long c = (long long)j * nx + i;
+    GAS T* out_p = pops<T>(D, dst);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], out_p + k * N + c);
+    if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                int off, int psel)
+{
+    const CaseClock kc = K[0];
+    window_thread<T>(C[0], kc, off, psel, (long long)blockIdx.x * 256 + threadIdx.x);
 }
 template __global__ void k_window<double>(const DevCase*, const CaseClock*, int, int);
 template __global__ void k_window<float>(const DevCase*, const CaseClock*, int, int);
 
 // Step n of the outer ring alone, src -> dst (the middle plane of a two-step
 // pass, which then reads the ring's step n there instead of recomputing it
-// in phase 1 one row at a time); records rho/u/v when step n is recorded.
+// in phase 1 one row at a time); one node per thread.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
                                              int psel)
 {
-    const DevCase& D = C[0];
     const CaseClock kc = K[0];
-    const int n = kc.n0 + off;
-    if (n > kc.n_end) return;
-    int src, dst;
-    step_planes(D, n, psel, src, dst);
-    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
-    ring_thread<T>(D, src, dst, macro, (int)(blockIdx.x * 256 + threadIdx.x));
+    ring_thread1<T>(C[0], kc, off, psel, (int)(blockIdx.x * 256 + threadIdx.x));
 }
 template __global__ void k_ring<double>(const DevCase*, const CaseClock*, int, int);
 template __global__ void k_ring<float>(const DevCase*, const CaseClock*, int, int);
+
+// The tail of a two-step pass followed by another: the window's second step
+// (offset off, planes psel) on workgroups 0..wblocks-1 and the next pass's
+// ring step (offset off + 1, planes ring_psel) on the rest, in one launch.
+// They touch disjoint nodes: the window step reads the middle plane inside W
+// and writes W; the ring step reads the ring's neighbourhood of the plane the
+// far pass just wrote and writes the ring into the middle plane, whose ring
+// values of the step before that pass has read.
+template <typename T>
+__global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                     int off, int psel, int ring_psel, int wblocks)
+{
+    const CaseClock kc = K[0];
+    const int b = (int)blockIdx.x;
+    if (b < wblocks) window_thread<T>(C[0], kc, off, psel, (long long)b * 256 + threadIdx.x);
+    else ring_thread1<T>(C[0], kc, off + 1, ring_psel, (b - wblocks) * 256 + (int)threadIdx.x);
+}
+template __global__ void k_window_ring<double>(const DevCase*, const CaseClock*, int, int, int, int);
+template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Two steps of the far field in one pass
