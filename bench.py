@@ -77,8 +77,9 @@ PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every pha
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), both scaled by a 1 GiB
 # device copy measured the same way (the lattice traffic summary in profiles/)
 LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
-# the same for a k_far2<double> launch (two updates of the far field)
-FAR_TRAFFIC_BYTES = {}
+# the same for a k_far2<double> launch (two updates of the far field):
+# FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) of a pass that records nothing
+FAR_TRAFFIC_BYTES = {(4096, 2048): (396557 * 2 + 594236) * 1024}
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
