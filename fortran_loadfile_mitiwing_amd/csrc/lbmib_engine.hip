@@ -258,16 +258,16 @@ int engine_prepare(Engine* E)
     int rc = 0;
     if (fuse_ok(E)) {
         // the middle plane, and the far-field grid: strips of at most FAR_OUT
-        // columns, segments of about 10 rows (FP64) or 8 (FP32).  Many short
+        // columns, segments of about 8 rows (FP64) or 6 (FP32).  Many short
         // segments beat one workgroup per CU slot with tall ones (4096x2048,
-        // FP64: 0.338 ms per pass at 224 segments against 0.488 ms at 28):
-        // the workgroups in flight, taken in launch order, then cover one band
-        // of consecutive rows, and the last ones balance the load; shorter
-        // segments than these repeat more of phase 1.  LBMIB_FAR_SEGS overrides.
+        // FP64: 0.303 ms per pass at 256 segments, 0.321 ms at 102, 0.49 ms
+        // at 28): the pass is bound by the rows a workgroup has in flight,
+        // not by the halo rows that short segments read again, and many
+        // workgroups balance the load at the end.  LBMIB_FAR_SEGS overrides.
         if ((rc = ensure_mid_plane(E->members[0]))) return rc;
         const DevCase& D = E->members[0]->D;
         E->far_strips = (D.nx + FAR_OUT - 1) / FAR_OUT;
-        E->far_segs = std::max(1, D.ny / (D.prec == LBMIB_PREC_FP64 ? 10 : 8));
+        E->far_segs = std::max(1, D.ny / (D.prec == LBMIB_PREC_FP64 ? 8 : 6));
         if (const char* e = getenv("LBMIB_FAR_SEGS")) E->far_segs = std::max(1, std::min(D.ny / 4, atoi(e)));
         HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)far2_smem_bytes(true)));

@@ -460,15 +460,25 @@ template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, 
 //
 // A workgroup of FAR_LANES lanes owns a strip of output columns [a, b)
 // (b - a <= FAR_OUT) and a segment of output rows [j0, j1]; lane L stands at
-// column i = a - 1 + L.  It sweeps the rows upwards.  At row r it forms step
-// n at (i, r) -- phase 1: pull from plane src, moments, collide, or the
-// ring's update -- into slot r & 3 of a 4-row LDS ring, then, after one
-// workgroup barrier, step n+1 at (i, r-1) from the ring (phase 2) for the
-// lanes of the strip's columns.  The bottom wall row pulls rows 0..2, so it
-// takes its step one row later, with row 1; the top wall row pulls rows
-// ny-3..ny-1, all in the ring when it is reached.  A ring slot is written
-// again four rows later, after every lane has passed the barrier of the
-// row in between, so one barrier per row orders all LDS traffic.
+// column i = a - 1 + L, and the lanes of columns a-1..b take part.  It sweeps
+// the rows upwards.  At row r it forms step n at (i, r) -- phase 1: pull from
+// plane src, moments, collide, or read the ring's step n -- then, after one
+// workgroup barrier, step n+1 at (i, r-1) (phase 2) for the lanes of the
+// strip's columns, pulling step n from the neighbouring lanes.
+//
+// Step n of a row is kept only as far as phase 2 pulls it across lanes: the
+// populations k = 1, 3 (pulled from the same row), 5, 6 (pulled by the row
+// above) and 7, 8 (by the row below) go to LDS rings of 3, 4 and 2 rows;
+// the lane's own k = 0, 2, 4 stay in its registers (rows r-1, r-2 and r),
+// and for the outer columns the lanes of columns 1, 2, nx-3 and nx-2 also
+// put their k = 0, 2, 4 into a small side ring (the edge nodes pull those
+// columns).  Phase 2 of row q reads k = 7, 8 of row q+1, 1, 3 of row q and
+// 5, 6 of row q-1; a wave can be at most one row ahead of another (it waits
+// at the next barrier), so the row written next never lands on a slot still
+// being read.  37 KB of LDS in FP64 (four workgroups per CU), 19 KB in FP32.
+//
+// The bottom wall row pulls rows 0..2 and the top wall row rows ny-3..ny-1,
+// exactly the pull of rows 1 and ny-2, so each takes its step with that row.
 //
 // Every node keeps the single-step arithmetic (the same functions, the same
 // operations in the same order), so two steps here equal two single steps
@@ -480,32 +490,63 @@ template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, 
 //
 // psel packs the planes: src = psel & 3, dst = (psel >> 2) & 3, middle =
 // (psel >> 4) & 3.
-constexpr int FAR_SLOTS = 4;
-size_t far2_smem_bytes(bool f64) { return (size_t)FAR_SLOTS * 9 * FAR_LANES * (f64 ? 8 : 4); }
+constexpr int FAR_R13 = 3, FAR_R56 = 4, FAR_R78 = 2, FAR_RSIDE = 4;
+constexpr int FAR_O56 = FAR_R13 * 2 * FAR_LANES;         // element offsets of the rings
+constexpr int FAR_O78 = FAR_O56 + FAR_R56 * 2 * FAR_LANES;
+constexpr int FAR_OSIDE = FAR_O78 + FAR_R78 * 2 * FAR_LANES;
+constexpr int FAR_NELEM = FAR_OSIDE + FAR_RSIDE * 3 * 4;
+size_t far2_smem_bytes(bool f64) { return (size_t)FAR_NELEM * (f64 ? 8 : 4); }
 
+// Side-ring column index of the columns the edge nodes pull: 1, 2, nx-3,
+// nx-2 -> 0..3, else -1.
+__device__ __forceinline__ int far_side_col(int i, int nx)
+{
+    return i == 1 ? 0 : i == 2 ? 1 : i == nx - 3 ? 2 : i == nx - 2 ? 3 : -1;
+}
+
+// Pull accessor over the rings for the outer nodes' updates: gives the
+// nine populations of step n streamed into (ic, jc), where jc is the row
+// whose pull the lane's registers hold (own0: k = 0 at row jc, own2: k = 2
+// at row jc - 1, own4: k = 4 at row jc + 1) and ic is the lane's own column
+// or one of the side ring's columns.
 template <typename T>
-struct LdsPull {
-    const T* ring;   // [FAR_SLOTS][9][FAR_LANES]
-    int c0;          // column of lane 0
+struct FarPull {
+    const T* lds;
+    int c0, i_own, nx;
+    T own0, own2, own4;
     const double* W;
-    __device__ __forceinline__ void operator()(int i, int j, double* f) const
+    __device__ __forceinline__ void operator()(int ic, int jc, double* f) const
     {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int ex = k == 1 || k == 5 || k == 8 ? 1 : (k == 3 || k == 6 || k == 7 ? -1 : 0);
-            const int ey = k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 || k == 8 ? -1 : 0);
-            const int sl = (j - ey) & (FAR_SLOTS - 1);
-            f[k] = lift(ring[(sl * 9 + k) * FAR_LANES + (i - ex - c0)], W[k]);
+        const int l = ic - c0;
+        const int s13 = jc % FAR_R13, s56 = (jc - 1) & (FAR_R56 - 1), s78 = (jc + 1) & (FAR_R78 - 1);
+        T g[9];
+        g[1] = lds[(s13 * 2 + 0) * FAR_LANES + l - 1];
+        g[3] = lds[(s13 * 2 + 1) * FAR_LANES + l + 1];
+        g[5] = lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + l - 1];
+        g[6] = lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + l + 1];
+        g[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + l + 1];
+        g[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + l - 1];
+        if (ic == i_own) {
+            g[0] = own0;
+            g[2] = own2;
+            g[4] = own4;
+        } else {
+            const int e = far_side_col(ic, nx);
+            g[0] = lds[FAR_OSIDE + ((jc & (FAR_RSIDE - 1)) * 3 + 0) * 4 + e];
+            g[2] = lds[FAR_OSIDE + (((jc - 1) & (FAR_RSIDE - 1)) * 3 + 1) * 4 + e];
+            g[4] = lds[FAR_OSIDE + (((jc + 1) & (FAR_RSIDE - 1)) * 3 + 2) * 4 + e];
         }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[k] = lift(g[k], W[k]);
     }
 };
 
 template <typename T>
-__global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                    int off, int psel)
+__device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
+                                          int psel)
 {
     extern __shared__ __align__(16) unsigned char far_smem[];
-    T* ring = reinterpret_cast<T*>(far_smem);
+    T* lds = reinterpret_cast<T*>(far_smem);
     const DevCase& D = C[0];
     const CaseClock kc = K[0];
     const int n = kc.n0 + off;  // the pair's first step
@@ -521,33 +562,42 @@ __global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ 
     const int c0 = a - 1;
     const int i = c0 + L;
     const int wi0 = D.gi0, wi1 = D.gi0 + D.gw - 1, wj0 = D.gj0, wj1 = D.gj0 + D.gh - 1;
-    const bool col_in = i >= 0 && i < nx;
+    const bool col_in = i >= 0 && i < nx && i <= b;
     const bool own_col = i >= a && i < b;
+    const int side = col_in ? far_side_col(i, nx) : -1;
     double* mac1 = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
     double* mac2 = n + 1 >= kc.flag_from ? D.macro[(n + 1) & 1] : nullptr;
-    const LdsPull<T> lpull{ring, c0, D.W};
     GAS T* __restrict__ out2 = pops<T>(D, dst);
     GAS T* __restrict__ outm = pops<T>(D, mid);
+    // the lane's own k = 0, 2, 4 of step n: k = 0 and 2 of the row before
+    // (h0, h2a), k = 2 of the row before that (h2b), and of the current row
+    T h0 = 0, h2a = 0, h2b = 0, c0v = 0, c2v = 0, c4v = 0;
 
-    // phase 2 at output row q for this lane (a strip column)
+    // phase 2 at output row q for this lane (a strip column); the registers
+    // and rings hold the pull of row r - 1 (q itself, or the wall's neighbour)
     auto phase2 = [&](int q) {
         if (!own_col) return;
         if (i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1) return;
         const long long c = (long long)q * nx + i;
         double rho, u, v;
         if (i == 0 || i == nx - 1 || q == 0 || q == ny - 1) {
+            const FarPull<T> fp{lds, c0, i, nx, h0, h2b, c4v, D.W};
             double out[9];
-            ring_node(D, lpull, i, q, out, rho, u, v);
+            ring_node(D, fp, i, q, out, rho, u, v);
 #pragma unroll
             for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], out2 + k * N + c);
         } else {
+            const int s13 = q % FAR_R13, s56 = (q - 1) & (FAR_R56 - 1), s78 = (q + 1) & (FAR_R78 - 1);
             T p[9], out[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int ex = k == 1 || k == 5 || k == 8 ? 1 : (k == 3 || k == 6 || k == 7 ? -1 : 0);
-                const int ey = k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 || k == 8 ? -1 : 0);
-                p[k] = ring[(((q - ey) & (FAR_SLOTS - 1)) * 9 + k) * FAR_LANES + (L - ex)];
-            }
+            p[0] = h0;
+            p[1] = lds[(s13 * 2 + 0) * FAR_LANES + L - 1];
+            p[2] = h2b;
+            p[3] = lds[(s13 * 2 + 1) * FAR_LANES + L + 1];
+            p[4] = c4v;
+            p[5] = lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + L - 1];
+            p[6] = lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + L + 1];
+            p[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L + 1];
+            p[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L - 1];
             interior_step(D, p, out, rho, u, v);
 #pragma unroll
             for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], out2 + k * N + c);
@@ -577,48 +627,82 @@ __global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ 
             for (int k = 0; k < 9; ++k) pre[k] = outm[k * N + c];
         }
     };
+    // the top wall takes its step with row ny-2, so the sweep ends there
+    const int rend = j1 == ny - 1 ? ny - 1 : j1 + 1;
     int kpre = kind1(j0 - 1);
     load1(kpre, j0 - 1);
-    for (int r = j0 - 1; r <= j1 + 1; ++r) {
+    for (int r = j0 - 1; r <= rend; ++r) {
         const int kr = kpre;
         T p[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) p[k] = pre[k];
-        kpre = r + 1 <= j1 + 1 ? kind1(r + 1) : 0;
+        kpre = r + 1 <= rend ? kind1(r + 1) : 0;
         load1(kpre, r + 1);
         // phase 1: step n at (i, r)
         if (kr != 0) {
             const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
-            {
-                const long long c = (long long)r * nx + i;
-                T* slot = ring + (size_t)(r & (FAR_SLOTS - 1)) * 9 * FAR_LANES + L;
-                T st[9];
-                double rho = 0.0, u = 0.0, v = 0.0;
-                if (kr == 2) {
+            const long long c = (long long)r * nx + i;
+            T st[9];
+            double rho = 0.0, u = 0.0, v = 0.0;
+            if (kr == 2) {
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) st[k] = p[k];
-                } else {
-                    interior_step(D, p, st, rho, u, v);
-                }
-#pragma unroll
-                for (int k = 0; k < 9; ++k) slot[k * FAR_LANES] = st[k];
-                const bool own = own_col && r >= j0 && r <= j1 && !in_w && kr == 1;
-                if (own && i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1) {
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // F's layer around W, step n
-                }
-                if (own && mac1) { mac1[c] = rho; mac1[N + c] = u; mac1[2 * N + c] = v; }
+                for (int k = 0; k < 9; ++k) st[k] = p[k];
+            } else {
+                interior_step(D, p, st, rho, u, v);
             }
+            const int s13 = r % FAR_R13, s56 = r & (FAR_R56 - 1), s78 = r & (FAR_R78 - 1);
+            lds[(s13 * 2 + 0) * FAR_LANES + L] = st[1];
+            lds[(s13 * 2 + 1) * FAR_LANES + L] = st[3];
+            lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + L] = st[5];
+            lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + L] = st[6];
+            lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L] = st[7];
+            lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L] = st[8];
+            if (side >= 0) {
+                const int ss = r & (FAR_RSIDE - 1);
+                lds[FAR_OSIDE + (ss * 3 + 0) * 4 + side] = st[0];
+                lds[FAR_OSIDE + (ss * 3 + 1) * 4 + side] = st[2];
+                lds[FAR_OSIDE + (ss * 3 + 2) * 4 + side] = st[4];
+            }
+            c0v = st[0];
+            c2v = st[2];
+            c4v = st[4];
+            const bool own = own_col && r >= j0 && r <= j1 && !in_w && kr == 1;
+            if (own && i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // F's layer around W, step n
+            }
+            if (own && mac1) { mac1[c] = rho; mac1[N + c] = u; mac1[2 * N + c] = v; }
         }
         lds_barrier();  // the next row's pulls stay in flight across it
-        // phase 2: step n+1 at row r-1 (the bottom wall row with row 1)
+        // phase 2: step n+1 at row r-1; the bottom wall row with row 1, the
+        // top wall row with row ny-2
         const int q = r - 1;
-        if (q >= j0 && q <= j1 && q != 0) phase2(q);
-        if (q == 1 && j0 == 0) phase2(0);
+        // (one call site in a loop of two: each inlined copy of the outer
+        // nodes' update costs registers)
+        const int qw = (q == 1 && j0 == 0) ? 0 : (q == ny - 2 && j1 == ny - 1) ? ny - 1 : -1;
+#pragma unroll 1
+        for (int t = 0; t < 2; ++t) {
+            if (t == 0 ? (q >= j0 && q <= j1 && q != 0) : qw >= 0) phase2(t == 0 ? q : qw);
+        }
+        h2b = h2a;
+        h2a = c2v;
+        h0 = c0v;
     }
 }
-template __global__ void k_far2<double>(const DevCase*, const CaseClock*, int, int);
-template __global__ void k_far2<float>(const DevCase*, const CaseClock*, int, int);
+// FP64: 133 registers, three workgroups per CU.  FP32: capped at 96
+// registers (no spills) so that five workgroups fit per CU, as its LDS allows.
+template <>
+__global__ __launch_bounds__(FAR_LANES) void k_far2<double>(const DevCase* __restrict__ C,
+                                                            const CaseClock* __restrict__ K, int off, int psel)
+{
+    far2_body<double>(C, K, off, psel);
+}
+template <>
+__global__ __launch_bounds__(FAR_LANES) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_far2<float>(
+    const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel)
+{
+    far2_body<float>(C, K, off, psel);
+}
 
 int lattice_tiles(int nx, int ny, int npt)
 {
