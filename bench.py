@@ -79,7 +79,7 @@ PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every pha
 LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
 # the same for a k_far2<double> launch (two updates of the far field):
 # FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) of a pass that records nothing
-FAR_TRAFFIC_BYTES = {(4096, 2048): (396557 * 2 + 594236) * 1024}
+FAR_TRAFFIC_BYTES = {(4096, 2048): (395203 * 2 + 594673) * 1024}
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
