@@ -14,8 +14,12 @@ default launch shape): the far-field kernel k_far2 advances every node
 outside the lattice window that holds the body by two updates per launch, so
 its algorithmic bytes are 2 x 144 per far-field node (9 FP64 populations read
 + 9 written per update) over its mean launch time, timed with HIP events on
-the stream it runs on.  With single steps (fuse_steps=0 in --shape) it is the
-fused stream+collide kernel k_lattice, 144 bytes per interior node.
+the stream it runs on.  It reads and writes each population once per two
+updates, so this algorithmic rate can pass the HBM peak; `traffic` is the
+HBM bytes a launch moves (rocprofv3 FETCH_SIZE/WRITE_SIZE) and
+`traffic_frac` that rate against the peak.  With single steps
+(fuse_steps=0 in --shape) the dominant kernel is the fused stream+collide
+kernel k_lattice, 144 bytes per interior node.
 cpu_baseline = the CPU oracle (C restatement of the reference step, OpenMP
 over the host cores) on the first steps of the same workload.
 
@@ -729,7 +733,8 @@ def main():
                    "nx": p.nx, "ny": p.ny, "markers": p.pts_per_wing - 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
-                     "bytes_per_launch": bytes_launch, "mean_launch_ms": round(lat_s * 1e3, 5)},
+                     "bytes_per_launch": bytes_launch, "mean_launch_ms": round(lat_s * 1e3, 5),
+                     "traffic_frac": round(traffic / lat_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None},
         "lattice_window": {"i0": wi0, "j0": wj0, "width": ww, "height": wh},
         "whole_step_hbm_frac": round(value * 1e6 * bpn / (HBM_PEAK_GBS * 1e9), 4),
         "phases_ms": {k: round(v, 5) for k, v in tm.items()},

@@ -258,16 +258,17 @@ int engine_prepare(Engine* E)
     int rc = 0;
     if (fuse_ok(E)) {
         // the middle plane, and the far-field grid: strips of at most FAR_OUT
-        // columns, segments of about 8 rows (FP64) or 6 (FP32).  Many short
-        // segments beat one workgroup per CU slot with tall ones (4096x2048,
-        // FP64: 0.303 ms per pass at 256 segments, 0.321 ms at 102, 0.49 ms
-        // at 28): the pass is bound by the rows a workgroup has in flight,
-        // not by the halo rows that short segments read again, and many
-        // workgroups balance the load at the end.  LBMIB_FAR_SEGS overrides.
+        // columns, segments of about 5 rows.  Many short segments beat one
+        // workgroup per CU slot with tall ones (4096x2048, FP64: 0.279 ms per
+        // pass at 410 segments, 0.291 ms at 256, 0.49 ms at 28 with an older
+        // kernel): the pass is bound by the rows the workgroups have in
+        // flight, not by the halo rows that short segments read again, and
+        // many workgroups balance the load at the end.  LBMIB_FAR_SEGS
+        // overrides.
         if ((rc = ensure_mid_plane(E->members[0]))) return rc;
         const DevCase& D = E->members[0]->D;
         E->far_strips = (D.nx + FAR_OUT - 1) / FAR_OUT;
-        E->far_segs = std::max(1, D.ny / (D.prec == LBMIB_PREC_FP64 ? 8 : 6));
+        E->far_segs = std::max(1, D.ny / 5);
         if (const char* e = getenv("LBMIB_FAR_SEGS")) E->far_segs = std::max(1, std::min(D.ny / 4, atoi(e)));
         HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)far2_smem_bytes(true)));
@@ -497,14 +498,13 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 //   IB(n+1): after W(n) (and IB(n) in its own stream)
 //   W(n+1):  after F in F's stream (its layer around W), W(n) and the IB
 //            collide of step n
-// The ring's step n (k_ring) goes first into F's stream: it needs only the
-// far pass or lattice update before it there.  When another pass follows
-// (next_pair), its ring step shares one launch with W(n+1) (k_window_ring),
-// so the next pass finds it done (ring_ready) and the path from one far
-// pass to the next holds a single launch.
+// The outer ring takes its steps in F's stream: step n (k_ring, X -> middle)
+// before F, which reads it there, and step n+1 (middle -> Y) after F, which
+// puts the ring's neighbourhood of step n into the middle plane, in one
+// launch with W(n+1) (k_window_ring).
 // ev as in launch_step for step n, plus [5]/[6] around W(n); [0]/[1] around F.
 int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
-                hipEvent_t wnodes, hipEvent_t* ev, int X, bool ring_ready, bool next_pair)
+                hipEvent_t wnodes, hipEvent_t* ev, int X)
 {
     const int R = Engine::RING;
     const long long gs = E->gstep;
@@ -535,8 +535,7 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     // it on this stream wrote
     const DevCase& D = E->members[0]->D;
     const unsigned rblocks = (unsigned)ring_blocks1(D.nx, D.ny);
-    if (!ring_ready)
-        hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(rblocks), dim3(256), 0, s0, C, K, off, pa);
+    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(rblocks), dim3(256), 0, s0, C, K, off, pa);
     // window step n on the shared CUs
     const unsigned wblocks = (unsigned)(((long long)D.gw * D.gh + 255) / 256);
     const dim3 wg(wblocks);
@@ -567,15 +566,11 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     HIP_TRY(hipEventRecord(E->ev_ib[(gs + 1) % R], s1));
     // window step n+1 right after the far pass in its stream (no hand-over on
     // the path from one pass to the next); W(n) and the IB collide of step n
-    // have long finished by then.  The next pass's ring step n+2 (planes Y
-    // -> middle) rides along.
+    // have long finished by then.  The ring's step n+1 rides along.
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[gs % R], 0));
-    if (next_pair)
-        hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(wblocks + rblocks), dim3(256), 0,
-                           s0, C, K, off + 1, pb, make_psel(Y, M), (int)wblocks);
-    else
-        hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s0, C, K, off + 1, pb);
+    hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(wblocks + rblocks), dim3(256), 0, s0,
+                       C, K, off + 1, pb, make_psel(M, Y), (int)wblocks);
     HIP_TRY(hipEventRecord(E->ev_step[(gs + 1) % R], s0));
     E->gstep += 2;
     return 0;
@@ -598,7 +593,6 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
     const int Eper = Engine::EPER;
     const int n0 = par ? E->members[0]->steps_done + 1 : 0;
     if (every > 0) E->samp_kind.assign((size_t)(len + every - 1) / every, 2);
-    bool ring_ready = false;  // the ring step of the pass at off is already enqueued
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
         if ((phase + off) % B == 0 || (fresh && off == 0)) {
@@ -665,11 +659,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         int rc;
         if (pair) {
             const int X = (n0 + off - 1 + *par) & 1;
-            // the pass after this one starts at off + 2 (even position) when
-            // its second step is inside the call
-            const bool next_pair = off + 3 < len;
-            rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X, ring_ready, next_pair);
-            ring_ready = next_pair;
+            rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X);
             *par ^= 1;
             ++off;
         } else {

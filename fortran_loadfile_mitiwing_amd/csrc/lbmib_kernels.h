@@ -192,10 +192,6 @@ constexpr int FAR_LANES = 256;             // phase-1 columns per strip
 constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip
 template <typename T>
 __global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
-template <>
-__global__ void k_far2<double>(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
-template <>
-__global__ void k_far2<float>(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
 template <typename T>
 __global__ void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
 template <typename T>

@@ -157,6 +157,19 @@ __device__ __forceinline__ void edge_column_state(const DevCase& D, const PULL& 
 // effective state is computed), the corners reading the left/right results
 // at rows 1 and ny-2, recomputed here.  Reads only the previous populations
 // and the stale ring f0.
+// The top or bottom wall node away from the corners, from f0n, the pull of
+// its neighbour node in the next row in (rho of that node, at rest).
+__device__ __forceinline__ void wall_node(const DevCase& D, const double* f0n, double* out, double& ro, double& uo,
+                                          double& vo)
+{
+    double rn, un, vn;
+    moments(f0n, rn, un, vn);
+    ro = rn;
+    uo = 0.0;
+    vo = 0.0;
+    nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
+}
+
 template <typename PULL>
 __device__ __forceinline__ void ring_node(const DevCase& D, const PULL& pull, int i, int j, double* out, double& ro,
                                           double& uo, double& vo)
@@ -170,22 +183,22 @@ __device__ __forceinline__ void ring_node(const DevCase& D, const PULL& pull, in
     }
     // top wall (row ny-1, neighbour ny-2) or bottom wall (row 0, neighbour 1)
     const int jn = j == 0 ? 1 : ny - 2;
-    double rn, un, vn, f0n[9];
+    double f0n[9];
     if (i == 0 || i == nx - 1) {
         const int col = i == 0 ? 0 : 1;
-        double fdummy[9], r2, u2, v2;
+        double rn, un, vn, fdummy[9], r2, u2, v2;
         edge_column_state(D, pull, D.cs, col, jn, rn, un, vn, fdummy, r2, u2, v2);
         const GAS double* f0c = (const GAS double*)(col == 0 ? D.f0L : D.f0R);
 #pragma unroll
         for (int k = 0; k < 9; ++k) f0n[k] = f0c[k * ny + jn];
+        ro = rn;
+        uo = 0.0;
+        vo = 0.0;
+        nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
     } else {
         pull(i, jn, f0n);
-        moments(f0n, rn, un, vn);
+        wall_node(D, f0n, out, ro, uo, vo);
     }
-    ro = rn;
-    uo = 0.0;
-    vo = 0.0;
-    nee_node(ro, uo, vo, rn, un, vn, f0n, D.R, out);
 }
 
 // Ring thread t of a lattice update: threads 0..ny-3 take the left and right
@@ -428,13 +441,13 @@ __global__ __launch_bounds__(256) void k_ring(const DevCase* __restrict__ C, con
 template __global__ void k_ring<double>(const DevCase*, const CaseClock*, int, int);
 template __global__ void k_ring<float>(const DevCase*, const CaseClock*, int, int);
 
-// The tail of a two-step pass followed by another: the window's second step
-// (offset off, planes psel) on workgroups 0..wblocks-1 and the next pass's
-// ring step (offset off + 1, planes ring_psel) on the rest, in one launch.
+// The tail of a two-step pass: step n+1 (offset off) of the window (planes
+// psel, workgroups 0..wblocks-1) and of the outer ring (planes ring_psel:
+// the middle plane to the pass's output plane, the rest), in one launch.
 // They touch disjoint nodes: the window step reads the middle plane inside W
-// and writes W; the ring step reads the ring's neighbourhood of the plane the
-// far pass just wrote and writes the ring into the middle plane, whose ring
-// values of the step before that pass has read.
+// and writes W; the ring step reads the ring's neighbourhood of the middle
+// plane (the far pass put step n of that neighbourhood there) and writes the
+// ring.
 template <typename T>
 __global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                                                      int off, int psel, int ring_psel, int wblocks)
@@ -442,7 +455,7 @@ __global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__
     const CaseClock kc = K[0];
     const int b = (int)blockIdx.x;
     if (b < wblocks) window_thread<T>(C[0], kc, off, psel, (long long)b * 256 + threadIdx.x);
-    else ring_thread1<T>(C[0], kc, off + 1, ring_psel, (b - wblocks) * 256 + (int)threadIdx.x);
+    else ring_thread1<T>(C[0], kc, off, ring_psel, (b - wblocks) * 256 + (int)threadIdx.x);
 }
 template __global__ void k_window_ring<double>(const DevCase*, const CaseClock*, int, int, int, int);
 template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, int, int, int, int);
@@ -453,93 +466,49 @@ template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, 
 // An engine of one case may take its steps in pairs (n, n+1): the nodes of
 // the window W (every IB node of the run lies at least 3 nodes inside it)
 // advance one step at a time with the IB chain between the steps (k_window,
-// k_band), and every other node -- the far field F, the outer ring included
-// -- takes both steps in one pass here, reading each population of step n-1
-// once and writing each of step n+1 once: 144 (FP64) or 72 (FP32) bytes per
-// node for two steps.
+// k_band), the outer ring takes its steps in kernels of its own (k_ring
+// before the pass, step n; k_window_ring after it, step n+1), and every
+// other node -- the far field F -- takes both steps in one pass here,
+// reading each population of step n-1 once and writing each of step n+1
+// once: 144 (FP64) or 72 (FP32) bytes per node for two steps.
 //
 // A workgroup of FAR_LANES lanes owns a strip of output columns [a, b)
 // (b - a <= FAR_OUT) and a segment of output rows [j0, j1]; lane L stands at
 // column i = a - 1 + L, and the lanes of columns a-1..b take part.  It sweeps
 // the rows upwards.  At row r it forms step n at (i, r) -- phase 1: pull from
-// plane src, moments, collide, or read the ring's step n -- then, after one
-// workgroup barrier, step n+1 at (i, r-1) (phase 2) for the lanes of the
-// strip's columns, pulling step n from the neighbouring lanes.
+// plane src, moments, collide; at a ring node read its step n from the
+// middle plane -- then, after one workgroup barrier, step n+1 at (i, r-1)
+// (phase 2) for the lanes of the strip's columns but the ring, pulling step n
+// from the neighbouring lanes.
 //
 // Step n of a row is kept only as far as phase 2 pulls it across lanes: the
 // populations k = 1, 3 (pulled from the same row), 5, 6 (pulled by the row
 // above) and 7, 8 (by the row below) go to LDS rings of 3, 4 and 2 rows;
-// the lane's own k = 0, 2, 4 stay in its registers (rows r-1, r-2 and r),
-// and for the outer columns the lanes of columns 1, 2, nx-3 and nx-2 also
-// put their k = 0, 2, 4 into a small side ring (the edge nodes pull those
-// columns).  Phase 2 of row q reads k = 7, 8 of row q+1, 1, 3 of row q and
-// 5, 6 of row q-1; a wave can be at most one row ahead of another (it waits
-// at the next barrier), so the row written next never lands on a slot still
-// being read.  37 KB of LDS in FP64 (four workgroups per CU), 19 KB in FP32.
-//
-// The bottom wall row pulls rows 0..2 and the top wall row rows ny-3..ny-1,
-// exactly the pull of rows 1 and ny-2, so each takes its step with that row.
+// the lane's own k = 0, 2, 4 stay in its registers (rows r-1, r-2 and r).
+// Phase 2 of row q reads k = 7, 8 of row q+1, 1, 3 of row q and 5, 6 of row
+// q-1; a wave can be at most one row ahead of another (it waits at the next
+// barrier), so the row written next never lands on a slot still being read.
+// 36 KB of LDS in FP64, 18 KB in FP32.  With the ring's updates (which pull
+// up to three columns or two rows inwards, and fill registers) kept out of
+// this kernel, it runs four workgroups per CU in FP64 and eight in FP32.
 //
 // Every node keeps the single-step arithmetic (the same functions, the same
 // operations in the same order), so two steps here equal two single steps
 // bit for bit.  Phase 1 skips W's nodes but its outer layer (phase 2 pulls
-// only that layer of W); it writes step n of F's layer around W into the
-// middle plane (the window's second step pulls it there), and both phases
+// only that layer of W); it writes step n of F's layer around W (the
+// window's second step pulls it) and of the ring's neighbourhood (columns
+// 1..3 and nx-4..nx-2, rows 1, 2 and ny-3, ny-2: the ring's update pulls
+// them) into the middle plane, and both phases
 // record rho/u/v of their step when it is recorded.  Each of these writes
 // comes from the workgroup whose output range holds the node.
 //
 // psel packs the planes: src = psel & 3, dst = (psel >> 2) & 3, middle =
 // (psel >> 4) & 3.
-constexpr int FAR_R13 = 3, FAR_R56 = 4, FAR_R78 = 2, FAR_RSIDE = 4;
+constexpr int FAR_R13 = 3, FAR_R56 = 4, FAR_R78 = 2;
 constexpr int FAR_O56 = FAR_R13 * 2 * FAR_LANES;         // element offsets of the rings
 constexpr int FAR_O78 = FAR_O56 + FAR_R56 * 2 * FAR_LANES;
-constexpr int FAR_OSIDE = FAR_O78 + FAR_R78 * 2 * FAR_LANES;
-constexpr int FAR_NELEM = FAR_OSIDE + FAR_RSIDE * 3 * 4;
+constexpr int FAR_NELEM = FAR_O78 + FAR_R78 * 2 * FAR_LANES;
 size_t far2_smem_bytes(bool f64) { return (size_t)FAR_NELEM * (f64 ? 8 : 4); }
-
-// Side-ring column index of the columns the edge nodes pull: 1, 2, nx-3,
-// nx-2 -> 0..3, else -1.
-__device__ __forceinline__ int far_side_col(int i, int nx)
-{
-    return i == 1 ? 0 : i == 2 ? 1 : i == nx - 3 ? 2 : i == nx - 2 ? 3 : -1;
-}
-
-// Pull accessor over the rings for the outer nodes' updates: gives the
-// nine populations of step n streamed into (ic, jc), where jc is the row
-// whose pull the lane's registers hold (own0: k = 0 at row jc, own2: k = 2
-// at row jc - 1, own4: k = 4 at row jc + 1) and ic is the lane's own column
-// or one of the side ring's columns.
-template <typename T>
-struct FarPull {
-    const T* lds;
-    int c0, i_own, nx;
-    T own0, own2, own4;
-    const double* W;
-    __device__ __forceinline__ void operator()(int ic, int jc, double* f) const
-    {
-        const int l = ic - c0;
-        const int s13 = jc % FAR_R13, s56 = (jc - 1) & (FAR_R56 - 1), s78 = (jc + 1) & (FAR_R78 - 1);
-        T g[9];
-        g[1] = lds[(s13 * 2 + 0) * FAR_LANES + l - 1];
-        g[3] = lds[(s13 * 2 + 1) * FAR_LANES + l + 1];
-        g[5] = lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + l - 1];
-        g[6] = lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + l + 1];
-        g[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + l + 1];
-        g[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + l - 1];
-        if (ic == i_own) {
-            g[0] = own0;
-            g[2] = own2;
-            g[4] = own4;
-        } else {
-            const int e = far_side_col(ic, nx);
-            g[0] = lds[FAR_OSIDE + ((jc & (FAR_RSIDE - 1)) * 3 + 0) * 4 + e];
-            g[2] = lds[FAR_OSIDE + (((jc - 1) & (FAR_RSIDE - 1)) * 3 + 1) * 4 + e];
-            g[4] = lds[FAR_OSIDE + (((jc + 1) & (FAR_RSIDE - 1)) * 3 + 2) * 4 + e];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) f[k] = lift(g[k], W[k]);
-    }
-};
 
 template <typename T>
 __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
@@ -563,8 +532,10 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
     const int i = c0 + L;
     const int wi0 = D.gi0, wi1 = D.gi0 + D.gw - 1, wj0 = D.gj0, wj1 = D.gj0 + D.gh - 1;
     const bool col_in = i >= 0 && i < nx && i <= b;
-    const bool own_col = i >= a && i < b;
-    const int side = col_in ? far_side_col(i, nx) : -1;
+    // the strip's columns but the outer ones
+    const bool own_col = i >= a && i < b && i >= 1 && i <= nx - 2;
+    // columns whose step n the outer columns' update pulls
+    const bool ring_nb = (i >= 1 && i <= 3) || (i >= nx - 4 && i <= nx - 2);
     double* mac1 = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
     double* mac2 = n + 1 >= kc.flag_from ? D.macro[(n + 1) & 1] : nullptr;
     GAS T* __restrict__ out2 = pops<T>(D, dst);
@@ -573,42 +544,31 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
     // (h0, h2a), k = 2 of the row before that (h2b), and of the current row
     T h0 = 0, h2a = 0, h2b = 0, c0v = 0, c2v = 0, c4v = 0;
 
-    // phase 2 at output row q for this lane (a strip column); the registers
-    // and rings hold the pull of row r - 1 (q itself, or the wall's neighbour)
+    // phase 2 at output row q for this lane
     auto phase2 = [&](int q) {
-        if (!own_col) return;
-        if (i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1) return;
+        const int s13 = q % FAR_R13, s56 = (q - 1) & (FAR_R56 - 1), s78 = (q + 1) & (FAR_R78 - 1);
+        T p[9], out[9];
+        p[0] = h0;
+        p[1] = lds[(s13 * 2 + 0) * FAR_LANES + L - 1];
+        p[2] = h2b;
+        p[3] = lds[(s13 * 2 + 1) * FAR_LANES + L + 1];
+        p[4] = c4v;
+        p[5] = lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + L - 1];
+        p[6] = lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + L + 1];
+        p[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L + 1];
+        p[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L - 1];
         const long long c = (long long)q * nx + i;
         double rho, u, v;
-        if (i == 0 || i == nx - 1 || q == 0 || q == ny - 1) {
-            const FarPull<T> fp{lds, c0, i, nx, h0, h2b, c4v, D.W};
-            double out[9];
-            ring_node(D, fp, i, q, out, rho, u, v);
+        interior_step(D, p, out, rho, u, v);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) lower(out[k], D.W[k], out2 + k * N + c);
-        } else {
-            const int s13 = q % FAR_R13, s56 = (q - 1) & (FAR_R56 - 1), s78 = (q + 1) & (FAR_R78 - 1);
-            T p[9], out[9];
-            p[0] = h0;
-            p[1] = lds[(s13 * 2 + 0) * FAR_LANES + L - 1];
-            p[2] = h2b;
-            p[3] = lds[(s13 * 2 + 1) * FAR_LANES + L + 1];
-            p[4] = c4v;
-            p[5] = lds[FAR_O56 + (s56 * 2 + 0) * FAR_LANES + L - 1];
-            p[6] = lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + L + 1];
-            p[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L + 1];
-            p[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L - 1];
-            interior_step(D, p, out, rho, u, v);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], out2 + k * N + c);
-        }
+        for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], out2 + k * N + c);
         if (mac2) { mac2[c] = rho; mac2[N + c] = u; mac2[2 * N + c] = v; }
     };
 
     // Phase 1 of row r at this lane: 0 nothing, 1 an interior node, 2 a ring
-    // node, whose step n k_ring has put into the middle plane.  The loads of
-    // both are issued one row ahead, so they overlap the barrier and phase 2
-    // of the row before.
+    // node, whose step n is in the middle plane.  The loads of both are
+    // issued one row ahead, so they overlap the barrier and phase 2 of the
+    // row before.
     auto kind1 = [&](int r) -> int {
         if (r < 0 || r >= ny || !col_in) return 0;
         const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
@@ -627,8 +587,9 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             for (int k = 0; k < 9; ++k) pre[k] = outm[k * N + c];
         }
     };
-    // the top wall takes its step with row ny-2, so the sweep ends there
-    const int rend = j1 == ny - 1 ? ny - 1 : j1 + 1;
+    // phase 2 of row q needs phase 1 of rows q-1..q+1 (the top wall row is
+    // the last phase-1 row)
+    const int rend = min(j1 + 1, ny - 1);
     int kpre = kind1(j0 - 1);
     load1(kpre, j0 - 1);
     for (int r = j0 - 1; r <= rend; ++r) {
@@ -657,52 +618,36 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             lds[FAR_O56 + (s56 * 2 + 1) * FAR_LANES + L] = st[6];
             lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L] = st[7];
             lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L] = st[8];
-            if (side >= 0) {
-                const int ss = r & (FAR_RSIDE - 1);
-                lds[FAR_OSIDE + (ss * 3 + 0) * 4 + side] = st[0];
-                lds[FAR_OSIDE + (ss * 3 + 1) * 4 + side] = st[2];
-                lds[FAR_OSIDE + (ss * 3 + 2) * 4 + side] = st[4];
-            }
             c0v = st[0];
             c2v = st[2];
             c4v = st[4];
             const bool own = own_col && r >= j0 && r <= j1 && !in_w && kr == 1;
-            if (own && i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1) {
+            if (own && (ring_nb || r <= 2 || r >= ny - 3 ||
+                        (i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1))) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // F's layer around W, step n
+                for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // step n for the window and the ring
             }
             if (own && mac1) { mac1[c] = rho; mac1[N + c] = u; mac1[2 * N + c] = v; }
         }
         lds_barrier();  // the next row's pulls stay in flight across it
-        // phase 2: step n+1 at row r-1; the bottom wall row with row 1, the
-        // top wall row with row ny-2
+        // phase 2: step n+1 at row r-1 (ring nodes excluded)
         const int q = r - 1;
-        // (one call site in a loop of two: each inlined copy of the outer
-        // nodes' update costs registers)
-        const int qw = (q == 1 && j0 == 0) ? 0 : (q == ny - 2 && j1 == ny - 1) ? ny - 1 : -1;
-#pragma unroll 1
-        for (int t = 0; t < 2; ++t) {
-            if (t == 0 ? (q >= j0 && q <= j1 && q != 0) : qw >= 0) phase2(t == 0 ? q : qw);
-        }
+        if (own_col && q >= max(j0, 1) && q <= min(j1, ny - 2) && !(i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1))
+            phase2(q);
         h2b = h2a;
         h2a = c2v;
         h0 = c0v;
     }
 }
-// FP64: 133 registers, three workgroups per CU.  FP32: capped at 96
-// registers (no spills) so that five workgroups fit per CU, as its LDS allows.
-template <>
-__global__ __launch_bounds__(FAR_LANES) void k_far2<double>(const DevCase* __restrict__ C,
-                                                            const CaseClock* __restrict__ K, int off, int psel)
+
+template <typename T>
+__global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                                    int off, int psel)
 {
-    far2_body<double>(C, K, off, psel);
+    far2_body<T>(C, K, off, psel);
 }
-template <>
-__global__ __launch_bounds__(FAR_LANES) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_far2<float>(
-    const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel)
-{
-    far2_body<float>(C, K, off, psel);
-}
+template __global__ void k_far2<double>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_far2<float>(const DevCase*, const CaseClock*, int, int);
 
 int lattice_tiles(int nx, int ny, int npt)
 {
