@@ -11,7 +11,8 @@ exactly K steps between two device synchronisations.
 value = lattice updates per second of whole steps (MLUPS, nx*ny per step).
 roofline = the dominant kernel.  Steps are taken in two-step passes (the
 default launch shape): the far-field kernel k_far2 advances every node
-outside the lattice window that holds the body by two updates per launch, so
+outside the lattice window that holds the body (and off the outer ring,
+which takes its steps in kernels of its own) by two updates per launch, so
 its algorithmic bytes are 2 x 144 per far-field node (9 FP64 populations read
 + 9 written per update) over its mean launch time, timed with HIP events on
 the stream it runs on.  It reads and writes each population once per two
@@ -83,7 +84,7 @@ PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every pha
 LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
 # the same for a k_far2<double> launch (two updates of the far field):
 # FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) of a pass that records nothing
-FAR_TRAFFIC_BYTES = {(4096, 2048): (395203 * 2 + 594673) * 1024}
+FAR_TRAFFIC_BYTES = {(4096, 2048): (440378 * 2 + 616294) * 1024}
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
@@ -702,8 +703,8 @@ def main():
     wi0, wj0, ww, wh = g.lattice_window()
     if tm["far_passes"] > 0:
         # two-step passes: the far-field kernel advances every node outside
-        # the lattice window by two updates per launch
-        n_far = cells - ww * wh
+        # the lattice window and off the outer ring by two updates per launch
+        n_far = cells - ww * wh - (2 * p.nx + 2 * p.ny - 4)
         kern = "k_far2<%s>" % ("float" if fp32 else "double")
         bytes_launch = 2 * n_far * bpn
         lat_s = tm["far_pass_ms"] * 1e-3
