@@ -291,16 +291,8 @@ int lbmib_create(const lbmib_params* p, const double* xb0, const double* yb0, lb
     for (int k = 0; k < 9; ++k) {
         D.W[k] = feq_ref(1.0, 0.0, 0.0, k);
         D.rho0 += D.W[k];
-        D.K.s[k] = (float)p->s[k];
     }
-    {
-        const double* W = D.W;
-        const double e_w = -4 * W[0] - W[1] - W[2] - W[3] - W[4] + 2 * (W[5] + W[6] + W[7] + W[8]);
-        const double eps_w = 4 * W[0] - 2 * (W[1] + W[2] + W[3] + W[4]) + W[5] + W[6] + W[7] + W[8];
-        D.K.c0 = (float)(-D.rho0);
-        D.K.c1 = (float)(-2.0 * D.rho0 - e_w);
-        D.K.c2 = (float)(D.rho0 - eps_w);
-    }
+    D.K = make_devconst(p->s, D.W);
     D.smooth = p->smooth;
     D.nw = nw; D.ppw = ppw; D.spw = ppw - 1; D.nseg = nw * (ppw - 1);
     D.batch = GEO_BATCH;

@@ -582,8 +582,12 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 //
 // An engine of one case that takes two-step passes (par >= 0) names the
 // planes of every step itself: the state of step m lies in plane
-// (m + par) & 1, and each pass flips par.  Passes start at even positions of
-// the geometry batch, so they never straddle a batch boundary.
+// (m + par) & 1, and each pass flips par.  Passes start at any position; a
+// pass whose second step opens a geometry batch takes that batch's waits at
+// its start (the geometry of the batch it enters was built a batch earlier),
+// and the boundary's own work (the forces of the batch just finished, the
+// geometry of the batch after next into the finished batch's slots) follows
+// the pass, waiting for both of its steps.
 int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par)
 {
     const int B = E->B, BR = Engine::BRING;
@@ -593,55 +597,70 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
     const int Eper = Engine::EPER;
     const int n0 = par ? E->members[0]->steps_done + 1 : 0;
     if (every > 0) E->samp_kind.assign((size_t)(len + every - 1) / every, 2);
+    // Work of the geometry-batch boundary at window offset off, after every
+    // step enqueued so far: the forces of offsets fupto..off-1, then on s2
+    // (after the last enqueued step, whose batch's slots the next geometry
+    // reuses) the geometry of the batch after the one starting, and for a
+    // fresh case at offset 0 that of its first batch before.  Returns the
+    // events the step at off waits for (wfirst/wnodes: the first batch;
+    // wnext: the batch starting, built at the boundary before).
+    auto boundary = [&](int off, hipEvent_t& wfirst, hipEvent_t& wnext, hipEvent_t& wnodes) -> int {
+        if (off > fupto) {
+            int rc = enqueue_forces(E, d, fupto, off - 1, E->gstep - 1);
+            if (rc) return rc;
+            fupto = off;
+        }
+        hipStream_t st = E->s[2];
+        if (off > 0) {
+            HIP_TRY(hipStreamWaitEvent(st, E->ev_step[(E->gstep - 1) % Engine::RING], 0));
+            HIP_TRY(hipStreamWaitEvent(st, E->ev_ib[(E->gstep - 1) % Engine::RING], 0));
+        }
+        const long long bi = E->gbound++;
+        hipEvent_t* gev = nullptr;
+        if (every > 0) {
+            while (E->gevpool.size() < (size_t)3 * (E->gev_used + 1)) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
+                E->gevpool.push_back(e);
+            }
+            gev = &E->gevpool[(size_t)3 * E->gev_used++];
+            HIP_TRY(hipEventRecord(gev[0], st));
+        }
+        int rc;
+        if (fresh && off == 0) {
+            // the lattice update needs only the node sets; the IB
+            // solves wait for the factorisation as well
+            if ((rc = launch_geometry(E, d, off, GEO_FIRST))) return rc;
+            HIP_TRY(hipEventRecord(E->ev_nodes[bi % BR], st));
+            wnodes = E->ev_nodes[bi % BR];
+            if ((rc = launch_factor(E, d, off, GEO_FIRST, nullptr))) return rc;
+            HIP_TRY(hipEventRecord(E->ev_first[bi % BR], st));
+            wfirst = E->ev_first[bi % BR];
+        }
+        if (!E->per_step) {
+            if ((rc = launch_geometry(E, d, off, GEO_NEXT))) return rc;
+            if ((rc = launch_factor(E, d, off, GEO_NEXT, gev))) return rc;
+            HIP_TRY(hipEventRecord(E->ev_next[bi % BR], st));
+            wnext = prev_next;
+            prev_next = E->ev_next[bi % BR];
+        } else if (gev) {
+            HIP_TRY(hipEventRecord(gev[1], st));
+            HIP_TRY(hipEventRecord(gev[2], st));
+        }
+        return 0;
+    };
+    auto is_boundary = [&](int off) { return (phase + off) % B == 0 || (fresh && off == 0); };
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
-        if ((phase + off) % B == 0 || (fresh && off == 0)) {
-            if (off > fupto) {
-                int rc = enqueue_forces(E, d, fupto, off - 1, E->gstep - 1);
-                if (rc) return rc;
-                fupto = off;
-            }
-            hipStream_t st = E->s[2];
-            // the next batch reuses the slots of the one just finished: its
-            // lattice updates read their node grids
-            if (off > 0) {
-                HIP_TRY(hipStreamWaitEvent(st, E->ev_step[(E->gstep - 1) % Engine::RING], 0));
-                HIP_TRY(hipStreamWaitEvent(st, E->ev_ib[(E->gstep - 1) % Engine::RING], 0));
-            }
-            const long long bi = E->gbound++;
-            hipEvent_t* gev = nullptr;
-            if (every > 0) {
-                while (E->gevpool.size() < (size_t)3 * (E->gev_used + 1)) {
-                    hipEvent_t e;
-                    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing events
-                    E->gevpool.push_back(e);
-                }
-                gev = &E->gevpool[(size_t)3 * E->gev_used++];
-                HIP_TRY(hipEventRecord(gev[0], st));
-            }
-            int rc;
-            if (fresh && off == 0) {
-                // the lattice update needs only the node sets; the IB
-                // solves wait for the factorisation as well
-                if ((rc = launch_geometry(E, d, off, GEO_FIRST))) return rc;
-                HIP_TRY(hipEventRecord(E->ev_nodes[bi % BR], st));
-                wnodes = E->ev_nodes[bi % BR];
-                if ((rc = launch_factor(E, d, off, GEO_FIRST, nullptr))) return rc;
-                HIP_TRY(hipEventRecord(E->ev_first[bi % BR], st));
-                wfirst = E->ev_first[bi % BR];
-            }
-            if (!E->per_step) {
-                if ((rc = launch_geometry(E, d, off, GEO_NEXT))) return rc;
-                if ((rc = launch_factor(E, d, off, GEO_NEXT, gev))) return rc;
-                HIP_TRY(hipEventRecord(E->ev_next[bi % BR], st));
-                wnext = prev_next;
-                prev_next = E->ev_next[bi % BR];
-            } else if (gev) {
-                HIP_TRY(hipEventRecord(gev[1], st));
-                HIP_TRY(hipEventRecord(gev[2], st));
-            }
+        if (is_boundary(off)) {
+            if (int rc = boundary(off, wfirst, wnext, wnodes)) return rc;
         }
-        const bool pair = par && (phase + off) % 2 == 0 && off + 1 < len;
+        const bool pair = par && off + 1 < len;
+        // a pass entering a new batch at its second step waits at its start
+        // for that batch's geometry, built at the boundary before (at most one
+        // boundary falls inside a pass: batches are longer than two steps)
+        const bool straddle = pair && is_boundary(off + 1);
+        if (straddle && !wnext) wnext = prev_next;
         hipEvent_t* ev = nullptr;
         int q = -1;
         if (every > 0 && off % every == 0) q = off / every;
@@ -662,6 +681,10 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X);
             *par ^= 1;
             ++off;
+            if (!rc && straddle) {
+                hipEvent_t f2 = nullptr, n2 = nullptr, w2 = nullptr;
+                rc = boundary(off, f2, n2, w2);
+            }
         } else {
             const int n = n0 + off;
             const int psel = par ? make_psel((n - 1 + *par) & 1, (n + *par) & 1) : -1;

@@ -197,69 +197,88 @@ LBM_HD void nee_node(double rhe, double ue, double ve, double rhn, double un, do
 struct DevConst {
     float s[9];
     float c0, c1, c2;
+    float oms[9];  // 1 - s[k]
+    float s0c0;    // s[0] * c0
 };
 
-// f = M^-1 m in FP32 with the exact rational inverse M^T diag(1/9, 1/36, 1/36,
-// 1/6, 1/12, 1/6, 1/12, 1/4, 1/4).
-LBM_HD void mrt_inverse_f32(const float* m, float* f)
+// The constants of the FP32 form for relaxation rates s and rest state W
+// (rho0 = sum W, summed in index order).
+LBM_HD DevConst make_devconst(const double* s, const double* W)
 {
-#pragma clang fp contract(fast)
-    const float a0 = m[0] * (1.0f / 9.0f);
-    const float r1 = m[1] * (1.0f / 36.0f), r2 = m[2] * (1.0f / 18.0f);
-    const float p3 = m[3] * (1.0f / 6.0f), p4 = m[4] * (1.0f / 6.0f);
-    const float p5 = m[5] * (1.0f / 6.0f), p6 = m[6] * (1.0f / 6.0f), p7 = m[7] * 0.25f;
-    const float s1 = m[1] * (1.0f / 18.0f), s2 = m[2] * (1.0f / 36.0f);
-    const float q4 = m[4] * (1.0f / 12.0f), q6 = m[6] * (1.0f / 12.0f), p8 = m[8] * 0.25f;
-    f[0] = a0 - m[1] * (1.0f / 9.0f) + m[2] * (1.0f / 9.0f);
-    f[1] = a0 - r1 - r2 + p3 - p4 + p7;
-    f[2] = a0 - r1 - r2 + p5 - p6 - p7;
-    f[3] = a0 - r1 - r2 - p3 + p4 + p7;
-    f[4] = a0 - r1 - r2 - p5 + p6 - p7;
-    f[5] = a0 + s1 + s2 + p3 + q4 + p5 + q6 + p8;
-    f[6] = a0 + s1 + s2 - p3 - q4 + p5 + q6 - p8;
-    f[7] = a0 + s1 + s2 - p3 - q4 - p5 - q6 + p8;
-    f[8] = a0 + s1 + s2 + p3 + q4 - p5 - q6 - p8;
+    DevConst K;
+    double rho0 = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        rho0 += W[k];
+        K.s[k] = (float)s[k];
+    }
+    const double e_w = -4 * W[0] - W[1] - W[2] - W[3] - W[4] + 2 * (W[5] + W[6] + W[7] + W[8]);
+    const double eps_w = 4 * W[0] - 2 * (W[1] + W[2] + W[3] + W[4]) + W[5] + W[6] + W[7] + W[8];
+    K.c0 = (float)(-rho0);
+    K.c1 = (float)(-2.0 * rho0 - e_w);
+    K.c2 = (float)(rho0 - eps_w);
+    for (int k = 0; k < 9; ++k) K.oms[k] = 1.0f - K.s[k];
+    K.s0c0 = K.s[0] * K.c0;
+    return K;
 }
 
-// M f in FP32 (the rows of mrt_forward), products and sums fused where they
-// meet: the FP32 form's collision has no operation order to reproduce.
-LBM_HD void mrt_forward_f32(const float* f, float* m)
-{
-#pragma clang fp contract(fast)
-    m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-    m[1] = -4 * f[0] - f[1] - f[2] - f[3] - f[4] + 2 * f[5] + 2 * f[6] + 2 * f[7] + 2 * f[8];
-    m[2] = 4 * f[0] - 2 * f[1] - 2 * f[2] - 2 * f[3] - 2 * f[4] + f[5] + f[6] + f[7] + f[8];
-    m[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
-    m[4] = -2 * f[1] + 2 * f[3] + f[5] - f[6] - f[7] + f[8];
-    m[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
-    m[6] = -2 * f[2] + 2 * f[4] + f[5] + f[6] - f[7] - f[8];
-    m[7] = f[1] - f[2] + f[3] - f[4];
-    m[8] = f[5] - f[6] + f[7] - f[8];
-}
-
-// Unforced collision of one node in deviation form, with fused multiply-adds;
-// returns drho, jx, jy.
+// Unforced collision of one node in deviation form; returns drho, jx, jy.
+// The FP32 form has no operation order to reproduce, so the transforms take
+// the fewest operations: M g from the sums and differences of opposite
+// populations, each moment relaxed as m (1 - s) + s m_eq, and the exact
+// rational inverse M^T diag(1/9, 1/36, 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4)
+// from shared partial sums, products and sums fused where they meet (77
+// operations a node instead of the 111 of the row-by-row forms).
 LBM_HD void collide_dev_f32(float* g, const DevConst& K, float& drho, float& jx, float& jy)
 {
 #pragma clang fp contract(fast)
-    float m[9], t[9];
-    mrt_forward_f32(g, m);
-    drho = m[0];
-    jx = m[3];
-    jy = m[5];
+    // m = M g
+    const float a = g[1] + g[3], b = g[2] + g[4], c = g[5] + g[7], d = g[6] + g[8];
+    const float e = g[1] - g[3], h = g[2] - g[4], p = g[5] - g[7], q = g[6] - g[8];
+    const float sa = a + b, sd = c + d, pmq = p - q, ppq = p + q;
+    const float m0 = g[0] + sa + sd;
+    const float m1 = -4.0f * g[0] + (2.0f * sd - sa);
+    const float m2 = 4.0f * g[0] + (sd - 2.0f * sa);
+    const float m3 = e + pmq;
+    const float m4 = -2.0f * e + pmq;
+    const float m5 = h + ppq;
+    const float m6 = -2.0f * h + ppq;
+    const float m7 = a - b;
+    const float m8 = c - d;
+    drho = m0;
+    jx = m3;
+    jy = m5;
+    // relaxation towards m_eq(drho, j) - M W
     const float jxx = jx * jx, jyy = jy * jy, j2 = jxx + jyy;
-    t[0] = K.c0;
-    t[1] = -2.0f * drho + 3.0f * j2 + K.c1;
-    t[2] = drho - 3.0f * j2 + K.c2;
-    t[3] = 0.0f;
-    t[4] = -jx;
-    t[5] = 0.0f;
-    t[6] = -jy;
-    t[7] = jxx - jyy;
-    t[8] = jx * jy;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) t[k] = m[k] + (t[k] - m[k]) * K.s[k];
-    mrt_inverse_f32(t, g);
+    const float q1 = -2.0f * drho + (3.0f * j2 + K.c1);
+    const float q2 = drho + (-3.0f * j2 + K.c2);
+    const float t0 = m0 * K.oms[0] + K.s0c0;
+    const float t1 = m1 * K.oms[1] + K.s[1] * q1;
+    const float t2 = m2 * K.oms[2] + K.s[2] * q2;
+    const float t3 = m3 * K.oms[3];
+    const float t4 = m4 * K.oms[4] - K.s[4] * jx;
+    const float t5 = m5 * K.oms[5];
+    const float t6 = m6 * K.oms[6] - K.s[6] * jy;
+    const float t7 = m7 * K.oms[7] + K.s[7] * (jxx - jyy);
+    const float t8 = m8 * K.oms[8] + K.s[8] * (jx * jy);
+    // g = M^-1 t
+    const float a0 = t0 * (1.0f / 9.0f);
+    const float A = a0 - t1 * (1.0f / 36.0f) - t2 * (1.0f / 18.0f);
+    const float B = a0 + t1 * (1.0f / 18.0f) + t2 * (1.0f / 36.0f);
+    const float Ap = A + t7 * 0.25f, Am = A - t7 * 0.25f;
+    const float Bp = B + t8 * 0.25f, Bm = B - t8 * 0.25f;
+    const float dx = t3 - t4, dy = t5 - t6;
+    const float U = t3 * (1.0f / 6.0f) + t4 * (1.0f / 12.0f);
+    const float V = t5 * (1.0f / 6.0f) + t6 * (1.0f / 12.0f);
+    const float uv = U + V, vu = V - U;
+    g[0] = a0 + (t2 - t1) * (1.0f / 9.0f);
+    g[1] = Ap + dx * (1.0f / 6.0f);
+    g[3] = Ap - dx * (1.0f / 6.0f);
+    g[2] = Am + dy * (1.0f / 6.0f);
+    g[4] = Am - dy * (1.0f / 6.0f);
+    g[5] = Bp + uv;
+    g[7] = Bp - uv;
+    g[6] = Bm + vu;
+    g[8] = Bm - vu;
 }
 
 // ---- delta kernel ----

@@ -4,8 +4,9 @@ in one pass over the far field, the window one step at a time with the IB
 chain between.  Every node keeps the single-step arithmetic, so the bar is
 bit-for-bit equality with the same case stepped one step at a time
 (fuse_steps = 0) -- populations, rho/u/v, force history, residual -- for
-call lengths that start passes at every position (odd chunks leave single
-steps), in both precisions, across a restart, and for a case that runs
+call lengths that start passes at every position (passes straddle the
+geometry-batch boundaries; an odd chunk leaves one single step), in both
+precisions, across a restart, and for a case that runs
 passes alone and then joins a batch (its state then lies in the other
 plane).  Parity with the oracle is covered by the single-case parity tests,
 which run with the default shape.
@@ -60,6 +61,27 @@ def test_passes_are_taken(lbmib_mod):
     g.evolution(8)
     t = g.timings()
     assert t["far_passes"] == 0 and t["lattice_steps"] == 8
+
+
+def test_passes_start_at_any_position(lbmib_mod):
+    # a window that starts at an odd position inside the geometry batch takes
+    # passes from its first step; the pass whose second step opens the next
+    # batch straddles the boundary
+    case = cases.config(1, n_steps=60, scale_down=4)
+    g = lbmib_mod.LBMIB(case)
+    g.evolution(5)
+    g.set_profiling(1)
+    g.evolution(20)  # steps 6..25: the pass (16, 17) straddles a boundary
+    t = g.timings()
+    assert t["far_passes"] == 10 and t["lattice_steps"] == 0
+    g.set_profiling(1)
+    g.evolution(15)  # steps 26..40: seven passes and one single step
+    t = g.timings()
+    assert t["far_passes"] == 7 and t["lattice_steps"] == 1
+    ref = run(lbmib_mod, case, 0, 0, [5, 20, 15])
+    same(g, ref)
+    g.close()
+    ref.close()
 
 
 def test_full_size_first_steps_equal_single_steps(lbmib_mod):
