@@ -218,15 +218,110 @@ Dims dims_of(const Engine* E)
     return d;
 }
 
-// Whether an engine takes two-step passes: one case, prescribed motion (the
-// dynamics step one at a time), and the lattice window of the run at least
-// 4 nodes away from every edge (the window's single steps then never touch
-// the outer ring).
+// Whether an engine takes two-step passes: prescribed motion (the dynamics
+// step one at a time), members of one precision (every member a pass) whose
+// lattice windows lie at least 4 nodes away from every edge (the window's
+// single steps then never touch the outer ring), and lattices narrower than
+// 2^16 columns (the far tiles pack two columns in one int).
 bool fuse_ok(const Engine* E)
 {
-    if (!E->shape.fuse || E->shape.graph || E->per_step || E->members.size() != 1) return false;
-    const DevCase& D = E->members[0]->D;
-    return D.gi0 >= 4 && D.gj0 >= 4 && D.gi0 + D.gw <= D.nx - 4 && D.gj0 + D.gh <= D.ny - 4;
+    if (!E->shape.fuse || E->shape.graph || E->per_step || E->members.empty()) return false;
+    for (const lbmib_handle* h : E->members) {
+        const DevCase& D = h->D;
+        if (D.nx >= (1 << 16)) return false;
+        if (!(D.gi0 >= 4 && D.gj0 >= 4 && D.gi0 + D.gw <= D.nx - 4 && D.gj0 + D.gh <= D.ny - 4)) return false;
+    }
+    return true;
+}
+
+// Workgroup tables of the two-step passes over every member.  Far tiles:
+// strips of at most FAR_OUT columns (lane L of a strip at column a-1+L) and
+// segments of about 5 rows.  Many short segments beat one workgroup per CU
+// slot with tall ones (4096x2048, FP64: 0.279 ms per pass at 410 segments,
+// 0.291 ms at 256, 0.49 ms at 28 with an older kernel): the pass is bound by
+// the rows the workgroups have in flight, not by the halo rows that short
+// segments read again, and many workgroups balance the load at the end.
+// LBMIB_FAR_SEGS overrides the segment count of every member.  The window,
+// ring and window + ring tables take window_blocks, ring_blocks1 and their
+// sum workgroups per member.
+int build_pass_tables(Engine* E)
+{
+    std::vector<int> key;
+    const char* es = getenv("LBMIB_FAR_SEGS");
+    key.push_back(es ? atoi(es) : -1);
+    for (const lbmib_handle* h : E->members) {
+        const DevCase& D = h->D;
+        for (int v : {D.nx, D.ny, D.gi0, D.gj0, D.gw, D.gh}) key.push_back(v);
+    }
+    if (key == E->pass_key && E->dfar) return 0;
+    std::vector<int4> far;
+    std::vector<int2> win, ring, wr;
+    for (size_t c = 0; c < E->members.size(); ++c) {
+        const DevCase& D = E->members[c]->D;
+        const int nx = D.nx, ny = D.ny;
+        const int strips = (nx + FAR_OUT - 1) / FAR_OUT;
+        int segs = std::max(1, ny / 5);
+        if (es) segs = std::max(1, std::min(ny / 4, atoi(es)));
+        for (int sg = 0; sg < segs; ++sg) {
+            const int j0 = (int)((long long)sg * ny / segs), j1 = (int)((long long)(sg + 1) * ny / segs) - 1;
+            for (int st = 0; st < strips; ++st) {
+                const int a = (int)((long long)st * nx / strips), b = (int)((long long)(st + 1) * nx / strips);
+                far.push_back(make_int4((int)c, a | (b << 16), j0, j1));
+            }
+        }
+        const int wb = window_blocks(D.gw, D.gh), rb = ring_blocks1(nx, ny);
+        for (int b = 0; b < wb; ++b) win.push_back(make_int2((int)c, b));
+        for (int b = 0; b < rb; ++b) ring.push_back(make_int2((int)c, b));
+        for (int b = 0; b < wb + rb; ++b) wr.push_back(make_int2((int)c, b));
+    }
+    std::vector<int2> all;
+    all.insert(all.end(), win.begin(), win.end());
+    all.insert(all.end(), ring.begin(), ring.end());
+    all.insert(all.end(), wr.begin(), wr.end());
+    int rc = dev_reserve(&E->dfar, E->far_cap, far.size());
+    if (!rc) rc = dev_reserve(&E->dwin, E->win_cap, all.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(E->dfar, far.data(), sizeof(int4) * far.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->dwin, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+    E->nfar = (int)far.size();
+    E->nwin = (int)win.size();
+    E->nring = (int)ring.size();
+    E->nwr = (int)wr.size();
+    E->pass_key.swap(key);
+    return 0;
+}
+
+// The case table goes to the device only when it changed since the last
+// window (byte copies, so that padding compares equal too).
+int upload_cases(Engine* E)
+{
+    const size_t nc = E->members.size();
+    std::vector<unsigned char> tab(sizeof(DevCase) * nc);
+    for (size_t c = 0; c < nc; ++c) std::memcpy(&tab[c * sizeof(DevCase)], &E->members[c]->D, sizeof(DevCase));
+    if (tab != E->tab_bytes) {
+        HIP_TRY(hipMemcpy(E->dcases, tab.data(), tab.size(), hipMemcpyHostToDevice));
+        E->tab_bytes.swap(tab);
+    }
+    return 0;
+}
+
+// Before a window of two-step passes every member that steps in it holds its
+// state in plane 0 (the planes named by a launch are then the same for all
+// members): a member whose state lies in plane 1 swaps its two state planes
+// and its parity.  Returns whether any member swapped.
+bool normalize_planes(Engine* E, const std::vector<int>& len_end)
+{
+    bool any = false;
+    for (size_t c = 0; c < E->members.size(); ++c) {
+        lbmib_handle* h = E->members[c];
+        if (len_end[c] <= h->steps_done || state_plane(h) == 0) continue;
+        DevCase& D = h->D;
+        std::swap(D.f[0], D.f[1]);
+        std::swap(D.g[0], D.g[1]);
+        D.pofs ^= 1;
+        any = true;
+    }
+    return any;
 }
 
 int engine_prepare(Engine* E)
@@ -257,30 +352,14 @@ int engine_prepare(Engine* E)
     }
     int rc = 0;
     if (fuse_ok(E)) {
-        // the middle plane, and the far-field grid: strips of at most FAR_OUT
-        // columns, segments of about 5 rows.  Many short segments beat one
-        // workgroup per CU slot with tall ones (4096x2048, FP64: 0.279 ms per
-        // pass at 410 segments, 0.291 ms at 256, 0.49 ms at 28 with an older
-        // kernel): the pass is bound by the rows the workgroups have in
-        // flight, not by the halo rows that short segments read again, and
-        // many workgroups balance the load at the end.  LBMIB_FAR_SEGS
-        // overrides.
-        if ((rc = ensure_mid_plane(E->members[0]))) return rc;
-        const DevCase& D = E->members[0]->D;
-        E->far_strips = (D.nx + FAR_OUT - 1) / FAR_OUT;
-        E->far_segs = std::max(1, D.ny / 5);
-        if (const char* e = getenv("LBMIB_FAR_SEGS")) E->far_segs = std::max(1, std::min(D.ny / 4, atoi(e)));
+        // every member's middle plane, and the workgroup tables
+        for (lbmib_handle* h : E->members)
+            if ((rc = ensure_mid_plane(h))) return rc;
+        if ((rc = build_pass_tables(E))) return rc;
         HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)far2_smem_bytes(true)));
     }
-    // the case table goes to the device only when it changed since the
-    // last window (byte copies, so that padding compares equal too)
-    std::vector<unsigned char> tab(sizeof(DevCase) * nc);
-    for (size_t c = 0; c < nc; ++c) std::memcpy(&tab[c * sizeof(DevCase)], &E->members[c]->D, sizeof(DevCase));
-    if (tab != E->tab_bytes) {
-        HIP_TRY(hipMemcpy(E->dcases, tab.data(), tab.size(), hipMemcpyHostToDevice));
-        E->tab_bytes.swap(tab);
-    }
+    if ((rc = upload_cases(E))) return rc;
     if (!E->tiles_valid) {
         std::vector<int2> t;
         build_tiles(E, t);
@@ -516,8 +595,9 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     const bool f64 = d.prec == LBMIB_PREC_FP64;
     const int Y = X ^ 1, M = 2;
     const int pa = make_psel(X, M), pb = make_psel(M, Y);
-    if (!planes_ok(E->members[0]->D, {X, Y, M}))
-        return fail(LBMIB_E_STATE, "two-step pass: planes %d, %d, %d not all allocated", X, Y, M);
+    for (const lbmib_handle* h : E->members)
+        if (!planes_ok(h->D, {X, Y, M}))
+            return fail(LBMIB_E_STATE, "two-step pass: planes %d, %d, %d not all allocated", X, Y, M);
     // IB chain of step n on the solves' stream
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[(gs - 1) % R], 0));
     if (wfirst) HIP_TRY(hipStreamWaitEvent(s1, wfirst, 0));
@@ -527,18 +607,18 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (int rc = launch_solve(E, d, s1, off)) return rc;
     if (ev) HIP_TRY(hipEventRecord(ev[3], s1));
     const int band_x = std::max(1, (3 * d.maxns + 255) / 256);
-    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, 1), dim3(256), 0, s1, C, K, off, pa);
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, s1, C, K, off, pa);
     if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
     HIP_TRY(hipEventRecord(E->ev_ib[gs % R], s1));
     // the ring's step n into the middle plane: it reads only the ring's
     // neighbourhood of plane X, which the far pass (or lattice update) before
     // it on this stream wrote
-    const DevCase& D = E->members[0]->D;
-    const unsigned rblocks = (unsigned)ring_blocks1(D.nx, D.ny);
-    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(rblocks), dim3(256), 0, s0, C, K, off, pa);
+    const int2* wtab = E->dwin;
+    const int2* rtab = E->dwin + E->nwin;
+    const int2* wrtab = rtab + E->nring;
+    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(E->nring), dim3(256), 0, s0, C, K, rtab, off, pa);
     // window step n on the shared CUs
-    const unsigned wblocks = (unsigned)(((long long)D.gw * D.gh + 255) / 256);
-    const dim3 wg(wblocks);
+    const dim3 wg(E->nwin);
     if (after_prev) {
         HIP_TRY(hipStreamWaitEvent(s3, E->ev_step[(gs - 1) % R], 0));
         HIP_TRY(hipStreamWaitEvent(s3, E->ev_ib[(gs - 1) % R], 0));
@@ -546,31 +626,31 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (wfirst) HIP_TRY(hipStreamWaitEvent(s3, wnodes ? wnodes : wfirst, 0));
     if (wnext) HIP_TRY(hipStreamWaitEvent(s3, wnext, 0));
     if (ev) HIP_TRY(hipEventRecord(ev[5], s3));
-    hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s3, C, K, off, pa);
+    hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s3, C, K, wtab, off, pa);
     if (ev) HIP_TRY(hipEventRecord(ev[6], s3));
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
     // the far field, both steps
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    const dim3 fg((unsigned)E->far_strips, (unsigned)E->far_segs);
+    const dim3 fg((unsigned)E->nfar);
     const size_t fsm = far2_smem_bytes(f64);
     const int pf = X | (Y << 2) | (M << 4);  // k_far2's planes: src, dst, middle
-    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, off, pf);
-    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, off, pf);
+    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, off, pf);
+    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, off, pf);
     if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     // IB chain of step n+1
     HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[gs % R], 0));
     if (int rc = launch_rhs(E, d, s1, off + 1, pb)) return rc;
     if (int rc = launch_solve(E, d, s1, off + 1)) return rc;
-    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, 1), dim3(256), 0, s1, C, K, off + 1, pb);
+    hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, s1, C, K, off + 1, pb);
     HIP_TRY(hipEventRecord(E->ev_ib[(gs + 1) % R], s1));
     // window step n+1 right after the far pass in its stream (no hand-over on
     // the path from one pass to the next); W(n) and the IB collide of step n
     // have long finished by then.  The ring's step n+1 rides along.
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));
     HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[gs % R], 0));
-    hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(wblocks + rblocks), dim3(256), 0, s0,
-                       C, K, off + 1, pb, make_psel(M, Y), (int)wblocks);
+    hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(E->nwr), dim3(256), 0, s0, C, K, wrtab,
+                       off + 1, pb, make_psel(M, Y));
     HIP_TRY(hipEventRecord(E->ev_step[(gs + 1) % R], s0));
     E->gstep += 2;
     return 0;
@@ -580,22 +660,23 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 // inside its geometry batch, fresh = some case takes its first step at
 // offset 0.  With prof, samples every prof_every-th step.
 //
-// An engine of one case that takes two-step passes (par >= 0) names the
-// planes of every step itself: the state of step m lies in plane
-// (m + par) & 1, and each pass flips par.  Passes start at any position; a
+// An engine that takes two-step passes (par given) names the planes of every
+// step itself: every member that steps starts the window with its state in
+// plane 0 (normalize_planes), the step at offset off reads plane
+// (off + par) & 1, and each pass flips par (passes only at offsets below
+// pair_limit, while every member that steps in the window still steps).  Passes start at any position; a
 // pass whose second step opens a geometry batch takes that batch's waits at
 // its start (the geometry of the batch it enters was built a batch earlier),
 // and the boundary's own work (the forces of the batch just finished, the
 // geometry of the batch after next into the finished batch's slots) follows
 // the pass, waiting for both of its steps.
-int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par)
+int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par, int pair_limit)
 {
     const int B = E->B, BR = Engine::BRING;
     int fupto = 0;
     hipEvent_t prev_next = nullptr;
     const int every = prof ? E->prof_every : 0;
     const int Eper = Engine::EPER;
-    const int n0 = par ? E->members[0]->steps_done + 1 : 0;
     if (every > 0) E->samp_kind.assign((size_t)(len + every - 1) / every, 2);
     // Work of the geometry-batch boundary at window offset off, after every
     // step enqueued so far: the forces of offsets fupto..off-1, then on s2
@@ -655,7 +736,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         if (is_boundary(off)) {
             if (int rc = boundary(off, wfirst, wnext, wnodes)) return rc;
         }
-        const bool pair = par && off + 1 < len;
+        const bool pair = par && off + 1 < len && off + 1 < pair_limit;
         // a pass entering a new batch at its second step waits at its start
         // for that batch's geometry, built at the boundary before (at most one
         // boundary falls inside a pass: batches are longer than two steps)
@@ -677,7 +758,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         }
         int rc;
         if (pair) {
-            const int X = (n0 + off - 1 + *par) & 1;
+            const int X = (off + *par) & 1;
             rc = launch_pair(E, d, off, off > 0, wfirst, wnext, wnodes, ev, X);
             *par ^= 1;
             ++off;
@@ -686,8 +767,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
                 rc = boundary(off, f2, n2, w2);
             }
         } else {
-            const int n = n0 + off;
-            const int psel = par ? make_psel((n - 1 + *par) & 1, (n + *par) & 1) : -1;
+            const int psel = par ? make_psel((off + *par) & 1, (off + 1 + *par) & 1) : -1;
             rc = launch_step(E, d, off, off > 0, wfirst, wnext, wnodes, ev, psel);
         }
         if (rc) return rc;
@@ -768,7 +848,7 @@ int capture_graph(Engine* E, const Dims& d)
     for (int i = 1; i < Engine::NSTREAM; ++i)
         if (hipStreamWaitEvent(E->s[i], E->ev_fork, 0) != hipSuccess)
             return abort_capture(fail(LBMIB_E_HIP, "graph fork wait failed"));
-    int rc = launch_window(E, d, len, 0, false, false, nullptr);
+    int rc = launch_window(E, d, len, 0, false, false, nullptr, len);
     if (rc) return abort_capture(rc);
     for (int i = 1; i < Engine::NSTREAM; ++i) {
         if (hipEventRecord(E->ev_join[i], E->s[i]) != hipSuccess || hipStreamWaitEvent(s0, E->ev_join[i], 0) != hipSuccess)
@@ -837,7 +917,8 @@ void engine_destroy(Engine* E)
     (void)hipSetDevice(E->dev);
     destroy_streams(E);
     if (E->gexec) (void)hipGraphExecDestroy(E->gexec);
-    for (void* p : {(void*)E->dcases, (void*)E->dclock, (void*)E->dtiles, (void*)E->derr})
+    for (void* p : {(void*)E->dcases, (void*)E->dclock, (void*)E->dtiles, (void*)E->derr, (void*)E->dfar,
+                    (void*)E->dwin})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : E->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : E->gevpool) (void)hipEventDestroy(e);
@@ -932,6 +1013,15 @@ int engine_run(Engine* E, int k)
         }
         if (graph) len = kwin / (2 * B) * (2 * B);
         for (size_t c = 0; c < nc; ++c) len_end[c] = std::min(E->members[c]->steps_done + len, final_end[c]);
+        // passes only while every member that steps in the window steps
+        int pair_limit = len;
+        if (fuse) {
+            for (size_t c = 0; c < nc; ++c) {
+                const int mc = len_end[c] - E->members[c]->steps_done;
+                if (mc > 0) pair_limit = std::min(pair_limit, mc);
+            }
+            if (normalize_planes(E, len_end) && (rc = upload_cases(E))) return rc;
+        }
         if ((rc = upload_clocks(E, len_end, final_end))) return rc;
         if (graph) {
             if (!E->graph_valid) {
@@ -941,9 +1031,11 @@ int engine_run(Engine* E, int k)
             for (int r = 0; r < len / E->graph_len; ++r) HIP_TRY(hipGraphLaunch(E->gexec, E->s[0]));
         } else {
             if (every > 0) E->gev_used = 0;
-            int par = fuse ? E->members[0]->D.pofs : 0;
-            if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr))) return rc;
-            if (fuse) E->members[0]->D.pofs = par;
+            int par = 0;
+            if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr, pair_limit))) return rc;
+            if (fuse && par)
+                for (size_t c = 0; c < nc; ++c)
+                    if (len_end[c] > E->members[c]->steps_done) E->members[c]->D.pofs ^= 1;
             if (every > 0) {
                 const int nsamp = (len + every - 1) / every;
                 if ((rc = sync_all(E))) return rc;

@@ -73,7 +73,7 @@ struct Shape {
     int band = 0;
     int rhs = 1;
     int unit = 0;
-    int fuse = 1;  // an engine of one case takes two-step passes over the far field
+    int fuse = 1;  // two-step passes over the far field
 };
 
 // Drives the kernels of a set of cases on one device.  Cases advance in
@@ -115,8 +115,14 @@ struct Engine {
     hipGraphExec_t gexec = nullptr;
     int graph_len = 0;
     bool graph_valid = false;
-    // two-step passes (an engine of one case): the far kernel's grid
-    int far_strips = 0, far_segs = 0;
+    // two-step passes: workgroup tables of the far pass and of the window,
+    // ring and window + ring launches over every member (see k_far2), and
+    // the layout they were built for
+    int4* dfar = nullptr;
+    int2* dwin = nullptr;
+    size_t far_cap = 0, win_cap = 0;
+    int nfar = 0, nwin = 0, nring = 0, nwr = 0;  // dwin holds the window, ring and window + ring tables in turn
+    std::vector<int> pass_key;
     // profiling: EPER events per sampled step or pass (see launch_pair), the
     // kind of each sample (0 single step, 1 pass, 2 unused)
     static constexpr int EPER = 7;

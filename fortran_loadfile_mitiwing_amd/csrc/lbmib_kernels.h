@@ -187,18 +187,27 @@ __host__ __device__ inline int make_psel(int src, int dst) { return src | (dst <
 template <typename T, int NPT>
 __global__ void k_lattice(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                           const int2* __restrict__ tiles, int ntiles, int off, int psel);
-// two-step far-field pass and the window's single steps (an engine of one case)
+// two-step far-field passes and the windows' single steps.  Each launch
+// covers every case of the engine through a table of workgroups: far tiles
+// {case, a | b << 16, j0, j1} (output columns [a, b), rows j0..j1), and
+// {case, block} for the window, ring and window + ring launches (in the
+// last, blocks below the case's window block count step the window, the
+// rest the ring).
 constexpr int FAR_LANES = 256;             // phase-1 columns per strip
 constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip
 template <typename T>
-__global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+__global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int4* __restrict__ tiles,
+                       int off, int psel);
 template <typename T>
-__global__ void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+__global__ void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int2* __restrict__ tab,
+                         int off, int psel);
 template <typename T>
-__global__ void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel);
+__global__ void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int2* __restrict__ tab,
+                       int off, int psel);
 template <typename T>
-__global__ void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off, int psel,
-                              int ring_psel, int wblocks);
+__global__ void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                              const int2* __restrict__ tab, int off, int psel, int ring_psel);
+__host__ __device__ inline int window_blocks(int gw, int gh) { return (int)(((long long)gw * gh + 255) / 256); }
 size_t far2_smem_bytes(bool f64);
 __host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
 // workgroups of k_ring: one ring node per thread

@@ -418,47 +418,59 @@ __device__ __forceinline__ void ring_thread1(const DevCase& D, const CaseClock& 
     if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
 }
 
+// The case of a table entry, made wave-uniform so that every field of the
+// case stays in scalar registers.
+__device__ __forceinline__ int table_case(int c) { return __builtin_amdgcn_readfirstlane(c); }
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                int off, int psel)
+                                                const int2* __restrict__ tab, int off, int psel)
 {
-    const CaseClock kc = K[0];
-    window_thread<T>(C[0], kc, off, psel, (long long)blockIdx.x * 256 + threadIdx.x);
+    const int2 e = tab[blockIdx.x];
+    const int c = table_case(e.x);
+    const CaseClock kc = K[c];
+    window_thread<T>(C[c], kc, off, psel, (long long)e.y * 256 + threadIdx.x);
 }
-template __global__ void k_window<double>(const DevCase*, const CaseClock*, int, int);
-template __global__ void k_window<float>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_window<double>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_window<float>(const DevCase*, const CaseClock*, const int2*, int, int);
 
 // Step n of the outer ring alone, src -> dst (the middle plane of a two-step
 // pass, which then reads the ring's step n there instead of recomputing it
 // in phase 1 one row at a time); one node per thread.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
-                                             int psel)
+__global__ __launch_bounds__(256) void k_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                             const int2* __restrict__ tab, int off, int psel)
 {
-    const CaseClock kc = K[0];
-    ring_thread1<T>(C[0], kc, off, psel, (int)(blockIdx.x * 256 + threadIdx.x));
+    const int2 e = tab[blockIdx.x];
+    const int c = table_case(e.x);
+    const CaseClock kc = K[c];
+    ring_thread1<T>(C[c], kc, off, psel, e.y * 256 + (int)threadIdx.x);
 }
-template __global__ void k_ring<double>(const DevCase*, const CaseClock*, int, int);
-template __global__ void k_ring<float>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_ring<double>(const DevCase*, const CaseClock*, const int2*, int, int);
+template __global__ void k_ring<float>(const DevCase*, const CaseClock*, const int2*, int, int);
 
 // The tail of a two-step pass: step n+1 (offset off) of the window (planes
-// psel, workgroups 0..wblocks-1) and of the outer ring (planes ring_psel:
-// the middle plane to the pass's output plane, the rest), in one launch.
+// psel, a case's workgroups 0..window_blocks-1) and of the outer ring (planes
+// ring_psel: the middle plane to the pass's output plane, the case's other
+// workgroups), in one launch.
 // They touch disjoint nodes: the window step reads the middle plane inside W
 // and writes W; the ring step reads the ring's neighbourhood of the middle
 // plane (the far pass put step n of that neighbourhood there) and writes the
 // ring.
 template <typename T>
 __global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                     int off, int psel, int ring_psel, int wblocks)
+                                                     const int2* __restrict__ tab, int off, int psel, int ring_psel)
 {
-    const CaseClock kc = K[0];
-    const int b = (int)blockIdx.x;
-    if (b < wblocks) window_thread<T>(C[0], kc, off, psel, (long long)b * 256 + threadIdx.x);
-    else ring_thread1<T>(C[0], kc, off, ring_psel, (b - wblocks) * 256 + (int)threadIdx.x);
+    const int2 e = tab[blockIdx.x];
+    const int c = table_case(e.x);
+    const DevCase& D = C[c];
+    const CaseClock kc = K[c];
+    const int wb = window_blocks(D.gw, D.gh), b = e.y;
+    if (b < wb) window_thread<T>(D, kc, off, psel, (long long)b * 256 + threadIdx.x);
+    else ring_thread1<T>(D, kc, off, ring_psel, (b - wb) * 256 + (int)threadIdx.x);
 }
-template __global__ void k_window_ring<double>(const DevCase*, const CaseClock*, int, int, int, int);
-template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, int, int, int, int);
+template __global__ void k_window_ring<double>(const DevCase*, const CaseClock*, const int2*, int, int, int);
+template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, const int2*, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Two steps of the far field in one pass
@@ -511,22 +523,22 @@ constexpr int FAR_NELEM = FAR_O78 + FAR_R78 * 2 * FAR_LANES;
 size_t far2_smem_bytes(bool f64) { return (size_t)FAR_NELEM * (f64 ? 8 : 4); }
 
 template <typename T>
-__device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off,
-                                          int psel)
+__device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                          const int4* __restrict__ tiles, int off, int psel)
 {
     extern __shared__ __align__(16) unsigned char far_smem[];
     T* lds = reinterpret_cast<T*>(far_smem);
-    const DevCase& D = C[0];
-    const CaseClock kc = K[0];
+    const int4 tl = tiles[blockIdx.x];
+    const int cs = table_case(tl.x);
+    const DevCase& D = C[cs];
+    const CaseClock kc = K[cs];
     const int n = kc.n0 + off;  // the pair's first step
     if (n + 1 > kc.n_end) return;
     const int src = psel & 3, dst = (psel >> 2) & 3, mid = (psel >> 4) & 3;
     const int nx = D.nx, ny = D.ny;
     const long long N = D.N;
-    const int a = (int)(((long long)blockIdx.x * nx) / gridDim.x);
-    const int b = (int)(((long long)(blockIdx.x + 1) * nx) / gridDim.x);
-    const int j0 = (int)(((long long)blockIdx.y * ny) / gridDim.y);
-    const int j1 = (int)(((long long)(blockIdx.y + 1) * ny) / gridDim.y) - 1;
+    const int a = tl.y & 0xffff, b = tl.y >> 16;
+    const int j0 = tl.z, j1 = tl.w;
     const int L = (int)threadIdx.x;
     const int c0 = a - 1;
     const int i = c0 + L;
@@ -642,12 +654,12 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
 
 template <typename T>
 __global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                    int off, int psel)
+                                                    const int4* __restrict__ tiles, int off, int psel)
 {
-    far2_body<T>(C, K, off, psel);
+    far2_body<T>(C, K, tiles, off, psel);
 }
-template __global__ void k_far2<double>(const DevCase*, const CaseClock*, int, int);
-template __global__ void k_far2<float>(const DevCase*, const CaseClock*, int, int);
+template __global__ void k_far2<double>(const DevCase*, const CaseClock*, const int4*, int, int);
+template __global__ void k_far2<float>(const DevCase*, const CaseClock*, const int4*, int, int);
 
 int lattice_tiles(int nx, int ny, int npt)
 {

@@ -35,9 +35,12 @@ import numpy as np
 
 from . import lbmib
 
-# device bytes per lattice node: populations (two planes of 9) + rho/u/v
+# device bytes per lattice node: populations (two state planes of 9 and the
+# middle plane of the two-step passes) + rho/u/v
 # planes of the last two recorded steps
-BYTES_PER_NODE = {lbmib.PREC_FP64: 2 * 9 * 8 + 2 * 3 * 8, lbmib.PREC_FP32DEV: 2 * 9 * 4 + 2 * 3 * 8}
+BYTES_PER_NODE = {lbmib.PREC_FP64: 3 * 9 * 8 + 2 * 3 * 8, lbmib.PREC_FP32DEV: 3 * 9 * 4 + 2 * 3 * 8}
+# the middle plane of the two-step passes, allocated on a case's first pass
+MID_PLANE_BYTES = {lbmib.PREC_FP64: 9 * 8, lbmib.PREC_FP32DEV: 9 * 4}
 
 
 @dataclasses.dataclass
@@ -85,7 +88,9 @@ class GPUBackend:
         return lbmib.LBMIB(case, precision=self.precision, device=self.device)
 
     def device_bytes(self, sim) -> int:
-        return sim.device_bytes()
+        # with the middle plane its first window will allocate
+        p = sim.case.params
+        return sim.device_bytes() + MID_PLANE_BYTES[self.precision] * p.nx * p.ny
 
     def add(self, sim):
         self.batch.add(sim)
