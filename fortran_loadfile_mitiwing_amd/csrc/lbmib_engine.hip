@@ -259,13 +259,18 @@ int build_pass_tables(Engine* E)
     for (size_t c = 0; c < E->members.size(); ++c) {
         const DevCase& D = E->members[c]->D;
         const int nx = D.nx, ny = D.ny;
-        const int strips = (nx + FAR_OUT - 1) / FAR_OUT;
+        // FP64: strips of at most FAR_OUT columns; FP32: of at most FAR_OUT2
+        // columns starting at even columns (two columns per lane)
+        const bool f64 = D.prec == LBMIB_PREC_FP64;
+        const int units = f64 ? nx : (nx + 1) / 2, per = f64 ? FAR_OUT : FAR_OUT2 / 2, scale = f64 ? 1 : 2;
+        const int strips = (units + per - 1) / per;
         int segs = std::max(1, ny / 5);
         if (es) segs = std::max(1, std::min(ny / 4, atoi(es)));
         for (int sg = 0; sg < segs; ++sg) {
             const int j0 = (int)((long long)sg * ny / segs), j1 = (int)((long long)(sg + 1) * ny / segs) - 1;
             for (int st = 0; st < strips; ++st) {
-                const int a = (int)((long long)st * nx / strips), b = (int)((long long)(st + 1) * nx / strips);
+                const int a = scale * (int)((long long)st * units / strips);
+                const int b = std::min(nx, scale * (int)((long long)(st + 1) * units / strips));
                 far.push_back(make_int4((int)c, a | (b << 16), j0, j1));
             }
         }
@@ -358,6 +363,8 @@ int engine_prepare(Engine* E)
         if ((rc = build_pass_tables(E))) return rc;
         HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)far2_smem_bytes(true)));
+        HIP_TRY(hipFuncSetAttribute((const void*)k_far2<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)far2_smem_bytes(false)));
     }
     if ((rc = upload_cases(E))) return rc;
     if (!E->tiles_valid) {

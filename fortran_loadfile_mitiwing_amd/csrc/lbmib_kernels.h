@@ -194,7 +194,8 @@ __global__ void k_lattice(const DevCase* __restrict__ C, const CaseClock* __rest
 // last, blocks below the case's window block count step the window, the
 // rest the ring).
 constexpr int FAR_LANES = 256;             // phase-1 columns per strip
-constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip
+constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip (FP64: one column per lane)
+constexpr int FAR_OUT2 = 2 * FAR_LANES - 4;  // FP32: two columns per lane, strips start at even columns
 template <typename T>
 __global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int4* __restrict__ tiles,
                        int off, int psel);

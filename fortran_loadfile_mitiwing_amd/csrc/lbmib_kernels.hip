@@ -229,29 +229,49 @@ __device__ __forceinline__ void ring_thread(const DevCase& D, int src, int dst, 
     }
 }
 
+// The moments a node's update leaves behind, in the form its arithmetic
+// has them: rho, u, v in FP64; in the FP32 deviation form the density
+// deviation and the momentum, from which rho, u, v follow only when the step
+// is recorded.
+struct NodeMom64 { double rho, u, v; };
+struct NodeMom32 { float drho, jx, jy; };
+template <typename T> struct NodeMomOf { typedef NodeMom64 type; };
+template <> struct NodeMomOf<float> { typedef NodeMom32 type; };
+
 // Fused streaming + physical_variables + colission (solution.f90:428-643) of
 // one interior node without forcing, from its pulled populations p (the
-// stored type: FP64 populations, or FP32 deviations g = f - W) into out;
-// rho/u/v of the step as well.  gforce is zero away from the IB nodes, so
-// the force moments vanish exactly.
-__device__ __forceinline__ void interior_step(const DevCase& D, const double* p, double* out, double& rho,
-                                              double& u, double& v)
+// stored type: FP64 populations, or FP32 deviations g = f - W) into out.
+// gforce is zero away from the IB nodes, so the force moments vanish
+// exactly.
+__device__ __forceinline__ void interior_step(const DevCase& D, const double* p, double* out, NodeMom64& m)
 {
 #pragma unroll
     for (int k = 0; k < 9; ++k) out[k] = p[k];
-    moments(out, rho, u, v);
-    collide_node(out, rho, u, v, D.R, false, 0.0, 0.0);
+    moments(out, m.rho, m.u, m.v);
+    collide_node(out, m.rho, m.u, m.v, D.R, false, 0.0, 0.0);
 }
-__device__ __forceinline__ void interior_step(const DevCase& D, const float* p, float* out, double& rho, double& u,
-                                              double& v)
+__device__ __forceinline__ void interior_step(const DevCase& D, const float* p, float* out, NodeMom32& m)
 {
 #pragma unroll
     for (int k = 0; k < 9; ++k) out[k] = p[k];
-    float drho, jx, jy;
-    collide_dev_f32(out, D.K, drho, jx, jy);
-    rho = D.rho0 + (double)drho;
-    u = (double)jx / rho;
-    v = (double)jy / rho;
+    collide_dev_f32(out, D.K, m.drho, m.jx, m.jy);
+}
+
+// rho/u/v of a recorded step into the macroscopic planes at node c.
+template <typename P>
+__device__ __forceinline__ void record_macro(const DevCase&, const NodeMom64& m, P* mac, long long N, long long c)
+{
+    mac[c] = m.rho;
+    mac[N + c] = m.u;
+    mac[2 * N + c] = m.v;
+}
+template <typename P>
+__device__ __forceinline__ void record_macro(const DevCase& D, const NodeMom32& m, P* mac, long long N, long long c)
+{
+    const double rho = D.rho0 + (double)m.drho;
+    mac[c] = rho;
+    mac[N + c] = (double)m.jx / rho;
+    mac[2 * N + c] = (double)m.jy / rho;
 }
 
 // The nine stored populations streamed into node c = j*nx + i of plane src.
@@ -296,13 +316,9 @@ __device__ __forceinline__ void lattice_node(const DevCase& D, int g, int src, i
     const long long c = (long long)j * nx + i;
     T p[9], out[9];
     pull_raw(pops<T>(D, src), N, nx, c, p);
-    double rho, u, v;
-    interior_step(D, p, out, rho, u, v);
-    if (macro) {
-        macro[c] = rho;
-        macro[N + c] = u;
-        macro[2 * N + c] = v;
-    }
+    typename NodeMomOf<T>::type m;
+    interior_step(D, p, out, m);
+    if (macro) record_macro(D, m, macro, N, c);
     GAS T* __restrict__ o = pops<T>(D, dst);
 #pragma unroll
     for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], o + k * N + c);
@@ -570,11 +586,11 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
         p[7] = lds[FAR_O78 + (s78 * 2 + 0) * FAR_LANES + L + 1];
         p[8] = lds[FAR_O78 + (s78 * 2 + 1) * FAR_LANES + L - 1];
         const long long c = (long long)q * nx + i;
-        double rho, u, v;
-        interior_step(D, p, out, rho, u, v);
+        typename NodeMomOf<T>::type m;
+        interior_step(D, p, out, m);
 #pragma unroll
         for (int k = 0; k < 9; ++k) __builtin_nontemporal_store(out[k], out2 + k * N + c);
-        if (mac2) { mac2[c] = rho; mac2[N + c] = u; mac2[2 * N + c] = v; }
+        if (mac2) record_macro(D, m, mac2, N, c);
     };
 
     // Phase 1 of row r at this lane: 0 nothing, 1 an interior node, 2 a ring
@@ -616,12 +632,12 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
             const long long c = (long long)r * nx + i;
             T st[9];
-            double rho = 0.0, u = 0.0, v = 0.0;
+            typename NodeMomOf<T>::type m = {};
             if (kr == 2) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) st[k] = p[k];
             } else {
-                interior_step(D, p, st, rho, u, v);
+                interior_step(D, p, st, m);
             }
             const int s13 = r % FAR_R13, s56 = r & (FAR_R56 - 1), s78 = r & (FAR_R78 - 1);
             lds[(s13 * 2 + 0) * FAR_LANES + L] = st[1];
@@ -639,7 +655,7 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
 #pragma unroll
                 for (int k = 0; k < 9; ++k) outm[k * N + c] = st[k];  // step n for the window and the ring
             }
-            if (own && mac1) { mac1[c] = rho; mac1[N + c] = u; mac1[2 * N + c] = v; }
+            if (own && mac1) record_macro(D, m, mac1, N, c);
         }
         lds_barrier();  // the next row's pulls stay in flight across it
         // phase 2: step n+1 at row r-1 (ring nodes excluded)
@@ -652,11 +668,241 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
     }
 }
 
+// The FP32 far pass with two adjacent columns per lane.  A 4-byte
+// population per lane and memory instruction moves half the bytes of an FP64
+// one per instruction, and an x-shifted pull of 256 bytes crosses one cache
+// line more in five than one of 512 bytes in nine: nine FP32 planes pulled
+// and nine written stream at 3.92 TB/s one node per lane and at 4.73 TB/s two
+// nodes per lane (FP64 one per lane: 4.85 TB/s; a probe of 8192x4096 nodes).
+// Lane L stands at columns a-2+2L and a-1+2L (a even, so the pair is 8-byte
+// aligned when nx is even; the x-shifted pulls are 4-byte aligned pairs); a
+// strip holds at most FAR_OUT2 output columns.  Rows, phases and every
+// node's arithmetic are those of far2_body.  Of the populations phase 2 pulls
+// across columns, those a lane's other node supplies stay in its registers
+// (kept for the rows they are pulled from); only those of the lane's edge
+// nodes go to the LDS rings, one float per lane and ring row as in far2_body
+// (18 KB).  A pair whose nodes differ in kind (the strip's edges, the outer
+// ring, the window's edge) loads and stores node by node.
+__device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
+                                           const int4* __restrict__ tiles, int off, int psel)
+{
+    typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+    constexpr int LW = FAR_LANES;  // floats per ring row
+    extern __shared__ __align__(16) unsigned char far_smem[];
+    float* lds = reinterpret_cast<float*>(far_smem);
+    const int4 tl = tiles[blockIdx.x];
+    const int cs = table_case(tl.x);
+    const DevCase& D = C[cs];
+    const CaseClock kc = K[cs];
+    const int n = kc.n0 + off;  // the pair's first step
+    if (n + 1 > kc.n_end) return;
+    const int src = psel & 3, dst = (psel >> 2) & 3, mid = (psel >> 4) & 3;
+    const int nx = D.nx, ny = D.ny;
+    const long long N = D.N;
+    const int a = tl.y & 0xffff, b = tl.y >> 16;
+    const int j0 = tl.z, j1 = tl.w;
+    const int L = (int)threadIdx.x;
+    const int ci = a - 2 + 2 * L;  // the lane's first column
+    const int wi0 = D.gi0, wi1 = D.gi0 + D.gw - 1, wj0 = D.gj0, wj1 = D.gj0 + D.gh - 1;
+    bool col_in[2], own_col[2], ring_nb[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const int i = ci + x;
+        col_in[x] = i >= 0 && i < nx && i >= a - 1 && i <= b;
+        own_col[x] = i >= a && i < b && i >= 1 && i <= nx - 2;
+        ring_nb[x] = (i >= 1 && i <= 3) || (i >= nx - 4 && i <= nx - 2);
+    }
+    GAS double* mac1 = n >= kc.flag_from ? (GAS double*)D.macro[n & 1] : nullptr;
+    GAS double* mac2 = n + 1 >= kc.flag_from ? (GAS double*)D.macro[(n + 1) & 1] : nullptr;
+    GAS float* __restrict__ out2 = pops<float>(D, dst);
+    GAS float* __restrict__ outm = pops<float>(D, mid);
+    const GAS float* __restrict__ srcp = pops<float>(D, src);
+    // each node's own k = 0, 2, 4 of step n (as in far2_body), and what one
+    // node of the lane pulls from the other: node 1's k = 1 and 5 come from
+    // node 0 (rows q and q-1: p1, p5a/p5b), node 0's k = 3 and 6 from node 1
+    // (p3, p6a/p6b); k = 7 and 8 come from the row phase 1 just formed
+    float h0[2] = {0, 0}, h2a[2] = {0, 0}, h2b[2] = {0, 0}, c0v[2] = {0, 0}, c2v[2] = {0, 0}, c4v[2] = {0, 0};
+    float p1 = 0, p3 = 0, p5a = 0, p5b = 0, p6a = 0, p6b = 0, c1 = 0, c3 = 0, c5 = 0, c6 = 0, c7 = 0, c8 = 0;
+
+    auto phase2 = [&](int q) {
+        const int s13 = q % FAR_R13, s56 = (q - 1) & (FAR_R56 - 1), s78 = (q + 1) & (FAR_R78 - 1);
+        const float* r1 = lds + (s13 * 2 + 0) * LW;
+        const float* r3 = lds + (s13 * 2 + 1) * LW;
+        const float* r5 = lds + FAR_O56 + (s56 * 2 + 0) * LW;
+        const float* r6 = lds + FAR_O56 + (s56 * 2 + 1) * LW;
+        const float* r7 = lds + FAR_O78 + (s78 * 2 + 0) * LW;
+        const float* r8 = lds + FAR_O78 + (s78 * 2 + 1) * LW;
+        float out[2][9];
+        NodeMom32 m[2];
+        {
+            float p[9];  // node 0 at column ci
+            p[0] = h0[0];
+            p[1] = r1[L - 1];
+            p[2] = h2b[0];
+            p[3] = p3;
+            p[4] = c4v[0];
+            p[5] = r5[L - 1];
+            p[6] = p6b;
+            p[7] = c7;
+            p[8] = r8[L - 1];
+            interior_step(D, p, out[0], m[0]);
+        }
+        {
+            float p[9];  // node 1 at column ci + 1
+            p[0] = h0[1];
+            p[1] = p1;
+            p[2] = h2b[1];
+            p[3] = r3[L + 1];
+            p[4] = c4v[1];
+            p[5] = p5b;
+            p[6] = r6[L + 1];
+            p[7] = r7[L + 1];
+            p[8] = c8;
+            interior_step(D, p, out[1], m[1]);
+        }
+        const long long c = (long long)q * nx + ci;
+        const bool in_wq = q >= wj0 && q <= wj1;
+        const bool ok0 = own_col[0] && !(in_wq && ci >= wi0 && ci <= wi1);
+        const bool ok1 = own_col[1] && !(in_wq && ci + 1 >= wi0 && ci + 1 <= wi1);
+        if (ok0 && ok1) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                f2u v = {out[0][k], out[1][k]};
+                __builtin_nontemporal_store(v, (GAS f2u*)(out2 + k * N + c));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (ok0) __builtin_nontemporal_store(out[0][k], out2 + k * N + c);
+                if (ok1) __builtin_nontemporal_store(out[1][k], out2 + k * N + c + 1);
+            }
+        }
+        if (mac2) {
+            if (ok0) record_macro(D, m[0], mac2, N, c);
+            if (ok1) record_macro(D, m[1], mac2, N, c + 1);
+        }
+    };
+
+    auto kind1 = [&](int x, int r) -> int {
+        const int i = ci + x;
+        if (r < 0 || r >= ny || !col_in[x]) return 0;
+        const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
+        const bool w_edge = in_w && (i == wi0 || i == wi1 || r == wj0 || r == wj1);
+        if (in_w && !w_edge) return 0;
+        return (i == 0 || i == nx - 1 || r == 0 || r == ny - 1) ? 2 : 1;
+    };
+    float pre[2][9];
+    auto load1 = [&](int k0, int k1, int r) {
+        const long long c = (long long)r * nx + ci;
+        if (k0 == 1 && k1 == 1) {
+            const long long off[9] = {0, -1, -(long long)nx, 1, nx, -(long long)nx - 1, -(long long)nx + 1,
+                                      (long long)nx + 1, (long long)nx - 1};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const f2u v = *(const GAS f2u*)(srcp + k * N + c + off[k]);
+                pre[0][k] = v.x;
+                pre[1][k] = v.y;
+            }
+        } else {
+            if (k0 == 1) pull_raw(srcp, N, nx, c, pre[0]);
+            else if (k0 == 2) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) pre[0][k] = outm[k * N + c];
+            }
+            if (k1 == 1) pull_raw(srcp, N, nx, c + 1, pre[1]);
+            else if (k1 == 2) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) pre[1][k] = outm[k * N + c + 1];
+            }
+        }
+    };
+    const int rend = min(j1 + 1, ny - 1);
+    int kp0 = kind1(0, j0 - 1), kp1 = kind1(1, j0 - 1);
+    load1(kp0, kp1, j0 - 1);
+    for (int r = j0 - 1; r <= rend; ++r) {
+        const int kr[2] = {kp0, kp1};
+        float p[2][9];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) p[x][k] = pre[x][k];
+        kp0 = r + 1 <= rend ? kind1(0, r + 1) : 0;
+        kp1 = r + 1 <= rend ? kind1(1, r + 1) : 0;
+        load1(kp0, kp1, r + 1);
+        // phase 1: step n at (ci, r) and (ci + 1, r)
+        const int s13 = r % FAR_R13, s56 = r & (FAR_R56 - 1), s78 = r & (FAR_R78 - 1);
+        float st[2][9];
+        NodeMom32 m[2] = {};
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            if (kr[x] == 2) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) st[x][k] = p[x][k];
+            } else if (kr[x] == 1) {
+                interior_step(D, p[x], st[x], m[x]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) st[x][k] = 0.0f;
+            }
+        }
+        if (kr[0] != 0 || kr[1] != 0) {
+            // the lane's edge nodes to the neighbouring lanes: node 1's k = 1,
+            // 5, 8 (pulled by the next lane's node 0), node 0's k = 3, 6, 7
+            // (pulled by the previous lane's node 1)
+            lds[(s13 * 2 + 0) * LW + L] = st[1][1];
+            lds[(s13 * 2 + 1) * LW + L] = st[0][3];
+            lds[FAR_O56 + (s56 * 2 + 0) * LW + L] = st[1][5];
+            lds[FAR_O56 + (s56 * 2 + 1) * LW + L] = st[0][6];
+            lds[FAR_O78 + (s78 * 2 + 0) * LW + L] = st[0][7];
+            lds[FAR_O78 + (s78 * 2 + 1) * LW + L] = st[1][8];
+            c1 = st[0][1];
+            c3 = st[1][3];
+            c5 = st[0][5];
+            c6 = st[1][6];
+            c7 = st[1][7];
+            c8 = st[0][8];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                c0v[x] = st[x][0];
+                c2v[x] = st[x][2];
+                c4v[x] = st[x][4];
+                const int i = ci + x;
+                const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
+                const bool own = own_col[x] && r >= j0 && r <= j1 && !in_w && kr[x] == 1;
+                const long long c = (long long)r * nx + i;
+                if (own && (ring_nb[x] || r <= 2 || r >= ny - 3 ||
+                            (i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1))) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) outm[k * N + c] = st[x][k];  // step n for the window and the ring
+                }
+                if (own && mac1) record_macro(D, m[x], mac1, N, c);
+            }
+        }
+        lds_barrier();  // the next row's pulls stay in flight across it
+        // phase 2: step n+1 at row r-1 (ring nodes excluded)
+        const int q = r - 1;
+        if ((own_col[0] || own_col[1]) && q >= max(j0, 1) && q <= min(j1, ny - 2)) phase2(q);
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            h2b[x] = h2a[x];
+            h2a[x] = c2v[x];
+            h0[x] = c0v[x];
+        }
+        p1 = c1;
+        p3 = c3;
+        p5b = p5a;
+        p5a = c5;
+        p6b = p6a;
+        p6a = c6;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                                                     const int4* __restrict__ tiles, int off, int psel)
 {
-    far2_body<T>(C, K, tiles, off, psel);
+    if constexpr (sizeof(T) == 4) far2_pairs(C, K, tiles, off, psel);
+    else far2_body<T>(C, K, tiles, off, psel);
 }
 template __global__ void k_far2<double>(const DevCase*, const CaseClock*, const int4*, int, int);
 template __global__ void k_far2<float>(const DevCase*, const CaseClock*, const int4*, int, int);
