@@ -22,7 +22,11 @@
  *   dense vs sparse assembly (bit-identical), symmetric-airfoil zero lift,
  *   mirrored-plunge lift, the force-row layout and erranalysis;
  *   boundary_condition1's characteristic update against the textbook LODI
- *   equations, its NEE nodes against the second-order equilibrium and the
+ *   equations (within 1% of the update), and to round-off (1e-13 of the
+ *   update) against the same equations with the reference's two readings
+ *   written in -- the velocity step divided by the already updated density,
+ *   1/(2 c^2) as the literal 1.5 -- each of which the bar tells from its
+ *   alternative; its NEE nodes against the second-order equilibrium and the
  *   per-moment (1 - s) scaling; the stale ring f0 against the closed-form
  *   initial equilibrium D2Q9feq(1, U, 0) with REAL(4) weights, bit for bit
  *   after stepping; every wall node (corners included) and the edge columns
@@ -41,11 +45,10 @@
  *     L D L^T of the same SPD system.  At cond(D D^T) ~ 1e8 (128 markers) to
  *     2.8e9 (1024 markers) two valid LU codes already differ by about as much
  *     as L D L^T and the textbook LU do, so this order cannot be pinned.
- *   - the characteristic velocity update divided by the already updated
- *     density (boundary_condition.f90:413-569) is held to the LODI form only
- *     within 1% of the update; only the reference itself could fix it bit
- *     for bit, and it does not build here (it needs mpi, cublas and
- *     cusolverDn modules).
+ *   The two readings of the characteristic update above come from the
+ *   reference's statements (boundary_condition.f90:413-569); the reference
+ *   itself does not build here (it needs mpi, cublas and cusolverDn
+ *   modules), so they rest on reading it.
  */
 #ifndef LBMIB_ORACLE_H
 #define LBMIB_ORACLE_H

@@ -7,8 +7,11 @@
   L5 = (u+c)(p_x + rho c u_x), L3 = u v_x from second-order one-sided
   differences, the amplitudes of waves entering the domain set to zero, and
   rho -= (L1+L5)/(2c^2), u -= (L5-L1)/(2 rho c), v -= L3.  The reference
-  divides the velocity update by the already updated density; the check
-  allows that (1% of the update) and nothing larger.
+  divides the velocity update by the already updated density and writes
+  1/(2 c^2) as 1.5; the textbook check allows that (1% of the update), and a
+  second check holds the update to round-off (1e-13 of it) against the same
+  equations with those two readings written in, and shows that the bar
+  tells each reading from its alternative.
 * The non-equilibrium extrapolation (NEE) keeps the neighbour's
   non-equilibrium moments scaled by 1 - s per moment around the edge's
   equilibrium, whose moments are those of the second-order D2Q9 equilibrium.
@@ -94,6 +97,51 @@ def test_characteristic_update_is_lodi(oracle_mod, side):
         assert np.all(np.abs(got - want) <= 1e-2 * step + 1e-15), (states, got, want)
     # a still, uniform state is left alone
     assert np.array_equal(oracle_mod.char_update(side, c, [(1.0, 0.05, 0.0)] * 3), np.array([1.0, 0.05, 0.0]))
+
+
+def lodi_as_discretised(side, c, states, rho_for_u):
+    """The same LODI update with the two readings of the reference's
+    discretisation (boundary_condition.f90:418-440, 500-521): the density step
+    uses 1/(2 c^2) as the literal 1.5 (c^2 = 1/3 exactly, while c is the
+    REAL(4) sqrt(3)/3), and the velocity step divides by the density given by
+    rho_for_u: "new" the already updated density (the reference's order of
+    statements), "old" the node's density before the update."""
+    (r0, u0, v0), (r1, u1, v1), (r2, u2, v2) = states
+    sgn = -1.0 if side == 0 else 1.0
+    d = lambda a0, a1, a2: sgn * (3.0 * a0 - 4.0 * a1 + a2) * 0.5
+    px, ux, vx = c * c * d(r0, r1, r2), d(u0, u1, u2), d(v0, v1, v2)
+    lam = {1: u0 - c, 5: u0 + c, 3: u0}
+    L = {1: lam[1] * (px - r0 * c * ux), 5: lam[5] * (px + r0 * c * ux), 3: lam[3] * vx}
+    for k in L:
+        if (side == 0 and lam[k] > 0.0) or (side == 1 and lam[k] < 0.0):
+            L[k] = 0.0
+    rn = r0 - (L[1] + L[5]) * 1.5
+    rd = rn if rho_for_u == "new" else r0
+    return np.array([rn, u0 - (L[5] - L[1]) / (2.0 * rd * c), v0 - L[3]])
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_characteristic_update_is_the_discretised_lodi(oracle_mod, side):
+    """The oracle's characteristic update equals the LODI update with the
+    reference's two readings (above) to round-off, 1e-13 of the update; the
+    other reading of the velocity step's density, and 1/(2 c^2) from the
+    REAL(4) c, each move the result by far more (checked, so the bar tells the
+    readings apart)."""
+    rng = np.random.default_rng(17 + side)
+    c = oracle_mod.cs()
+    apart_rho, apart_c = 0, 0
+    for _ in range(400):
+        base = np.array([rng.uniform(0.97, 1.03), rng.uniform(-0.1, 0.1), rng.uniform(-0.05, 0.05)])
+        states = [tuple(base + rng.normal(0.0, 1e-3, 3)) for _ in range(3)]
+        got = oracle_mod.char_update(side, c, states)
+        want = lodi_as_discretised(side, c, states, "new")
+        step = np.abs(want - np.array(states[0]))
+        assert np.all(np.abs(got - want) <= 1e-13 * step + 1e-16), (states, got, want)
+        other = lodi_as_discretised(side, c, states, "old")
+        apart_rho += bool(np.any(np.abs(other - want) > 1e-10 * step))
+        textbook = lodi(side, c, states)
+        apart_c += bool(abs(textbook[0] - want[0]) > 1e-12 * step[0])
+    assert apart_rho > 300 and apart_c > 300
 
 
 @pytest.fixture(scope="module")
