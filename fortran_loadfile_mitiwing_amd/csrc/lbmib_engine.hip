@@ -311,21 +311,26 @@ int upload_cases(Engine* E)
 }
 
 // Before a window of two-step passes every member that steps in it holds its
-// state in plane 0 (the planes named by a launch are then the same for all
-// members): a member whose state lies in plane 1 swaps its two state planes
-// and its parity.  Returns whether any member swapped.
-bool normalize_planes(Engine* E, const std::vector<int>& len_end)
+// state in the plane of the first such member (the planes named by a launch
+// are then the same for all members): a member whose state lies in the other
+// plane swaps its two state planes and its parity.  Returns that plane in
+// *plane (0 if no member steps) and whether any member swapped.
+bool normalize_planes(Engine* E, const std::vector<int>& len_end, int* plane)
 {
     bool any = false;
+    int target = -1;
     for (size_t c = 0; c < E->members.size(); ++c) {
         lbmib_handle* h = E->members[c];
-        if (len_end[c] <= h->steps_done || state_plane(h) == 0) continue;
+        if (len_end[c] <= h->steps_done) continue;
+        if (target < 0) target = state_plane(h);
+        if (state_plane(h) == target) continue;
         DevCase& D = h->D;
         std::swap(D.f[0], D.f[1]);
         std::swap(D.g[0], D.g[1]);
         D.pofs ^= 1;
         any = true;
     }
+    *plane = target < 0 ? 0 : target;
     return any;
 }
 
@@ -367,6 +372,16 @@ int engine_prepare(Engine* E)
                                     (int)far2_smem_bytes(false)));
     }
     if ((rc = upload_cases(E))) return rc;
+    if (nc > E->pin_cap) {
+        if (E->hclk_pin) (void)hipHostFree(E->hclk_pin);
+        if (E->herr_pin) (void)hipHostFree(E->herr_pin);
+        E->hclk_pin = nullptr;
+        E->herr_pin = nullptr;
+        E->pin_cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&E->hclk_pin, sizeof(CaseClock) * nc, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&E->herr_pin, sizeof(int) * nc, hipHostMallocDefault));
+        E->pin_cap = nc;
+    }
     if (!E->tiles_valid) {
         std::vector<int2> t;
         build_tiles(E, t);
@@ -669,7 +684,7 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 //
 // An engine that takes two-step passes (par given) names the planes of every
 // step itself: every member that steps starts the window with its state in
-// plane 0 (normalize_planes), the step at offset off reads plane
+// plane par (normalize_planes), the step at offset off reads plane
 // (off + par) & 1, and each pass flips par (passes only at offsets below
 // pair_limit, while every member that steps in the window still steps).  Passes start at any position; a
 // pass whose second step opens a geometry batch takes that batch's waits at
@@ -801,19 +816,36 @@ int upload_clocks(Engine* E, const std::vector<int>& len_end, const std::vector<
         k.flag_from = final_end[c] - 1;
         k.fresh = h->geo_fresh ? 1 : 0;
     }
-    HIP_TRY(hipMemcpy(E->dclock, E->hclock.data(), sizeof(CaseClock) * nc, hipMemcpyHostToDevice));
+    // asynchronous from pinned memory on s0; the other streams wait for it
+    // (every engine call ends with its streams synchronised, so the pinned
+    // copy is free again)
+    std::memcpy(E->hclk_pin, E->hclock.data(), sizeof(CaseClock) * nc);
+    HIP_TRY(hipMemcpyAsync(E->dclock, E->hclk_pin, sizeof(CaseClock) * nc, hipMemcpyHostToDevice, E->s[0]));
+    HIP_TRY(hipEventRecord(E->ev_clk, E->s[0]));
+    for (int i = 1; i < Engine::NSTREAM; ++i) HIP_TRY(hipStreamWaitEvent(E->s[i], E->ev_clk, 0));
     return 0;
 }
 
-int check_errors(Engine* E, const Dims& d)
+// At the end of a window's work: the members' error flags gathered on s0
+// after every stream's work, and copied to pinned memory, so that the
+// window's closing synchronisation also brings them to the host.
+int enqueue_error_gather(Engine* E, const Dims& d)
 {
+    for (int i = 1; i < Engine::NSTREAM; ++i) {
+        HIP_TRY(hipEventRecord(E->ev_end[i], E->s[i]));
+        HIP_TRY(hipStreamWaitEvent(E->s[0], E->ev_end[i], 0));
+    }
     hipLaunchKernelGGL(k_gather_err, dim3((d.nc + 255) / 256), dim3(256), 0, E->s[0], (const DevCase*)E->dcases, d.nc,
                        E->derr);
-    std::vector<int> err(d.nc);
-    HIP_TRY(hipMemcpyAsync(err.data(), E->derr, sizeof(int) * d.nc, hipMemcpyDeviceToHost, E->s[0]));
-    HIP_TRY(hipStreamSynchronize(E->s[0]));
+    HIP_TRY(hipMemcpyAsync(E->herr_pin, E->derr, sizeof(int) * d.nc, hipMemcpyDeviceToHost, E->s[0]));
+    return 0;
+}
+
+// The gathered flags, after the window's synchronisation.
+int check_errors(Engine* E, const Dims& d)
+{
     for (int c = 0; c < d.nc; ++c) {
-        const int v = err[c];
+        const int v = E->herr_pin[c];
         if (!v) continue;
         if (v & ERR_GEOMETRY) return fail(LBMIB_E_GEOMETRY, "case %d: an immersed-boundary node left the interior margin (4..nx-3, 4..ny-3)", c);
         if (v & ERR_CAPACITY) return fail(LBMIB_E_CAPACITY, "case %d: an immersed-boundary list overflowed", c);
@@ -907,6 +939,8 @@ int engine_create(int dev, Engine** out)
     for (int i = 0; i < Engine::BRING; ++i)
         for (hipEvent_t* ev : {&E->ev_first[i], &E->ev_next[i], &E->ev_nodes[i]}) ok = ok && mk(ev) == hipSuccess;
     ok = ok && mk(&E->ev_fork) == hipSuccess;
+    ok = ok && mk(&E->ev_clk) == hipSuccess;
+    for (int i = 0; i < Engine::NSTREAM; ++i) ok = ok && mk(&E->ev_end[i]) == hipSuccess;
     for (int i = 0; i < Engine::NSTREAM; ++i) ok = ok && mk(&E->ev_join[i]) == hipSuccess;
     if (!ok) {
         engine_destroy(E);
@@ -936,6 +970,11 @@ void engine_destroy(Engine* E)
         for (hipEvent_t e : {E->ev_first[i], E->ev_next[i], E->ev_nodes[i]})
             if (e) (void)hipEventDestroy(e);
     if (E->ev_fork) (void)hipEventDestroy(E->ev_fork);
+    if (E->ev_clk) (void)hipEventDestroy(E->ev_clk);
+    for (hipEvent_t e : E->ev_end)
+        if (e) (void)hipEventDestroy(e);
+    if (E->hclk_pin) (void)hipHostFree(E->hclk_pin);
+    if (E->herr_pin) (void)hipHostFree(E->herr_pin);
     for (hipEvent_t e : E->ev_join)
         if (e) (void)hipEventDestroy(e);
     delete E;
@@ -1021,13 +1060,13 @@ int engine_run(Engine* E, int k)
         if (graph) len = kwin / (2 * B) * (2 * B);
         for (size_t c = 0; c < nc; ++c) len_end[c] = std::min(E->members[c]->steps_done + len, final_end[c]);
         // passes only while every member that steps in the window steps
-        int pair_limit = len;
+        int pair_limit = len, plane0 = 0;
         if (fuse) {
             for (size_t c = 0; c < nc; ++c) {
                 const int mc = len_end[c] - E->members[c]->steps_done;
                 if (mc > 0) pair_limit = std::min(pair_limit, mc);
             }
-            if (normalize_planes(E, len_end) && (rc = upload_cases(E))) return rc;
+            if (normalize_planes(E, len_end, &plane0) && (rc = upload_cases(E))) return rc;
         }
         if ((rc = upload_clocks(E, len_end, final_end))) return rc;
         if (graph) {
@@ -1036,11 +1075,13 @@ int engine_run(Engine* E, int k)
                 if ((rc = capture_graph(E, d))) return rc;
             }
             for (int r = 0; r < len / E->graph_len; ++r) HIP_TRY(hipGraphLaunch(E->gexec, E->s[0]));
+            if ((rc = enqueue_error_gather(E, d))) return rc;
         } else {
             if (every > 0) E->gev_used = 0;
-            int par = 0;
+            int par = plane0;
             if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr, pair_limit))) return rc;
-            if (fuse && par)
+            if ((rc = enqueue_error_gather(E, d))) return rc;
+            if (fuse && par != plane0)
                 for (size_t c = 0; c < nc; ++c)
                     if (len_end[c] > E->members[c]->steps_done) E->members[c]->D.pofs ^= 1;
             if (every > 0) {

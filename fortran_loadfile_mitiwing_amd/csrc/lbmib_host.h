@@ -106,6 +106,12 @@ struct Engine {
     CaseClock* dclock = nullptr;
     int2* dtiles = nullptr;
     int* derr = nullptr;             // gathered error flags of the members
+    // pinned host copies of the clock table and the gathered error flags
+    // (asynchronous copies at a window's start and end)
+    CaseClock* hclk_pin = nullptr;
+    int* herr_pin = nullptr;
+    size_t pin_cap = 0;
+    hipEvent_t ev_clk = nullptr, ev_end[4] = {};
     size_t case_cap = 0, tile_cap = 0, err_cap = 0;
     int ntiles = 0;
     bool tiles_valid = false;
