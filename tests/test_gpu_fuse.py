@@ -84,6 +84,25 @@ def test_passes_start_at_any_position(lbmib_mod):
     ref.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_odd_lattice_widths(lbmib_mod, prec):
+    # odd nx: the FP32 pass's last strip has an odd width and its last pair
+    # holds a node past the lattice; strips of different widths
+    for nx, ny in ((417, 203), (1031, 517)):
+        case = cases.make_case(nx, ny, 28.0, 112, 400.0, 40, heave=0.45, pitch_deg=20.0, phase_deg=80.0,
+                               strouhal=0.3, name="odd_%dx%d" % (nx, ny))
+        print("  %s fuse on / off" % case.name, flush=True)
+        a = run(lbmib_mod, case, prec, 1, [7, 16, 9])
+        b = run(lbmib_mod, case, prec, 0, [7, 16, 9])
+        a.set_profiling(1)
+        a.evolution(2)
+        assert a.timings()["far_passes"] == 1
+        b.evolution(2)
+        same(a, b)
+        a.close()
+        b.close()
+
+
 def test_full_size_first_steps_equal_single_steps(lbmib_mod):
     case = cases.config(1, n_steps=40)  # 4096x2048, 1024 markers
     a = run(lbmib_mod, case, 0, 1, [7, 18])
