@@ -654,11 +654,14 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     // the far field, both steps
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    const dim3 fg((unsigned)E->nfar);
+    // one workgroup per far tile, or the persistent grid of the launch shape
+    int fgrid = E->nfar;
+    if (E->shape.persist > 0) fgrid = std::min(fgrid, std::max(8, E->shape.persist * E->ncu / 8 * 8));
+    const dim3 fg((unsigned)fgrid);
     const size_t fsm = far2_smem_bytes(f64);
     const int pf = X | (Y << 2) | (M << 4);  // k_far2's planes: src, dst, middle
-    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, off, pf);
-    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, off, pf);
+    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
+    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
     if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     // IB chain of step n+1
     HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[gs % R], 0));

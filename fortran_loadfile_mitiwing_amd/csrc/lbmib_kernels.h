@@ -198,7 +198,7 @@ constexpr int FAR_OUT = FAR_LANES - 2;     // output columns per strip (FP64: on
 constexpr int FAR_OUT2 = 2 * FAR_LANES - 4;  // FP32: two columns per lane, strips start at even columns
 template <typename T>
 __global__ void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int4* __restrict__ tiles,
-                       int off, int psel);
+                       int ntiles, int off, int psel);
 template <typename T>
 __global__ void k_window(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, const int2* __restrict__ tab,
                          int off, int psel);

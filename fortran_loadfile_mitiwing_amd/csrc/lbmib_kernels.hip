@@ -540,11 +540,10 @@ size_t far2_smem_bytes(bool f64) { return (size_t)FAR_NELEM * (f64 ? 8 : 4); }
 
 template <typename T>
 __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                          const int4* __restrict__ tiles, int off, int psel)
+                                          const int4 tl, int off, int psel)
 {
     extern __shared__ __align__(16) unsigned char far_smem[];
     T* lds = reinterpret_cast<T*>(far_smem);
-    const int4 tl = tiles[blockIdx.x];
     const int cs = table_case(tl.x);
     const DevCase& D = C[cs];
     const CaseClock kc = K[cs];
@@ -684,13 +683,12 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
 // (18 KB).  A pair whose nodes differ in kind (the strip's edges, the outer
 // ring, the window's edge) loads and stores node by node.
 __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                           const int4* __restrict__ tiles, int off, int psel)
+                                           const int4 tl, int off, int psel)
 {
     typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
     constexpr int LW = FAR_LANES;  // floats per ring row
     extern __shared__ __align__(16) unsigned char far_smem[];
     float* lds = reinterpret_cast<float*>(far_smem);
-    const int4 tl = tiles[blockIdx.x];
     const int cs = table_case(tl.x);
     const DevCase& D = C[cs];
     const CaseClock kc = K[cs];
@@ -897,15 +895,23 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
     }
 }
 
+// One far tile per workgroup, or (a grid smaller than the tile table: the
+// launch shape's persistent grid) tiles v, v + gridDim.x, .. in turn, with a
+// barrier between two tiles so that no wave writes the LDS rings of the next
+// tile while another still reads those of the last.
 template <typename T>
 __global__ __launch_bounds__(FAR_LANES) void k_far2(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
-                                                    const int4* __restrict__ tiles, int off, int psel)
+                                                    const int4* __restrict__ tiles, int ntiles, int off, int psel)
 {
-    if constexpr (sizeof(T) == 4) far2_pairs(C, K, tiles, off, psel);
-    else far2_body<T>(C, K, tiles, off, psel);
+    for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        if (v != (int)blockIdx.x) lds_barrier();
+        const int4 tl = tiles[v];
+        if constexpr (sizeof(T) == 4) far2_pairs(C, K, tl, off, psel);
+        else far2_body<T>(C, K, tl, off, psel);
+    }
 }
-template __global__ void k_far2<double>(const DevCase*, const CaseClock*, const int4*, int, int);
-template __global__ void k_far2<float>(const DevCase*, const CaseClock*, const int4*, int, int);
+template __global__ void k_far2<double>(const DevCase*, const CaseClock*, const int4*, int, int, int);
+template __global__ void k_far2<float>(const DevCase*, const CaseClock*, const int4*, int, int, int);
 
 int lattice_tiles(int nx, int ny, int npt)
 {

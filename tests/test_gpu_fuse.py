@@ -103,6 +103,28 @@ def test_odd_lattice_widths(lbmib_mod, prec):
         b.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_persistent_far_grid(lbmib_mod, prec):
+    # persistent=1: one workgroup per CU (256 on MI355X) loops over the far
+    # tiles (about 510 FP64 and 306 FP32 tiles at 1024x512, several thousand
+    # at 4096x2048), against one workgroup per tile
+    for case, chunks in ((cases.config(1, n_steps=60, scale_down=4), [5, 16, 11]),
+                         (cases.config(1, n_steps=20), [3, 10])):
+        if prec == 1 and case.params.nx > 1024:
+            continue
+        print("  %s persistent 1 / 0, chunks %s" % (case.name, chunks), flush=True)
+        a = lbmib_mod.LBMIB(case, precision=prec)
+        a.set_shape(persistent=1)
+        a.set_profiling(1)
+        for c in chunks:
+            a.evolution(c)
+        assert a.timings()["far_passes"] > 0
+        b = run(lbmib_mod, case, prec, 1, chunks)
+        same(a, b)
+        a.close()
+        b.close()
+
+
 def test_full_size_first_steps_equal_single_steps(lbmib_mod):
     case = cases.config(1, n_steps=40)  # 4096x2048, 1024 markers
     a = run(lbmib_mod, case, 0, 1, [7, 18])
