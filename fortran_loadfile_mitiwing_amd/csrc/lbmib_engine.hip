@@ -593,9 +593,12 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 // pass):
 //   IB(n):   after the window/lattice and the IB collide of step n-1
 //   W(n):    after the update and IB collide of step n-1 (plane X complete)
-//   F:       after the same (it reads X everywhere and writes Y and the
-//            middle plane's layer around W, whose readers of the step
-//            before are done)
+//   F:       after the window's and the ring's update of step n-1 (same
+//            stream).  F reads X outside W and in W's outer two layers and
+//            writes Y and the middle plane outside W; the IB kernels of
+//            step n-1 read and write only nodes at least 2 nodes inside W
+//            (the geometry flags an IB box closer than 3 nodes to W's
+//            edge), so F does not wait for the IB collide of step n-1
 //   IB(n+1): after W(n) (and IB(n) in its own stream)
 //   W(n+1):  after F in F's stream (its layer around W), W(n) and the IB
 //            collide of step n
@@ -652,7 +655,6 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (ev) HIP_TRY(hipEventRecord(ev[6], s3));
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
     // the far field, both steps
-    if (after_prev) HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[(gs - 1) % R], 0));
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
     // one workgroup per far tile, or the persistent grid of the launch shape
     int fgrid = E->nfar;
@@ -850,7 +852,9 @@ int check_errors(Engine* E, const Dims& d)
     for (int c = 0; c < d.nc; ++c) {
         const int v = E->herr_pin[c];
         if (!v) continue;
-        if (v & ERR_GEOMETRY) return fail(LBMIB_E_GEOMETRY, "case %d: an immersed-boundary node left the interior margin (4..nx-3, 4..ny-3)", c);
+        if (v & ERR_GEOMETRY)
+            return fail(LBMIB_E_GEOMETRY, "case %d: an immersed-boundary node left the interior margin (4..nx-3, "
+                        "4..ny-3) or came within 3 nodes of the lattice window's edge", c);
         if (v & ERR_CAPACITY) return fail(LBMIB_E_CAPACITY, "case %d: an immersed-boundary list overflowed", c);
         if (v & ERR_SOLVE) return fail(LBMIB_E_SOLVE, "case %d: immersed-boundary system not positive definite", c);
     }

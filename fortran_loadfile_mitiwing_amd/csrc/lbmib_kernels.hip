@@ -1057,9 +1057,13 @@ __global__ __launch_bounds__(256) void k_geo_markers(const DevCase* __restrict__
     int i1, i2, j1, j2;
     axis_box(X, D.ox, D.scale, D.nx, i1, i2);
     axis_box(Y, D.oy, D.scale, D.ny, j1, j2);
+    // the box (0-based nodes i1-1 .. i2-1) lies at least 3 nodes inside the
+    // lattice window, so the IB kernels (the box's nodes and their pull
+    // neighbours) never touch the window's outer two layers, which the far
+    // pass reads
     bool ok = i1 >= 4 && i2 <= D.nx - 3 && j1 >= 4 && j2 <= D.ny - 3 && i2 >= i1 && j2 >= j1 &&
-              (i2 - i1) < 6 && (j2 - j1) < 6 && i1 - 1 >= D.gi0 && i2 - 1 < D.gi0 + D.gw &&
-              j1 - 1 >= D.gj0 && j2 - 1 < D.gj0 + D.gh;
+              (i2 - i1) < 6 && (j2 - j1) < 6 && i1 - 1 >= D.gi0 + 3 && i2 - 1 < D.gi0 + D.gw - 3 &&
+              j1 - 1 >= D.gj0 + 3 && j2 - 1 < D.gj0 + D.gh - 3;
     if (!ok) {
         atomicOr(D.err, ERR_GEOMETRY);
         i1 = D.gi0 + 2; i2 = D.gi0 + 1; j1 = D.gj0 + 2; j2 = D.gj0 + 1;  // empty box
@@ -1696,8 +1700,8 @@ template __global__ void k_crit_rhs1<double>(const DevCase*, const CaseClock*, i
 template __global__ void k_crit_rhs1<float>(const DevCase*, const CaseClock*, int, int);
 
 // Forward L z = b, scaling x = D^-1 z and backward L^T y = x for both
-// right-hand sides.  Rows are dealt to the lanes of one wavefront in chunks of
-// 64: lane h keeps both right-hand sides of row c*64 + h of the active chunk
+// right-hand sides.  Rows are dealt to the lanes of a wavefront in chunks of
+// 64: lane h keeps the right-hand side(s) of row c*64 + h of the active chunk
 // c ("cur") and of row (c+1)*64 + h (forward) / (c-1)*64 + h (backward) of
 // its neighbour ("nxt"), which covers every row within bw <= 64 of the pivot.
 // Step k (q = k mod 64) takes the finished value of row k from lane q by
@@ -1709,8 +1713,11 @@ template __global__ void k_crit_rhs1<float>(const DevCase*, const CaseClock*, in
 //
 // The 2*npad steps form one timeline (forward steps, then backward steps);
 // their coefficients lie in that order in coef, 1 KB per step as per-lane
-// pairs or 256 B per step in the compact layout.  Wavefront 0
-// runs the steps and touches only LDS and registers: a global load in its
+// pairs or 256 B per step in the compact layout.  In k_crit_solve wavefronts
+// 0 and 1 run the steps of right-hand sides 0 and 1 (the two chains share the
+// coefficients and nothing else, and one wave issues one FP64 operation at a
+// time, so two waves halve the instructions each chain issues per step).  A
+// chain wave touches only LDS and registers: a global load in its
 // instruction stream would wait for the CU's vector-memory queue, which the
 // lattice kernel keeps full, and stall the whole dependency chain.  The
 // SV_NPROD producer wavefronts stream units of U steps from HBM into a
@@ -1723,9 +1730,9 @@ constexpr int SV_P = 4;      // steps of coefficients read ahead from LDS, divid
 // FMA: each update is one fused multiply-add (one rounding) instead of a
 // product and a difference; the separator form, whose operation order differs
 // from the oracle's anyway, takes it for the shorter dependency chain.
-template <bool FWD, int U, bool FMA = false>
-__device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane, int i0, double (&cur)[2],
-                                        double (&nxt)[2])
+template <bool FWD, int U, bool FMA = false, int NR = 2>
+__device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane, int i0, double (&cur)[NR],
+                                        double (&nxt)[NR])
 {
     double2 cf[SV_P];
 #pragma unroll
@@ -1735,21 +1742,22 @@ __device__ __forceinline__ void sv_unit(const double2* __restrict__ rb, int lane
         const double2 c = cf[i % SV_P];
         if (i + SV_P < U) cf[i % SV_P] = rb[(i + SV_P) * 64 + lane];
         const int q = FWD ? i0 + i : 63 - i0 - i;
-        const double z0 = readlane_f64(cur[0], q);
-        const double z1 = readlane_f64(cur[1], q);
-        if constexpr (FMA) {
-            cur[0] = __builtin_fma(-c.x, z0, cur[0]);
-            cur[1] = __builtin_fma(-c.x, z1, cur[1]);
-            nxt[0] = __builtin_fma(-c.y, z0, nxt[0]);
-            nxt[1] = __builtin_fma(-c.y, z1, nxt[1]);
-        } else {
-            cur[0] = cur[0] - c.x * z0;
-            cur[1] = cur[1] - c.x * z1;
-            nxt[0] = nxt[0] - c.y * z0;
-            nxt[1] = nxt[1] - c.y * z1;
+        double z[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) z[r] = readlane_f64(cur[r], q);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if constexpr (FMA) {
+                cur[r] = __builtin_fma(-c.x, z[r], cur[r]);
+                nxt[r] = __builtin_fma(-c.y, z[r], nxt[r]);
+            } else {
+                cur[r] = cur[r] - c.x * z[r];
+                nxt[r] = nxt[r] - c.y * z[r];
+            }
         }
         // keep this step's updates in this step
-        asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(nxt[0]), "+v"(nxt[1]));
+        if constexpr (NR == 2) asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(nxt[0]), "+v"(nxt[1]));
+        else asm volatile("" : "+v"(cur[0]), "+v"(nxt[0]));
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -1781,7 +1789,7 @@ __device__ __forceinline__ sv_d2 compact_pair(double v, int t, int np, int lane)
 // layout's bytes) and expand them into the per-lane LDS ring, so the chain's
 // instruction stream is the same for both layouts.
 template <int U, bool CMP>
-__global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCase* __restrict__ C,
+__global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve(const DevCase* __restrict__ C,
                                                                      const CaseClock* __restrict__ K, int off)
 {
     constexpr int UPC = 64 / U;              // units per lane chunk
@@ -1801,9 +1809,9 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
         zb[e] = p < n ? D.rhs[r * n + p] : 0.0;
     }
     for (int p = tid; p < np; p += blockDim.x) dv[p] = S.dinv[p];
-    if (wave > 0) {
+    if (wave > 1) {
         // ---- producer: unit u, steps u*U + w*SPW + s ----
-        const int w = wave - 1;
+        const int w = wave - 2;
         sv_d2* rg = (sv_d2*)ring;
         typedef typename std::conditional<CMP, double, sv_d2>::type Val;
         Val A[SPW], B[SPW];
@@ -1843,53 +1851,42 @@ __global__ __launch_bounds__(64 * (1 + SV_NPROD)) void k_crit_solve(const DevCas
             __syncthreads();
         }
     } else {
-        // ---- the chain ----
+        // ---- the chain of right-hand side r = wave ----
+        const int r = wave;
+        double* zr = zb + r * np;
         __syncthreads();
-        double cur[2], nxt[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            cur[r] = zb[r * np + lane];
-            nxt[r] = nch > 1 ? zb[r * np + 64 + lane] : 0.0;
-        }
+        double cur[1], nxt[1];
+        cur[0] = zr[lane];
+        nxt[0] = nch > 1 ? zr[64 + lane] : 0.0;
         // forward, then x = z * (1/d)
         for (int c = 0; c < nch; ++c) {
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = c * UPC + j;
-                sv_unit<true, U>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<true, U, false, 1>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
-                    const double di = dv[row];
-#pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        zb[r * np + row] = cur[r] * di;
-                        cur[r] = nxt[r];
-                        nxt[r] = c + 2 < nch ? zb[r * np + row + 128] : 0.0;
-                    }
+                    zr[row] = cur[0] * dv[row];
+                    cur[0] = nxt[0];
+                    nxt[0] = c + 2 < nch ? zr[row + 128] : 0.0;
                 }
                 __syncthreads();
             }
         }
         // backward; lane h starts on rows (nch-1)*64 + h and (nch-2)*64 + h,
         // which it finished itself
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            cur[r] = zb[r * np + (nch - 1) * 64 + lane];
-            nxt[r] = nch > 1 ? zb[r * np + (nch - 2) * 64 + lane] : 0.0;
-        }
+        cur[0] = zr[(nch - 1) * 64 + lane];
+        nxt[0] = nch > 1 ? zr[(nch - 2) * 64 + lane] : 0.0;
         for (int c = nch - 1; c >= 0; --c) {
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = (2 * nch - 1 - c) * UPC + j;
-                sv_unit<false, U>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<false, U, false, 1>(ring + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
-#pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        zb[r * np + row] = cur[r];
-                        cur[r] = nxt[r];
-                        nxt[r] = c >= 2 ? zb[r * np + row - 128] : 0.0;
-                    }
+                    zr[row] = cur[0];
+                    cur[0] = nxt[0];
+                    nxt[0] = c >= 2 ? zr[row - 128] : 0.0;
                 }
                 __syncthreads();
             }
@@ -1908,8 +1905,9 @@ template __global__ void k_crit_solve<64, true>(const DevCase*, const CaseClock*
 template __global__ void k_crit_solve<32, true>(const DevCase*, const CaseClock*, int);
 
 // Separator form of the solve (DevCase::dissect; factors from k_geo_factor2
-// and k_geo_schur; compact coefficients).  Wavefront 0 runs the top half's
-// chain, wavefront 1 the reversed bottom half's, each as in k_crit_solve
+// and k_geo_schur; compact coefficients).  Wavefronts 0 and 2 run the top
+// half's chains (right-hand sides 0 and 1), wavefronts 1 and 3 the reversed
+// bottom half's, each as in k_crit_solve
 // (per-lane LDS rings that the producer wavefronts fill from the compact
 // rows) but with fused multiply-adds, so the critical path is one half's
 // length at one operation per update.  Between the forward and
@@ -1920,7 +1918,7 @@ template __global__ void k_crit_solve<32, true>(const DevCase*, const CaseClock*
 // write partial sums into the free half of a coefficient ring; the chain
 // threads add them in fixed order.  The backward sweeps then give x_T and x_B.
 template <int U>
-__global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCase* __restrict__ C,
+__global__ __launch_bounds__(64 * (4 + SV_NPROD)) void k_crit_solve2(const DevCase* __restrict__ C,
                                                                       const CaseClock* __restrict__ K, int off)
 {
     constexpr int UPC = 64 / U;              // units per lane chunk
@@ -1952,8 +1950,8 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
     double* part = (double*)(ring[0] + ((F + 1) & 1) * U * 64);
     const long long cstride = (long long)(2 * np + 256) * CW;
     const int b3 = 3 * b, n1 = sep_seg1(b), L1 = sep_len1(b), n2 = sep_seg2(b), L2 = sep_len2(b);
-    if (wave >= 2) {
-        const int w = wave - 2, pt = tid - 128;
+    if (wave >= 4) {
+        const int w = wave - 4, pt = tid - 256;
         // pieces of M (row o1, terms q1*L1 ..) and N (row o2, terms q2*L2 ..)
         const int o1 = pt / n1, q1 = pt % n1, o2 = pt / n2, q2 = pt % n2;
         const bool a1 = o1 < b, a2 = o2 < 2 * b;
@@ -2050,32 +2048,25 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
             }
         }
     } else {
-        // ---- the two chains; wave h runs half h ----
-        const int h = wave;
-        double* z = zb + 2 * h * np;
+        // ---- the four chains; wave w runs half w & 1, right-hand side w >> 1 ----
+        const int h = wave & 1;
+        double* z = zb + 2 * h * np + (wave >> 1) * np;
         const double* di = dv + h * np;
         const double2* rb = ring[h];
         __syncthreads();
-        double cur[2], nxt[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            cur[r] = z[r * np + lane];
-            nxt[r] = nch > 1 ? z[r * np + 64 + lane] : 0.0;
-        }
+        double cur[1], nxt[1];
+        cur[0] = z[lane];
+        nxt[0] = nch > 1 ? z[64 + lane] : 0.0;
         for (int c = 0; c < nch; ++c) {
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = c * UPC + j;
-                sv_unit<true, U, true>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<true, U, true, 1>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
-                    const double dd = di[row];
-#pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        z[r * np + row] = cur[r] * dd;
-                        cur[r] = nxt[r];
-                        nxt[r] = c + 2 < nch ? z[r * np + row + 128] : 0.0;
-                    }
+                    z[row] = cur[0] * di[row];
+                    cur[0] = nxt[0];
+                    nxt[0] = c + 2 < nch ? z[row + 128] : 0.0;
                 }
                 __syncthreads();
             }
@@ -2099,24 +2090,18 @@ __global__ __launch_bounds__(64 * (2 + SV_NPROD)) void k_crit_solve2(const DevCa
         }
         __syncthreads();
         // backward; lane l starts on rows (nch-1)*64 + l and (nch-2)*64 + l
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            cur[r] = z[r * np + (nch - 1) * 64 + lane];
-            nxt[r] = nch > 1 ? z[r * np + (nch - 2) * 64 + lane] : 0.0;
-        }
+        cur[0] = z[(nch - 1) * 64 + lane];
+        nxt[0] = nch > 1 ? z[(nch - 2) * 64 + lane] : 0.0;
         for (int c = nch - 1; c >= 0; --c) {
 #pragma unroll
             for (int j = 0; j < UPC; ++j) {
                 const int u = (2 * nch - 1 - c) * UPC + j;
-                sv_unit<false, U, true>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
+                sv_unit<false, U, true, 1>(rb + (u & 1) * U * 64, lane, j * U, cur, nxt);
                 if (j == UPC - 1) {
                     const int row = c * 64 + lane;
-#pragma unroll
-                    for (int r = 0; r < 2; ++r) {
-                        z[r * np + row] = cur[r];
-                        cur[r] = nxt[r];
-                        nxt[r] = c >= 2 ? z[r * np + row - 128] : 0.0;
-                    }
+                    z[row] = cur[0];
+                    cur[0] = nxt[0];
+                    nxt[0] = c >= 2 ? z[row - 128] : 0.0;
                 }
                 __syncthreads();
             }
@@ -2152,12 +2137,12 @@ int solve2_unit(int np2)
         if (solve2_smem_bytes(np2) + 2 * 2 * (size_t)U * 64 * 16 <= 160 * 1024) return U;
     return 0;
 }
-int solve2_threads() { return 64 * (2 + SV_NPROD); }
+int solve2_threads() { return 64 * (4 + SV_NPROD); }
 
 // The 64-step staging unit when its two LDS buffers (128 KB) fit beside the
 // right-hand sides, the 32-step one otherwise.
 int solve_unit(int npad) { return 3 * 8 * (size_t)npad + 2 * 64 * 64 * 16 <= 160 * 1024 ? 64 : 32; }
-int solve_threads() { return 64 * (1 + SV_NPROD); }
+int solve_threads() { return 64 * (2 + SV_NPROD); }
 
 size_t factor_smem_bytes() { return sizeof(double) * ((size_t)WIN * WIN + 2 * WIN + 2 * CH * WIN + 4) + sizeof(int) * PAIRMAX; }
 size_t solve_smem_bytes(int npad) { return sizeof(double) * 3 * (size_t)npad; }
@@ -2226,10 +2211,11 @@ template __global__ void k_band<float>(const DevCase*, const CaseClock*, int, in
 // (boundary_condition.f90:1662-1675), and their normalisation
 // (solution.f90:327-330).  One workgroup per step of a batch.  The nodes'
 // terms are formed in chunks of FCH nodes into one of two LDS buffers by
-// wavefronts 1..3 while thread 0 advances the running sums node by node over
-// the other buffer, so the serial sums overlap the loads.  The nodes of a
-// wing are contiguous in the list (segment-major order).  Block (x, c) takes
-// step n0 + off + x of case c.
+// wavefront 3 while wavefronts 0, 1 and 2 advance the running sums F_x, F_y
+// and M node by node over the other buffer: each serial chain has a wave of
+// its own (a wave issues one FP64 add at a time), and the sums overlap the
+// loads.  The nodes of a wing are contiguous in the list (segment-major
+// order).  Block (x, c) takes step n0 + off + x of case c.
 constexpr int FCH = 1024;
 __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, const CaseClock* __restrict__ K, int off)
 {
@@ -2242,6 +2228,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
     __shared__ int wing[2][FCH];
     __shared__ double sab[8][3];  // body offsets sa, sb and body_extra_pos of each wing at this step
     const int tid = threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // 0, 1, 2: F_x, F_y, M; 3: forms terms
     const int na = *S.nadj;
     const double sc = D.scale;
     if (tid < D.nw) {
@@ -2285,42 +2272,42 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
             }
         }
     };
-    double F0 = 0.0, F1 = 0.0, M = 0.0;
+    double A = 0.0;  // this wave's running sum
     int curw = 0;
     double* hist = D.hist + (long long)step * 3 * D.nw;
     const double U = D.U;
     double* raw = D.raw ? D.raw + (long long)step * 3 * D.nw : nullptr;
     auto flush = [&](int w) {
-        if (tid != 0) return;
-        if (raw) { raw[3 * w] = F0; raw[3 * w + 1] = F1; raw[3 * w + 2] = M; }
-        hist[3 * w + 0] = 2.0 * F0 / (U * U * (1.0 / sc));
-        hist[3 * w + 1] = 2.0 * F1 / (U * U * (1.0 / sc));
-        hist[3 * w + 2] = -2.0 * M / (U * U * (1.0 / (sc * sc)));
+        if ((tid & 63) != 0) return;
+        if (raw) raw[3 * w + wv] = A;
+        if (wv == 2) hist[3 * w + 2] = -2.0 * A / (U * U * (1.0 / (sc * sc)));
+        else hist[3 * w + wv] = 2.0 * A / (U * U * (1.0 / sc));
     };
     const int nch = (na + FCH - 1) / FCH;
     if (nch > 0) form(0, 0, 256);
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
-        if (tid >= 64) {
-            if (ch + 1 < nch) form(ch + 1, 64, 192);
+        if (wv == 3) {
+            if (ch + 1 < nch) form(ch + 1, 192, 64);
         } else {
-            // wave 0 sums the chunk in ascending node order, every lane the
-            // same values: each run of one wing's nodes (wing ids never
-            // decrease along the list) is found with a ballot, then summed
-            // with no branch per node, the terms of PF nodes read ahead
+            // the wave sums its quantity over the chunk in ascending node
+            // order, every lane the same values: each run of one wing's nodes
+            // (wing ids never decrease along the list) is found with a
+            // ballot, then summed with no branch per node, the terms of PF
+            // nodes read ahead
             const int cnt = min(FCH, na - ch * FCH), b = ch & 1;
             int q = 0;
             while (q < cnt) {
                 const int w = __builtin_amdgcn_readfirstlane(wing[b][q]);
                 if (w != curw) {
                     flush(curw);
-                    for (int x = curw + 1; x < w; ++x) { F0 = F1 = M = 0.0; flush(x); }
+                    for (int x = curw + 1; x < w; ++x) { A = 0.0; flush(x); }
                     curw = w;
-                    F0 = F1 = M = 0.0;
+                    A = 0.0;
                 }
                 int e = cnt;
                 for (int base = q + 1; base < cnt; base += 64) {
-                    const int idx = base + tid;
+                    const int idx = base + (tid & 63);
                     const unsigned long long other = __ballot(idx < cnt && wing[b][idx] != w);
                     if (other) {
                         e = base + __builtin_ctzll(other);
@@ -2328,34 +2315,35 @@ __global__ __launch_bounds__(256) void k_forces(const DevCase* __restrict__ C, c
                     }
                 }
                 constexpr int PF = 8;
-                for (; q + PF <= e; q += PF) {
-                    double t0[PF], t1[PF], t2[PF], t3[PF];
+                if (wv == 2) {
+                    for (; q + PF <= e; q += PF) {
+                        double t2[PF], t3[PF];
 #pragma unroll
-                    for (int u = 0; u < PF; ++u) {
-                        t0[u] = terms[b][q + u][0];
-                        t1[u] = terms[b][q + u][1];
-                        t2[u] = terms[b][q + u][2];
-                        t3[u] = terms[b][q + u][3];
-                    }
+                        for (int u = 0; u < PF; ++u) {
+                            t2[u] = terms[b][q + u][2];
+                            t3[u] = terms[b][q + u][3];
+                        }
 #pragma unroll
-                    for (int u = 0; u < PF; ++u) {
-                        F0 = F0 - t0[u];
-                        F1 = F1 - t1[u];
-                        M = M - t2[u] + t3[u];
+                        for (int u = 0; u < PF; ++u) A = A - t2[u] + t3[u];
                     }
-                }
-                for (; q < e; ++q) {
-                    F0 = F0 - terms[b][q][0];
-                    F1 = F1 - terms[b][q][1];
-                    M = M - terms[b][q][2] + terms[b][q][3];
+                    for (; q < e; ++q) A = A - terms[b][q][2] + terms[b][q][3];
+                } else {
+                    for (; q + PF <= e; q += PF) {
+                        double t[PF];
+#pragma unroll
+                        for (int u = 0; u < PF; ++u) t[u] = terms[b][q + u][wv];
+#pragma unroll
+                        for (int u = 0; u < PF; ++u) A = A - t[u];
+                    }
+                    for (; q < e; ++q) A = A - terms[b][q][wv];
                 }
             }
         }
         __syncthreads();
     }
-    if (tid == 0) {
+    if (wv < 3) {
         flush(curw);
-        for (int x = curw + 1; x < D.nw; ++x) { F0 = F1 = M = 0.0; flush(x); }
+        for (int x = curw + 1; x < D.nw; ++x) { A = 0.0; flush(x); }
     }
 }
 
