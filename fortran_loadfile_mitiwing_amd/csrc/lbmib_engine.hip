@@ -242,8 +242,8 @@ bool fuse_ok(const Engine* E)
 // the rows the workgroups have in flight, not by the halo rows that short
 // segments read again, and many workgroups balance the load at the end.
 // LBMIB_FAR_SEGS overrides the segment count of every member.  The window,
-// ring and window + ring tables take window_blocks, ring_blocks1 and their
-// sum workgroups per member.
+// ring and window + ring (+ layer) tables take window_blocks, ring_blocks1
+// and window_blocks + ring_blocks1 + layer_blocks workgroups per member.
 int build_pass_tables(Engine* E)
 {
     std::vector<int> key;
@@ -274,10 +274,10 @@ int build_pass_tables(Engine* E)
                 far.push_back(make_int4((int)c, a | (b << 16), j0, j1));
             }
         }
-        const int wb = window_blocks(D.gw, D.gh), rb = ring_blocks1(nx, ny);
+        const int wb = window_blocks(D.gw, D.gh), rb = ring_blocks1(nx, ny), lb = layer_blocks(nx, ny);
         for (int b = 0; b < wb; ++b) win.push_back(make_int2((int)c, b));
         for (int b = 0; b < rb; ++b) ring.push_back(make_int2((int)c, b));
-        for (int b = 0; b < wb + rb; ++b) wr.push_back(make_int2((int)c, b));
+        for (int b = 0; b < wb + rb + lb; ++b) wr.push_back(make_int2((int)c, b));
     }
     std::vector<int2> all;
     all.insert(all.end(), win.begin(), win.end());
@@ -602,10 +602,13 @@ int launch_step(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 //   IB(n+1): after W(n) (and IB(n) in its own stream)
 //   W(n+1):  after F in F's stream (its layer around W), W(n) and the IB
 //            collide of step n
-// The outer ring takes its steps in F's stream: step n (k_ring, X -> middle)
-// before F, which reads it there, and step n+1 (middle -> Y) after F, which
-// puts the ring's neighbourhood of step n into the middle plane, in one
-// launch with W(n+1) (k_window_ring).
+// The outer ring's step n (k_ring, X -> middle) follows W(n) in W's stream,
+// beside F: it reads only the ring's neighbourhood of X and writes only the
+// ring's nodes of the middle plane, which F neither reads nor writes (F
+// leaves out the ring and the second step of the layer next to it).  Its
+// step n+1 and the layer's (middle -> Y) ride along with W(n+1)
+// (k_window_ring), after F, which puts step n of the nodes inwards of the
+// ring into the middle plane.
 // ev as in launch_step for step n, plus [5]/[6] around W(n); [0]/[1] around F.
 int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t wfirst, hipEvent_t wnext,
                 hipEvent_t wnodes, hipEvent_t* ev, int X)
@@ -635,13 +638,9 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, s1, C, K, off, pa);
     if (ev) HIP_TRY(hipEventRecord(ev[4], s1));
     HIP_TRY(hipEventRecord(E->ev_ib[gs % R], s1));
-    // the ring's step n into the middle plane: it reads only the ring's
-    // neighbourhood of plane X, which the far pass (or lattice update) before
-    // it on this stream wrote
     const int2* wtab = E->dwin;
     const int2* rtab = E->dwin + E->nwin;
     const int2* wrtab = rtab + E->nring;
-    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(E->nring), dim3(256), 0, s0, C, K, rtab, off, pa);
     // window step n on the shared CUs
     const dim3 wg(E->nwin);
     if (after_prev) {
@@ -653,6 +652,10 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (ev) HIP_TRY(hipEventRecord(ev[5], s3));
     hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s3, C, K, wtab, off, pa);
     if (ev) HIP_TRY(hipEventRecord(ev[6], s3));
+    // the ring's step n into the middle plane, from the ring's neighbourhood
+    // of plane X (written by the window + ring update, or the lattice update,
+    // of step n-1 and the far pass before it, all before ev_step of step n-1)
+    hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(E->nring), dim3(256), 0, s3, C, K, rtab, off, pa);
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
     // the far field, both steps
     if (ev) HIP_TRY(hipEventRecord(ev[0], s0));

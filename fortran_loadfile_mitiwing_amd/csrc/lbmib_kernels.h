@@ -213,6 +213,8 @@ size_t far2_smem_bytes(bool f64);
 __host__ __device__ inline int lattice_ring_blocks(int nx, int ny) { return (ny - 2 + nx + 255) / 256; }
 // workgroups of k_ring: one ring node per thread
 __host__ __device__ inline int ring_blocks1(int nx, int ny) { return (2 * (ny - 2) + 2 * nx + 255) / 256; }
+// workgroups of 256 threads over the layer next to the outer ring
+__host__ __device__ inline int layer_blocks(int nx, int ny) { return (2 * (ny - 2) + 2 * (nx - 4) + 255) / 256; }
 // first node of the interior tiles: row 1 rounded down to a multiple of 64 nodes
 __host__ __device__ inline int lattice_base(int nx) { return nx / 64 * 64; }
 int lattice_tiles(int nx, int ny, int npt);  // ring blocks + interior tiles of one case

@@ -434,6 +434,36 @@ __device__ __forceinline__ void ring_thread1(const DevCase& D, const CaseClock& 
     if (macro) { macro[c] = ro; macro[N + c] = uo; macro[2 * N + c] = vo; }
 }
 
+// Step n of one node of the layer next to the outer ring (columns 1 and
+// nx-2 at rows 1..ny-2, rows 1 and ny-2 at columns 2..nx-3), src -> dst,
+// thread t: threads 0..ny-3 the left column, the next ny-2 the right one,
+// then nx-4 the top row and nx-4 the bottom row.  The ordinary interior
+// update (the far pass leaves these nodes' second step out: their pulls
+// reach the ring's step n, which k_ring forms beside that pass).
+template <typename T>
+__device__ __forceinline__ void layer_thread(const DevCase& D, const CaseClock& kc, int off, int psel, int t)
+{
+    const int n = kc.n0 + off;
+    if (n > kc.n_end) return;
+    const int nx = D.nx, ny = D.ny;
+    int i, j;
+    if (t < 2 * (ny - 2)) {
+        const bool right = t >= ny - 2;
+        i = right ? nx - 2 : 1;
+        j = 1 + (right ? t - (ny - 2) : t);
+    } else {
+        t -= 2 * (ny - 2);
+        if (t >= 2 * (nx - 4)) return;
+        const bool bottom = t >= nx - 4;
+        i = 2 + (bottom ? t - (nx - 4) : t);
+        j = bottom ? 1 : ny - 2;
+    }
+    int src, dst;
+    step_planes(D, n, psel, src, dst);
+    double* macro = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
+    lattice_node<T>(D, (n - 1) % D.nslot, src, dst, macro, i, j);
+}
+
 // The case of a table entry, made wave-uniform so that every field of the
 // case stays in scalar registers.
 __device__ __forceinline__ int table_case(int c) { return __builtin_amdgcn_readfirstlane(c); }
@@ -466,13 +496,14 @@ template __global__ void k_ring<double>(const DevCase*, const CaseClock*, const 
 template __global__ void k_ring<float>(const DevCase*, const CaseClock*, const int2*, int, int);
 
 // The tail of a two-step pass: step n+1 (offset off) of the window (planes
-// psel, a case's workgroups 0..window_blocks-1) and of the outer ring (planes
-// ring_psel: the middle plane to the pass's output plane, the case's other
-// workgroups), in one launch.
+// psel, a case's workgroups 0..window_blocks-1), of the outer ring and of the
+// layer next to it (planes ring_psel: the middle plane to the pass's output
+// plane; the case's next ring_blocks1 workgroups, then layer_blocks), in one
+// launch.
 // They touch disjoint nodes: the window step reads the middle plane inside W
-// and writes W; the ring step reads the ring's neighbourhood of the middle
-// plane (the far pass put step n of that neighbourhood there) and writes the
-// ring.
+// and writes W; the ring and layer steps read the ring's neighbourhood of
+// the middle plane (k_ring put the ring's step n there, the far pass that of
+// the nodes inwards of it) and write the ring and the layer.
 template <typename T>
 __global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                                                      const int2* __restrict__ tab, int off, int psel, int ring_psel)
@@ -481,9 +512,10 @@ __global__ __launch_bounds__(256) void k_window_ring(const DevCase* __restrict__
     const int c = table_case(e.x);
     const DevCase& D = C[c];
     const CaseClock kc = K[c];
-    const int wb = window_blocks(D.gw, D.gh), b = e.y;
+    const int wb = window_blocks(D.gw, D.gh), rb = ring_blocks1(D.nx, D.ny), b = e.y;
     if (b < wb) window_thread<T>(D, kc, off, psel, (long long)b * 256 + threadIdx.x);
-    else ring_thread1<T>(D, kc, off, ring_psel, (b - wb) * 256 + (int)threadIdx.x);
+    else if (b < wb + rb) ring_thread1<T>(D, kc, off, ring_psel, (b - wb) * 256 + (int)threadIdx.x);
+    else layer_thread<T>(D, kc, off, ring_psel, (b - wb - rb) * 256 + (int)threadIdx.x);
 }
 template __global__ void k_window_ring<double>(const DevCase*, const CaseClock*, const int2*, int, int, int);
 template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, const int2*, int, int, int);
@@ -495,19 +527,20 @@ template __global__ void k_window_ring<float>(const DevCase*, const CaseClock*, 
 // the window W (every IB node of the run lies at least 3 nodes inside it)
 // advance one step at a time with the IB chain between the steps (k_window,
 // k_band), the outer ring takes its steps in kernels of its own (k_ring
-// before the pass, step n; k_window_ring after it, step n+1), and every
-// other node -- the far field F -- takes both steps in one pass here,
-// reading each population of step n-1 once and writing each of step n+1
-// once: 144 (FP64) or 72 (FP32) bytes per node for two steps.
+// beside the pass, step n; k_window_ring after it, step n+1, together with
+// the second step of the layer next to the ring), and every other node --
+// the far field F -- takes both steps in one pass here, reading each
+// population of step n-1 once and writing each of step n+1 once: 144 (FP64)
+// or 72 (FP32) bytes per node for two steps.
 //
 // A workgroup of FAR_LANES lanes owns a strip of output columns [a, b)
 // (b - a <= FAR_OUT) and a segment of output rows [j0, j1]; lane L stands at
 // column i = a - 1 + L, and the lanes of columns a-1..b take part.  It sweeps
 // the rows upwards.  At row r it forms step n at (i, r) -- phase 1: pull from
-// plane src, moments, collide; at a ring node read its step n from the
-// middle plane -- then, after one workgroup barrier, step n+1 at (i, r-1)
-// (phase 2) for the lanes of the strip's columns but the ring, pulling step n
-// from the neighbouring lanes.
+// plane src, moments, collide, at every node but the ring's -- then, after
+// one workgroup barrier, step n+1 at (i, r-1) (phase 2) for the lanes of the
+// strip's columns but the ring and the layer next to it (whose pulls would
+// reach the ring's step n), pulling step n from the neighbouring lanes.
 //
 // Step n of a row is kept only as far as phase 2 pulls it across lanes: the
 // populations k = 1, 3 (pulled from the same row), 5, 6 (pulled by the row
@@ -561,6 +594,9 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
     const bool col_in = i >= 0 && i < nx && i <= b;
     // the strip's columns but the outer ones
     const bool own_col = i >= a && i < b && i >= 1 && i <= nx - 2;
+    // ... and of those the ones whose step n+1 this pass forms (not the
+    // layer next to the ring, which k_window_ring advances)
+    const bool own2 = own_col && i >= 2 && i <= nx - 3;
     // columns whose step n the outer columns' update pulls
     const bool ring_nb = (i >= 1 && i <= 3) || (i >= nx - 4 && i <= nx - 2);
     double* mac1 = n >= kc.flag_from ? D.macro[n & 1] : nullptr;
@@ -592,30 +628,23 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
         if (mac2) record_macro(D, m, mac2, N, c);
     };
 
-    // Phase 1 of row r at this lane: 0 nothing, 1 an interior node, 2 a ring
-    // node, whose step n is in the middle plane.  The loads of both are
-    // issued one row ahead, so they overlap the barrier and phase 2 of the
-    // row before.
+    // Phase 1 of row r at this lane: 0 nothing, 1 an interior node (the ring
+    // is not needed: phase 2 never pulls it).  The loads are issued one row
+    // ahead, so they overlap the barrier and phase 2 of the row before.
     auto kind1 = [&](int r) -> int {
-        if (r < 0 || r >= ny || !col_in) return 0;
+        if (r < 1 || r > ny - 2 || !col_in || i < 1 || i > nx - 2) return 0;
         const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
         const bool w_edge = in_w && (i == wi0 || i == wi1 || r == wj0 || r == wj1);
-        if (in_w && !w_edge) return 0;
-        return (i == 0 || i == nx - 1 || r == 0 || r == ny - 1) ? 2 : 1;
+        return in_w && !w_edge ? 0 : 1;
     };
     const GAS T* __restrict__ srcp = pops<T>(D, src);
     T pre[9];
     auto load1 = [&](int kind, int r) {
-        const long long c = (long long)r * nx + i;
-        if (kind == 1) {
-            pull_raw(srcp, N, nx, c, pre);
-        } else if (kind == 2) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) pre[k] = outm[k * N + c];
-        }
+        if (kind == 1) pull_raw(srcp, N, nx, (long long)r * nx + i, pre);
     };
-    // phase 2 of row q needs phase 1 of rows q-1..q+1 (the top wall row is
-    // the last phase-1 row)
+    // phase 2 of row q (2 .. ny-3) needs phase 1 of rows q-1..q+1 (the rows
+    // past those are kind 0; the row loop keeps its bounds, which hold the
+    // register count down)
     const int rend = min(j1 + 1, ny - 1);
     int kpre = kind1(j0 - 1);
     load1(kpre, j0 - 1);
@@ -631,13 +660,8 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             const bool in_w = i >= wi0 && i <= wi1 && r >= wj0 && r <= wj1;
             const long long c = (long long)r * nx + i;
             T st[9];
-            typename NodeMomOf<T>::type m = {};
-            if (kr == 2) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) st[k] = p[k];
-            } else {
-                interior_step(D, p, st, m);
-            }
+            typename NodeMomOf<T>::type m;
+            interior_step(D, p, st, m);
             const int s13 = r % FAR_R13, s56 = r & (FAR_R56 - 1), s78 = r & (FAR_R78 - 1);
             lds[(s13 * 2 + 0) * FAR_LANES + L] = st[1];
             lds[(s13 * 2 + 1) * FAR_LANES + L] = st[3];
@@ -648,7 +672,7 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             c0v = st[0];
             c2v = st[2];
             c4v = st[4];
-            const bool own = own_col && r >= j0 && r <= j1 && !in_w && kr == 1;
+            const bool own = own_col && r >= j0 && r <= j1 && !in_w;
             if (own && (ring_nb || r <= 2 || r >= ny - 3 ||
                         (i >= wi0 - 1 && i <= wi1 + 1 && r >= wj0 - 1 && r <= wj1 + 1))) {
 #pragma unroll
@@ -657,9 +681,10 @@ __device__ __forceinline__ void far2_body(const DevCase* __restrict__ C, const C
             if (own && mac1) record_macro(D, m, mac1, N, c);
         }
         lds_barrier();  // the next row's pulls stay in flight across it
-        // phase 2: step n+1 at row r-1 (ring nodes excluded)
+        // phase 2: step n+1 at row r-1 (the ring and the layer next to it
+        // excluded)
         const int q = r - 1;
-        if (own_col && q >= max(j0, 1) && q <= min(j1, ny - 2) && !(i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1))
+        if (own2 && q >= max(j0, 2) && q <= min(j1, ny - 3) && !(i >= wi0 && i <= wi1 && q >= wj0 && q <= wj1))
             phase2(q);
         h2b = h2a;
         h2a = c2v;
@@ -760,8 +785,9 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
         }
         const long long c = (long long)q * nx + ci;
         const bool in_wq = q >= wj0 && q <= wj1;
-        const bool ok0 = own_col[0] && !(in_wq && ci >= wi0 && ci <= wi1);
-        const bool ok1 = own_col[1] && !(in_wq && ci + 1 >= wi0 && ci + 1 <= wi1);
+        // not the layer next to the ring (k_window_ring advances it)
+        const bool ok0 = own_col[0] && ci >= 2 && ci <= nx - 3 && !(in_wq && ci >= wi0 && ci <= wi1);
+        const bool ok1 = own_col[1] && ci + 1 >= 2 && ci + 1 <= nx - 3 && !(in_wq && ci + 1 >= wi0 && ci + 1 <= wi1);
         if (ok0 && ok1) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
@@ -781,6 +807,10 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
         }
     };
 
+    // phase 1 at node x of row r: 0 nothing, 1 an interior node, 2 a ring
+    // node, which phase 2 never pulls: it is carried through phase 1 with
+    // whatever its registers hold, no load (that form keeps the kernel at
+    // five workgroups per CU; zeroing the ring's values costs registers)
     auto kind1 = [&](int x, int r) -> int {
         const int i = ci + x;
         if (r < 0 || r >= ny || !col_in[x]) return 0;
@@ -789,7 +819,7 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
         if (in_w && !w_edge) return 0;
         return (i == 0 || i == nx - 1 || r == 0 || r == ny - 1) ? 2 : 1;
     };
-    float pre[2][9];
+    float pre[2][9] = {};
     auto load1 = [&](int k0, int k1, int r) {
         const long long c = (long long)r * nx + ci;
         if (k0 == 1 && k1 == 1) {
@@ -803,15 +833,7 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
             }
         } else {
             if (k0 == 1) pull_raw(srcp, N, nx, c, pre[0]);
-            else if (k0 == 2) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) pre[0][k] = outm[k * N + c];
-            }
             if (k1 == 1) pull_raw(srcp, N, nx, c + 1, pre[1]);
-            else if (k1 == 2) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) pre[1][k] = outm[k * N + c + 1];
-            }
         }
     };
     const int rend = min(j1 + 1, ny - 1);
@@ -877,9 +899,10 @@ __device__ __forceinline__ void far2_pairs(const DevCase* __restrict__ C, const 
             }
         }
         lds_barrier();  // the next row's pulls stay in flight across it
-        // phase 2: step n+1 at row r-1 (ring nodes excluded)
+        // phase 2: step n+1 at row r-1 (the ring and the layer next to it
+        // excluded)
         const int q = r - 1;
-        if ((own_col[0] || own_col[1]) && q >= max(j0, 1) && q <= min(j1, ny - 2)) phase2(q);
+        if ((own_col[0] || own_col[1]) && q >= max(j0, 2) && q <= min(j1, ny - 3)) phase2(q);
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
             h2b[x] = h2a[x];
