@@ -407,14 +407,17 @@ int engine_prepare(Engine* E)
     return 0;
 }
 
-// Forces of window offsets a..b (one workgroup per step and case) on s2,
-// after the IB-node collide of engine step gs_b (offset b).
-int enqueue_forces(Engine* E, const Dims& d, int a, int b, long long gs_b)
+// Forces of window offsets a..b (one workgroup per step and case) on s2
+// (or, for the window's last steps, s3), after the IB-node collide of engine
+// step gs_b (offset b).
+int enqueue_forces(Engine* E, const Dims& d, int a, int b, long long gs_b, bool tail = false)
 {
     if (b < a) return 0;
     // with per-step geometry the forces gate the next step: they go to the
-    // solves' stream, whose reserved CUs are idle once the IB nodes collide
-    hipStream_t st = E->per_step ? E->s[1] : E->s[2];
+    // solves' stream, whose reserved CUs are idle once the IB nodes collide.
+    // The window's last forces go to the window stream, which is idle by
+    // then, and not behind the geometry built ahead on s2.
+    hipStream_t st = E->per_step ? E->s[1] : tail ? E->s[3] : E->s[2];
     HIP_TRY(hipStreamWaitEvent(st, E->ev_ib[gs_b % Engine::RING], 0));
     hipLaunchKernelGGL(k_forces, dim3(b - a + 1, d.nc), dim3(256), 0, st, (const DevCase*)E->dcases,
                        (const CaseClock*)E->dclock, a);
@@ -626,6 +629,18 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     for (const lbmib_handle* h : E->members)
         if (!planes_ok(h->D, {X, Y, M}))
             return fail(LBMIB_E_STATE, "two-step pass: planes %d, %d, %d not all allocated", X, Y, M);
+    // the far field, both steps, enqueued first: it waits for nothing but
+    // the window + ring update of step n-1 before it in its stream
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
+    // one workgroup per far tile, or the persistent grid of the launch shape
+    int fgrid = E->nfar;
+    if (E->shape.persist > 0) fgrid = std::min(fgrid, std::max(8, E->shape.persist * E->ncu / 8 * 8));
+    const dim3 fg((unsigned)fgrid);
+    const size_t fsm = far2_smem_bytes(f64);
+    const int pf = X | (Y << 2) | (M << 4);  // k_far2's planes: src, dst, middle
+    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
+    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     // IB chain of step n on the solves' stream
     if (after_prev) HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[(gs - 1) % R], 0));
     if (wfirst) HIP_TRY(hipStreamWaitEvent(s1, wfirst, 0));
@@ -657,17 +672,6 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     // of step n-1 and the far pass before it, all before ev_step of step n-1)
     hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(E->nring), dim3(256), 0, s3, C, K, rtab, off, pa);
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
-    // the far field, both steps
-    if (ev) HIP_TRY(hipEventRecord(ev[0], s0));
-    // one workgroup per far tile, or the persistent grid of the launch shape
-    int fgrid = E->nfar;
-    if (E->shape.persist > 0) fgrid = std::min(fgrid, std::max(8, E->shape.persist * E->ncu / 8 * 8));
-    const dim3 fg((unsigned)fgrid);
-    const size_t fsm = far2_smem_bytes(f64);
-    const int pf = X | (Y << 2) | (M << 4);  // k_far2's planes: src, dst, middle
-    if (f64) hipLaunchKernelGGL(k_far2<double>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
-    else hipLaunchKernelGGL(k_far2<float>, fg, dim3(FAR_LANES), fsm, s0, C, K, E->dfar, E->nfar, off, pf);
-    if (ev) HIP_TRY(hipEventRecord(ev[1], s0));
     // IB chain of step n+1
     HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[gs % R], 0));
     if (int rc = launch_rhs(E, d, s1, off + 1, pb)) return rc;
@@ -802,7 +806,15 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         }
         if (rc) return rc;
     }
-    if (len > fupto) return enqueue_forces(E, d, fupto, len - 1, E->gstep - 1);
+    // the window's last forces: those of its last step alone, and before
+    // them those of the steps since the last boundary, which then start as
+    // soon as the step before the last has collided its IB nodes
+    if (len - 1 > fupto && !E->per_step)
+        if (int rc = enqueue_forces(E, d, fupto, len - 2, E->gstep - 2, true)) return rc;
+    if (len > fupto) {
+        const int a = E->per_step ? fupto : std::max(fupto, len - 1);
+        return enqueue_forces(E, d, a, len - 1, E->gstep - 1, true);
+    }
     return 0;
 }
 
