@@ -12,9 +12,10 @@ value = lattice updates per second of whole steps (MLUPS, nx*ny per step).
 roofline = the dominant kernel.  Steps are taken in two-step passes (the
 default launch shape): the far-field kernel k_far2 advances every node
 outside the lattice window that holds the body (and off the outer ring,
-which takes its steps in kernels of its own) by two updates per launch, so
-its algorithmic bytes are 2 x 144 per far-field node (9 FP64 populations read
-+ 9 written per update) over its mean launch time, timed with HIP events on
+which takes its steps in kernels of its own) by two updates per launch (the
+layer next to the ring by one: its second step goes with the ring's), so
+its algorithmic bytes are 144 per node update (9 FP64 populations read + 9
+written) times the updates of a launch over its mean launch time, timed with HIP events on
 the stream it runs on.  It reads and writes each population once per two
 updates, so this algorithmic rate can pass the HBM peak; `traffic` is the
 HBM bytes a launch moves (rocprofv3 FETCH_SIZE/WRITE_SIZE) and
@@ -84,7 +85,15 @@ PROFILE_EVERY = 17  # coprime with the 16-step geometry batch: samples every pha
 LATTICE_TRAFFIC_BYTES = {(4096, 2048): (311140 * 2 + 590815) * 1024}
 # the same for a k_far2<double> launch (two updates of the far field):
 # FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) of a pass that records nothing
-FAR_TRAFFIC_BYTES = {(4096, 2048): (440378 * 2 + 616294) * 1024}
+FAR_TRAFFIC_BYTES = {(4096, 2048): (436410 * 2 + 613215) * 1024}
+
+
+def far_updates(nx, ny, ww, wh):
+    """Node updates of one far-pass launch: step n at every node outside the
+    lattice window and off the outer ring, step n+1 at those nodes but the
+    layer next to the ring (which the window + ring kernel advances)."""
+    n_far = nx * ny - ww * wh - (2 * nx + 2 * ny - 4)
+    return 2 * n_far - (2 * (ny - 2) + 2 * (nx - 4))
 
 
 def cpu_baseline(case, seconds_budget=20.0, threads=None):
@@ -155,13 +164,12 @@ def run_batch(cs, prec, steps, tune, chunk=1000, label="", shape=None):
     dt = time.perf_counter() - t0
     tm = b.timings()
     hist = [s_.history() for s_ in sims]
-    # nodes of the far field (outside each case's lattice window, off its
-    # outer ring): what one far-pass launch advances two steps
-    tm["far_nodes"] = 0
+    # node updates of one far-pass launch over all cases (far_updates)
+    tm["far_updates"] = 0
     for c, s_ in zip(cs, sims):
         _, _, ww, wh = s_.lattice_window()
         p = c.params
-        tm["far_nodes"] += p.nx * p.ny - ww * wh - (2 * p.nx + 2 * p.ny - 4)
+        tm["far_updates"] += far_updates(p.nx, p.ny, ww, wh)
     b.close()
     for s_ in sims:
         s_.close()
@@ -219,7 +227,7 @@ def bench_batch(args):
     if tm["far_passes"] > 0:
         # two-step passes: the dominant kernel is the far pass over all cases
         kern = "k_far2<float> over all cases"
-        bytes_launch = 2 * tm["far_nodes"] * BYTES_PER_NODE_F32
+        bytes_launch = tm["far_updates"] * BYTES_PER_NODE_F32
         lat_s = tm["far_pass_ms"] * 1e-3
     else:
         kern = "k_lattice<float,%d> over all cases" % shape["nodes_per_thread"]
@@ -704,9 +712,11 @@ def main():
         g.set_shape(**parse_shape(args.shape))
     if W:
         g.evolution(W)
-    # phase events inside the timed region: on every step of a short run
-    # (up to 64 steps), else on every PROFILE_EVERY-th step
-    g.set_profiling(1 if K <= 64 else PROFILE_EVERY)
+    # phase events inside the timed region: on every step of a run shorter
+    # than 16 steps, every 4th step (every other two-step pass) up to 64
+    # steps, else every PROFILE_EVERY-th step.  Each sampled step costs the
+    # run about 1% of its time at 20 steps.
+    g.set_profiling(1 if K < 16 else 4 if K <= 64 else PROFILE_EVERY)
     t0 = time.perf_counter()
     g.evolution(K)  # synchronises the device before returning
     dt = time.perf_counter() - t0
@@ -716,11 +726,9 @@ def main():
     bpn = BYTES_PER_NODE_F32 if fp32 else BYTES_PER_NODE_F64
     wi0, wj0, ww, wh = g.lattice_window()
     if tm["far_passes"] > 0:
-        # two-step passes: the far-field kernel advances every node outside
-        # the lattice window and off the outer ring by two updates per launch
-        n_far = cells - ww * wh - (2 * p.nx + 2 * p.ny - 4)
+        # two-step passes: the far-field kernel's node updates per launch
         kern = "k_far2<%s>" % ("float" if fp32 else "double")
-        bytes_launch = 2 * n_far * bpn
+        bytes_launch = far_updates(p.nx, p.ny, ww, wh) * bpn
         lat_s = tm["far_pass_ms"] * 1e-3
         traffic = None if fp32 else FAR_TRAFFIC_BYTES.get((p.nx, p.ny))
     else:
