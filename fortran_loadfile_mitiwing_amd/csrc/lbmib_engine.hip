@@ -667,13 +667,18 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     if (ev) HIP_TRY(hipEventRecord(ev[5], s3));
     hipLaunchKernelGGL(f64 ? k_window<double> : k_window<float>, wg, dim3(256), 0, s3, C, K, wtab, off, pa);
     if (ev) HIP_TRY(hipEventRecord(ev[6], s3));
+    HIP_TRY(hipEventRecord(E->ev_win[gs % R], s3));  // the IB chain of step n+1 needs only W(n)
     // the ring's step n into the middle plane, from the ring's neighbourhood
     // of plane X (written by the window + ring update, or the lattice update,
     // of step n-1 and the far pass before it, all before ev_step of step n-1)
     hipLaunchKernelGGL(f64 ? k_ring<double> : k_ring<float>, dim3(E->nring), dim3(256), 0, s3, C, K, rtab, off, pa);
+    // step n is complete once its IB collide is too: the window + ring
+    // update after F then waits for one event (each wait is a packet that
+    // F's stream processes, even when long satisfied)
+    HIP_TRY(hipStreamWaitEvent(s3, E->ev_ib[gs % R], 0));
     HIP_TRY(hipEventRecord(E->ev_step[gs % R], s3));
     // IB chain of step n+1
-    HIP_TRY(hipStreamWaitEvent(s1, E->ev_step[gs % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s1, E->ev_win[gs % R], 0));
     if (int rc = launch_rhs(E, d, s1, off + 1, pb)) return rc;
     if (int rc = launch_solve(E, d, s1, off + 1)) return rc;
     hipLaunchKernelGGL(f64 ? k_band<double> : k_band<float>, dim3(band_x, d.nc), dim3(256), 0, s1, C, K, off + 1, pb);
@@ -681,8 +686,7 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
     // window step n+1 right after the far pass in its stream (no hand-over on
     // the path from one pass to the next); W(n) and the IB collide of step n
     // have long finished by then.  The ring's step n+1 rides along.
-    HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));
-    HIP_TRY(hipStreamWaitEvent(s0, E->ev_ib[gs % R], 0));
+    HIP_TRY(hipStreamWaitEvent(s0, E->ev_step[gs % R], 0));  // W(n), the ring's step n, the IB collide of step n
     hipLaunchKernelGGL(f64 ? k_window_ring<double> : k_window_ring<float>, dim3(E->nwr), dim3(256), 0, s0, C, K, wrtab,
                        off + 1, pb, make_psel(M, Y));
     HIP_TRY(hipEventRecord(E->ev_step[(gs + 1) % R], s0));
@@ -957,7 +961,8 @@ int engine_create(int dev, Engine** out)
     };
     bool ok = true;
     for (int i = 0; i < Engine::RING; ++i)
-        for (hipEvent_t* ev : {&E->ev_ib[i], &E->ev_step[i], &E->ev_rhs[i], &E->ev_solve[i]}) ok = ok && mk(ev) == hipSuccess;
+        for (hipEvent_t* ev : {&E->ev_ib[i], &E->ev_step[i], &E->ev_rhs[i], &E->ev_solve[i], &E->ev_win[i]})
+            ok = ok && mk(ev) == hipSuccess;
     for (int i = 0; i < Engine::BRING; ++i)
         for (hipEvent_t* ev : {&E->ev_first[i], &E->ev_next[i], &E->ev_nodes[i]}) ok = ok && mk(ev) == hipSuccess;
     ok = ok && mk(&E->ev_fork) == hipSuccess;
@@ -986,7 +991,7 @@ void engine_destroy(Engine* E)
     for (hipEvent_t e : E->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : E->gevpool) (void)hipEventDestroy(e);
     for (int i = 0; i < Engine::RING; ++i)
-        for (hipEvent_t e : {E->ev_ib[i], E->ev_step[i], E->ev_rhs[i], E->ev_solve[i]})
+        for (hipEvent_t e : {E->ev_ib[i], E->ev_step[i], E->ev_rhs[i], E->ev_solve[i], E->ev_win[i]})
             if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < Engine::BRING; ++i)
         for (hipEvent_t e : {E->ev_first[i], E->ev_next[i], E->ev_nodes[i]})

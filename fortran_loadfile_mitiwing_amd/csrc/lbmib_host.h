@@ -96,6 +96,7 @@ struct Engine {
     int streams_cus = -1;            // ib_cus the streams were made with
     static constexpr int RING = 64;  // per-step events of engine step q live in slot q % RING
     hipEvent_t ev_ib[RING] = {}, ev_step[RING] = {}, ev_rhs[RING] = {}, ev_solve[RING] = {};
+    hipEvent_t ev_win[RING] = {};  // two-step passes: the window's first step done
     static constexpr int BRING = 8;  // per-boundary events
     hipEvent_t ev_first[BRING] = {}, ev_next[BRING] = {}, ev_fork = nullptr, ev_join[NSTREAM] = {};
     hipEvent_t ev_nodes[BRING] = {};  // a fresh batch's node sets built (its factorisation may still run)
