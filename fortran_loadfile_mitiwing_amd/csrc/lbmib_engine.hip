@@ -708,10 +708,20 @@ int launch_pair(Engine* E, const Dims& d, int off, bool after_prev, hipEvent_t w
 // and the boundary's own work (the forces of the batch just finished, the
 // geometry of the batch after next into the finished batch's slots) follows
 // the pass, waiting for both of its steps.
-int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par, int pair_limit)
+//
+// A boundary fewer than GEO_DEFER steps before the window's end leaves the
+// next batch's geometry (about 1.3 ms of latency-bound kernels) to the
+// start of the next window (due: some member's next batch is not built),
+// so that the window does not end waiting for geometry its steps do not
+// use.  *next_off receives the offset of the last boundary that built the
+// next batch (-1: none).
+constexpr int GEO_DEFER = 12;
+int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool prof, int* par, int pair_limit,
+                  bool due = false, int* next_off = nullptr)
 {
     const int B = E->B, BR = Engine::BRING;
     int fupto = 0;
+    if (next_off) *next_off = -1;
     hipEvent_t prev_next = nullptr;
     const int every = prof ? E->prof_every : 0;
     const int Eper = Engine::EPER;
@@ -756,19 +766,28 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
             HIP_TRY(hipEventRecord(E->ev_first[bi % BR], st));
             wfirst = E->ev_first[bi % BR];
         }
-        if (!E->per_step) {
+        const bool defer = len - off < GEO_DEFER && !(off == 0 && (fresh || due));
+        if (!E->per_step && !defer) {
             if ((rc = launch_geometry(E, d, off, GEO_NEXT))) return rc;
             if ((rc = launch_factor(E, d, off, GEO_NEXT, gev))) return rc;
             HIP_TRY(hipEventRecord(E->ev_next[bi % BR], st));
             wnext = prev_next;
             prev_next = E->ev_next[bi % BR];
+            if (next_off) *next_off = off;
+        } else if (!E->per_step) {
+            wnext = prev_next;
+            prev_next = nullptr;
+            if (gev) {
+                HIP_TRY(hipEventRecord(gev[1], st));
+                HIP_TRY(hipEventRecord(gev[2], st));
+            }
         } else if (gev) {
             HIP_TRY(hipEventRecord(gev[1], st));
             HIP_TRY(hipEventRecord(gev[2], st));
         }
         return 0;
     };
-    auto is_boundary = [&](int off) { return (phase + off) % B == 0 || (fresh && off == 0); };
+    auto is_boundary = [&](int off) { return (phase + off) % B == 0 || ((fresh || due) && off == 0); };
     for (int off = 0; off < len; ++off) {
         hipEvent_t wfirst = nullptr, wnext = nullptr, wnodes = nullptr;
         if (is_boundary(off)) {
@@ -828,7 +847,8 @@ int sync_all(Engine* E)
     return 0;
 }
 
-int upload_clocks(Engine* E, const std::vector<int>& len_end, const std::vector<int>& final_end)
+int upload_clocks(Engine* E, const std::vector<int>& len_end, const std::vector<int>& final_end,
+                  const std::vector<int>& due)
 {
     const size_t nc = E->members.size();
     E->hclock.resize(nc);
@@ -839,6 +859,7 @@ int upload_clocks(Engine* E, const std::vector<int>& len_end, const std::vector<
         k.n_end = len_end[c];
         k.flag_from = final_end[c] - 1;
         k.fresh = h->geo_fresh ? 1 : 0;
+        k.next_due = due[c];
     }
     // asynchronous from pinned memory on s0; the other streams wait for it
     // (every engine call ends with its streams synchronised, so the pinned
@@ -1059,7 +1080,21 @@ int engine_run(Engine* E, int k)
         E->gev_used = 0;
     }
     int steps_total = 0, sampled = 0;
+    std::vector<int> due(nc), cur0(nc);
     for (;;) {
+        // geometry a window needs before its steps: a member whose current
+        // batch was never built (its boundary built nothing, last window)
+        // starts fresh; one whose next batch is missing builds it at offset 0
+        bool any_due = false;
+        for (size_t c = 0; c < nc; ++c) {
+            lbmib_handle* h = E->members[c];
+            due[c] = 0;
+            cur0[c] = h->steps_done / B;
+            if (E->per_step || h->geo_fresh || final_end[c] <= h->steps_done) continue;
+            if (h->geo_built < cur0[c]) h->geo_fresh = true;
+            else if (h->geo_built < cur0[c] + 1) due[c] = 1;
+            any_due = any_due || due[c];
+        }
         int kwin = 0;
         bool fresh = false;
         for (size_t c = 0; c < nc; ++c) {
@@ -1095,7 +1130,10 @@ int engine_run(Engine* E, int k)
             }
             if (normalize_planes(E, len_end, &plane0) && (rc = upload_cases(E))) return rc;
         }
-        if ((rc = upload_clocks(E, len_end, final_end))) return rc;
+        if ((rc = upload_clocks(E, len_end, final_end, due))) return rc;
+        std::vector<bool> was_fresh(nc);
+        for (size_t c = 0; c < nc; ++c) was_fresh[c] = E->members[c]->geo_fresh;
+        int next_off = -1;
         if (graph) {
             if (!E->graph_valid) {
                 // capture uses the clocks only through device pointers
@@ -1106,7 +1144,9 @@ int engine_run(Engine* E, int k)
         } else {
             if (every > 0) E->gev_used = 0;
             int par = plane0;
-            if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr, pair_limit))) return rc;
+            if ((rc = launch_window(E, d, len, phase, fresh, every > 0, fuse ? &par : nullptr, pair_limit, any_due,
+                                    &next_off)))
+                return rc;
             if ((rc = enqueue_error_gather(E, d))) return rc;
             if (fuse && par != plane0)
                 for (size_t c = 0; c < nc; ++c)
@@ -1145,6 +1185,15 @@ int engine_run(Engine* E, int k)
         }
         if ((rc = sync_all(E))) return rc;
         if ((rc = check_errors(E, d))) return rc;
+        // the batches whose geometry now exists: the first one of a fresh
+        // member, the one after each boundary that built it
+        for (size_t c = 0; c < nc; ++c) {
+            lbmib_handle* h = E->members[c];
+            if (len_end[c] <= h->steps_done || E->per_step) continue;
+            if (was_fresh[c]) h->geo_built = cur0[c];
+            if (next_off >= 0 && h->steps_done + 1 + next_off <= len_end[c])
+                h->geo_built = std::max(h->geo_built, (h->steps_done + next_off) / B + 1);
+        }
         advance_members(E, len_end, final_end);
         steps_total += len;
     }

@@ -42,6 +42,7 @@ struct lbmib_handle {
     double motion_end_time = 0.0;
     int steps_done = 0;
     bool geo_fresh = true;   // the next run builds the geometry of its first step's batch first
+    int geo_built = -1;      // (valid while !geo_fresh) highest geometry batch built or enqueued
     int last_flag_step[2] = {0, -1};  // step whose macros each slot holds
     // rigid-body dynamics: parameters, state (wing-indexed, as the
     // reference's body_rel_* / body_extra_* arrays), domain shift count,

@@ -134,7 +134,8 @@ struct DevCase {
 // at the same position inside a geometry batch.
 struct CaseClock {
     int n0, n_end, flag_from;
-    int fresh;  // 1: no geometry built for this case's steps yet (new run or restart)
+    int fresh;     // 1: no geometry built for this case's steps yet (new run or restart)
+    int next_due;  // 1: the batch after the one holding step n0 is not built yet: build it at offset 0
 };
 
 // Geometry launch modes: GEO_FIRST builds the batch of a fresh case's first

@@ -1005,12 +1005,15 @@ __device__ __forceinline__ bool geo_step(const DevCase& D, const CaseClock& k, i
 {
     const int n = k.n0 + off;
     // a fresh case (first step of a run or of a restart) builds at offset 0
-    // whatever its position inside the batch; the others at batch starts
-    const bool start = k.fresh && off == 0;
+    // whatever its position inside the batch, and so does a case whose next
+    // batch was left unbuilt at the end of its last window; the others at
+    // batch starts
+    const bool first = k.fresh && off == 0;
+    const bool start = (k.fresh || k.next_due) && off == 0;
     if (n > k.n_end || ((n - 1) % D.batch != 0 && !start)) return false;
     int j = (n - 1) / D.batch;
     if (mode == GEO_FIRST) {
-        if (!start) return false;
+        if (!first) return false;
     } else {
         j += 1;
     }
