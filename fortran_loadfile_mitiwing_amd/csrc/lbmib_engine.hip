@@ -408,16 +408,17 @@ int engine_prepare(Engine* E)
 }
 
 // Forces of window offsets a..b (one workgroup per step and case) on s2
-// (or, for the window's last steps, s3), after the IB-node collide of engine
+// (or, for the window's last steps, s1), after the IB-node collide of engine
 // step gs_b (offset b).
 int enqueue_forces(Engine* E, const Dims& d, int a, int b, long long gs_b, bool tail = false)
 {
     if (b < a) return 0;
-    // with per-step geometry the forces gate the next step: they go to the
-    // solves' stream, whose reserved CUs are idle once the IB nodes collide.
-    // The window's last forces go to the window stream, which is idle by
-    // then, and not behind the geometry built ahead on s2.
-    hipStream_t st = E->per_step ? E->s[1] : tail ? E->s[3] : E->s[2];
+    // with per-step geometry the forces gate the next step, and a window's
+    // last forces end it: they go to the solves' stream, whose reserved CUs
+    // are idle once the last IB nodes collide (on the shared CUs they would
+    // wait for the far pass to free room, and on s2 behind geometry built
+    // ahead)
+    hipStream_t st = E->per_step || tail ? E->s[1] : E->s[2];
     HIP_TRY(hipStreamWaitEvent(st, E->ev_ib[gs_b % Engine::RING], 0));
     hipLaunchKernelGGL(k_forces, dim3(b - a + 1, d.nc), dim3(256), 0, st, (const DevCase*)E->dcases,
                        (const CaseClock*)E->dclock, a);
@@ -829,15 +830,7 @@ int launch_window(Engine* E, const Dims& d, int len, int phase, bool fresh, bool
         }
         if (rc) return rc;
     }
-    // the window's last forces: those of its last step alone, and before
-    // them those of the steps since the last boundary, which then start as
-    // soon as the step before the last has collided its IB nodes
-    if (len - 1 > fupto && !E->per_step)
-        if (int rc = enqueue_forces(E, d, fupto, len - 2, E->gstep - 2, true)) return rc;
-    if (len > fupto) {
-        const int a = E->per_step ? fupto : std::max(fupto, len - 1);
-        return enqueue_forces(E, d, a, len - 1, E->gstep - 1, true);
-    }
+    if (len > fupto) return enqueue_forces(E, d, fupto, len - 1, E->gstep - 1, true);
     return 0;
 }
 
