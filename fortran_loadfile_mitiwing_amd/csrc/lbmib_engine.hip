@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <map>
 #include <mutex>
 
 namespace lbmib {
@@ -334,6 +335,21 @@ bool normalize_planes(Engine* E, const std::vector<int>& len_end, int* plane)
     return any;
 }
 
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize) only when kernel f on
+// device dev needs more than any earlier call set (the limit is a maximum,
+// so the largest serves every engine).
+int lds_limit(int dev, const void* f, size_t bytes)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> set;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = set[{dev, f}];
+    if (bytes <= have) return 0;
+    HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    have = bytes;
+    return 0;
+}
+
 int engine_prepare(Engine* E)
 {
     HIP_TRY(hipSetDevice(E->dev));
@@ -366,10 +382,6 @@ int engine_prepare(Engine* E)
         for (lbmib_handle* h : E->members)
             if ((rc = ensure_mid_plane(h))) return rc;
         if ((rc = build_pass_tables(E))) return rc;
-        HIP_TRY(hipFuncSetAttribute((const void*)k_far2<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)far2_smem_bytes(true)));
-        HIP_TRY(hipFuncSetAttribute((const void*)k_far2<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)far2_smem_bytes(false)));
     }
     if ((rc = upload_cases(E))) return rc;
     if (nc > E->pin_cap) {
@@ -392,17 +404,17 @@ int engine_prepare(Engine* E)
         E->tiles_valid = true;
         E->graph_valid = false;
     }
+    // the kernels' dynamic LDS limits: each is a maximum, kept per device and
+    // kernel at the largest any engine of the process has needed
     const Dims d = dims_of(E);
-    HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)factor_smem_bytes()));
+    if (int rc2 = lds_limit(E->dev, (const void*)k_far2<double>, far2_smem_bytes(true))) return rc2;
+    if (int rc2 = lds_limit(E->dev, (const void*)k_far2<float>, far2_smem_bytes(false))) return rc2;
+    if (int rc2 = lds_limit(E->dev, (const void*)k_geo_factor, factor_smem_bytes())) return rc2;
     for (bool cmp : {false, true})
-        HIP_TRY(hipFuncSetAttribute(solve_kernel(chain_unit(E, d), cmp), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)solve_smem_bytes(d.maxnpad)));
+        if (int rc2 = lds_limit(E->dev, solve_kernel(chain_unit(E, d), cmp), solve_smem_bytes(d.maxnpad))) return rc2;
     if (d.sep) {
-        HIP_TRY(hipFuncSetAttribute((const void*)k_geo_factor2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)factor_smem_bytes()));
-        HIP_TRY(hipFuncSetAttribute(solve2_kernel(sep_unit(E, d)), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)solve2_smem_bytes(d.maxnp2)));
+        if (int rc2 = lds_limit(E->dev, (const void*)k_geo_factor2, factor_smem_bytes())) return rc2;
+        if (int rc2 = lds_limit(E->dev, solve2_kernel(sep_unit(E, d)), solve2_smem_bytes(d.maxnp2))) return rc2;
     }
     return 0;
 }
