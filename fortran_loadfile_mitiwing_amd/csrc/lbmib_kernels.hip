@@ -1306,6 +1306,15 @@ __global__ __launch_bounds__(256) void k_geo_assemble(const DevCase* __restrict_
     S.G[t] = sum;
 }
 
+// Value of v in lane `lane` of the wavefront (wave-uniform result).
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)b, lane);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 constexpr int WIN = 64;  // LDS window slots of the banded factorisation (rows k..k+bw, bw <= WIN-2)
 constexpr int CH = 32;   // rows / columns staged per chunk
 constexpr int PAIRMAX = MAXBW * (MAXBW + 1) / 2;
@@ -1465,6 +1474,260 @@ __device__ __forceinline__ void band_factor(const GA& ga, int n, int bw, int ws,
     }
 }
 
+// Solve coefficients of step k (column k and row k final, both in the LDS
+// window win of band_factor_blocked) in band_factor's layouts; u is the
+// entry (0 .. 4*ws-1, or 0 .. 2*CW-1 compact).
+__device__ __forceinline__ void factor_coefs(const double* __restrict__ win, int k, int n, int bw, int ws, int np,
+                                             GAS double* __restrict__ coef, bool compact, int u)
+{
+    const int m = min(k + bw, n - 1) - k;
+    const int sk = (k % WIN) * WIN;
+    auto fwd = [&](int e) { return win[((k + e) % WIN) * WIN + e]; };  // l(k+e, k), 1 <= e <= m
+    if (compact) {
+        if (u < CW) {
+            const int e = u;
+            coef[(long long)k * CW + e] = (e >= 1 && e <= m) ? fwd(e) : 0.0;
+        } else if (u < 2 * CW) {
+            const int e = u - CW;
+            coef[(long long)(2 * np - 1 - k) * CW + e] = (e >= 1 && e <= min(bw, k)) ? win[sk + e] : 0.0;
+        }
+    } else {
+        const int q = k & (ws - 1);
+        const int hh = (u >> 1) & (ws - 1), j = u & 1;
+        if (u < 2 * ws) {
+            const int e = j == 0 ? hh - q : ws + hh - q;
+            const bool ok = (j == 0 ? hh > q : hh <= q) && e <= m;
+            coef[(long long)k * 2 * ws + u] = ok ? fwd(e) : 0.0;
+        } else if (u < 4 * ws) {
+            const int e = j == 0 ? q - hh : ws + q - hh;
+            const bool ok = (j == 0 ? hh < q : hh >= q) && e <= min(bw, k);
+            coef[(long long)(2 * np - 1 - k) * 2 * ws + u - 2 * ws] = ok ? win[sk + e] : 0.0;
+        }
+    }
+}
+
+// Blocked form of band_factor (no ltail; bw <= WIN - 2*FNB): the same
+// right-looking L D L^T with every entry receiving the same updates in
+// increasing k, in blocks of FNB columns k0 .. k0+FNB-1.
+//  * Panel (wavefront 0): lane a holds row k0+a (rows k0 .. k0+FNB-1+bw)
+//    with the block's columns in registers (register c = column k0+c) and
+//    factors the block column by column: the pivot and the unscaled entries
+//    of other rows come from their lanes by readlane, the updates among the
+//    block's own columns stay in registers.  It leaves the block's l and t
+//    in LDS (two buffers, by block parity), 1/d_k in dinv and, in the
+//    compact layout, the forward coefficients of the block's columns.
+//  * Trailing update of block b, split by column: T1 (the FNB columns of
+//    the next panel) by every wavefront right after the panel, T2 (the
+//    columns after those) by wavefronts 1-3 beside the next panel, together
+//    with block b's backward coefficients.  Each pair (a, c) (row
+//    k0+FNB+a >= column k0+FNB+c) applies the block's FNB updates in
+//    increasing k.  An entry of column j receives T2(b-1) before T1(b)
+//    (a barrier apart), and the panel of block b after T2(b-2) and T1(b-1).
+//  * Rows enter the window one group of FNB rows per block (wavefront 3):
+//    group g (rows g*FNB+bw .. g*FNB+FNB-1+bw, first read by block g) is
+//    loaded into registers during block g-3 and written to the window
+//    after the panel of block g-1, into slots of rows retired (backward
+//    coefficients written) before.
+// Each wavefront's memory counters hold one kind of traffic: the panel's
+// LDS waits never wait for global loads or stores, and the group loads are
+// unconditional, so wavefront 3 waits only for the groups it writes.  Two
+// barriers per block instead of two per column, and the panel's critical
+// chain (the pivot's reciprocal and the next pivot's update) shares its
+// phase only with T2 and the coefficients.
+constexpr int FNB = 8;
+constexpr int FGQ = (FNB * (WIN - 2 * FNB + 1) + 63) / 64;
+constexpr int FBW = WIN - 2 * FNB;                                   // largest bw of the blocked form
+constexpr int FPP1 = (FNB * FBW + 255) / 256;                        // T1 pairs per thread (256 threads)
+constexpr int FPP2 = ((FBW - FNB) * (FBW - FNB + 1) / 2 + 191) / 192;  // T2 pairs per thread (192 threads)
+template <typename GA>
+__device__ __forceinline__ void band_factor_blocked(const GA& ga, int n, int bw, int ws, int np,
+                                                    GAS double* __restrict__ coef, GAS double* __restrict__ dinv,
+                                                    int* err, bool compact)
+{
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int W = bw + 1;
+    double* win = smem;                 // [WIN][WIN]: row r in slot r % WIN, entry d = a(r, r-d)
+    double* lbuf = win + WIN * WIN;     // [2][l, t][FNB][WIN]: l(k0+a, k0+c) and t(k0+a, k0+c) at c*WIN + a
+    int* pairs1 = (int*)(lbuf + 4 * FNB * WIN);
+    int* pairs2 = pairs1 + FNB * FBW;
+    // T1 pairs (c < FNB) ordered by c then a, T2 pairs (c >= FNB) likewise
+    const int c1 = min(FNB, bw);
+    const int np1 = c1 * bw - c1 * (c1 - 1) / 2;
+    const int np2 = bw > FNB ? (bw - FNB) * (bw - FNB + 1) / 2 : 0;
+    for (int a = tid; a < bw; a += T)
+        for (int c = 0; c <= a; ++c) {
+            if (c < FNB) pairs1[c * bw - c * (c - 1) / 2 + a - c] = (a << 8) | c;
+            else pairs2[(c - FNB) * bw - (c * (c - 1) / 2 - FNB * (FNB - 1) / 2) + a - c] = (a << 8) | c;
+        }
+    for (int e = tid; e < min(FNB + bw, n) * W; e += T) {
+        const int r = e / W, d = e % W;
+        win[(r % WIN) * WIN + d] = ga(r, d);
+    }
+    const int lt = tid - 192;  // lane of wavefront 3
+    auto load_group = [&](int g, double (&p)[FGQ]) {
+#pragma unroll
+        for (int q = 0; q < FGQ; ++q) {
+            const int e = lt + q * 64;
+            const int r = g * FNB + bw + e / W, d = e % W;
+            const bool ok = e < FNB * W && r < n;
+            p[q] = ga(ok ? r : 0, ok ? d : 0);  // every lane loads (exact wait counts); store_group skips !ok
+        }
+    };
+    auto store_group = [&](int g, const double (&p)[FGQ]) {
+#pragma unroll
+        for (int q = 0; q < FGQ; ++q) {
+            const int e = lt + q * 64;
+            const int r = g * FNB + bw + e / W, d = e % W;
+            if (e < FNB * W && r < n) win[(r % WIN) * WIN + d] = p[q];
+        }
+    };
+    double pa[FGQ], pb[FGQ];
+    if (lt >= 0) {
+        load_group(1, pa);
+        load_group(2, pb);
+    }
+    __syncthreads();
+    int my1[FPP1], my2[FPP2];
+#pragma unroll
+    for (int r = 0; r < FPP1; ++r) {
+        const int t = tid + r * T;
+        my1[r] = t < np1 ? pairs1[t] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < FPP2; ++r) {
+        const int t = tid - 64 + r * 192;
+        my2[r] = (tid >= 64 && t < np2) ? pairs2[t] : 0;
+    }
+    const int ncoef = compact ? 2 * CW : 4 * ws;
+    // pair pr = (a, c) of the trailing update of the block at k0
+    auto trail = [&](int k0, int nb, const double* lc, int pr) {
+        const int a = pr >> 8, c = pr & 255;
+        const int i = k0 + FNB + a, j = k0 + FNB + c;
+        if (i >= n) return;
+        const double* tc = lc + FNB * WIN;
+        const int e = (i % WIN) * WIN + (i - j);
+        double v = win[e];
+#pragma unroll
+        for (int q = 0; q < FNB; ++q)
+            if (q < nb && i - k0 - q <= bw) v = v - lc[q * WIN + (i - k0)] * tc[q * WIN + (j - k0)];
+        win[e] = v;
+    };
+    // backward (and, in the ws layout, forward) coefficients of the block
+    // at k0, from the window; u0 .. ncoef-1 in steps of du
+    auto coefs = [&](int k0, int nb, int u0, int du) {
+        if (compact) {
+            // entry e of row k: l(k, k-e), 1 <= e <= min(bw, k)
+            for (int u = u0; u < FNB * CW; u += du) {
+                const int c = u / CW, e = u % CW, k = k0 + c;
+                if (c < nb)
+                    coef[(long long)(2 * np - 1 - k) * CW + e] =
+                        (e >= 1 && e <= min(bw, k)) ? win[(k % WIN) * WIN + e] : 0.0;
+            }
+        } else {
+            for (int c = 0; c < nb; ++c)
+                for (int u = u0; u < ncoef; u += du) factor_coefs(win, k0 + c, n, bw, ws, np, coef, compact, u);
+        }
+    };
+    auto block = [&](int b, double (&p)[FGQ]) {
+        const int k0 = b * FNB, nb = min(FNB, n - k0);
+        double* lc = lbuf + (b & 1) * 2 * FNB * WIN;
+        if (tid < 64) {
+            const int a = tid, i = k0 + a;
+            const bool live = a < FNB + bw && i < n;
+            const int si = (i % WIN) * WIN;
+            double P[FNB], Tt[FNB], R[FNB];
+#pragma unroll
+            for (int c = 0; c < FNB; ++c) {
+                const int dd = a - c;
+                P[c] = (live && dd >= 0 && dd <= bw) ? win[si + dd] : 0.0;
+                Tt[c] = 0.0;
+                R[c] = 0.0;
+            }
+            bool bad = false;
+            // column k0+c of the panel: selects, no branches, so that the
+            // updates of later columns fill the latency of the next pivot's
+            // reciprocal
+            auto column = [&](int c) {
+                const double d = readlane_f64(P[c], c);
+                const double r = 1.0 / d;
+                R[c] = r;
+                bad = bad || !(d > 0.0);
+                const bool below = live && a - c >= 1 && a - c <= bw;
+                const double t = P[c], l = t * r;
+                P[c] = below ? l : t;
+                Tt[c] = below ? t : 0.0;
+#pragma unroll
+                for (int c2 = c + 1; c2 < FNB; ++c2) {
+                    const double tj = readlane_f64(t, c2);
+                    const double u = P[c2] - l * tj;
+                    P[c2] = (below && a >= c2 && c2 - c <= bw) ? u : P[c2];
+                }
+            };
+            if (nb == FNB) {
+#pragma unroll
+                for (int c = 0; c < FNB; ++c) column(c);
+            } else {
+#pragma unroll
+                for (int c = 0; c < FNB; ++c)
+                    if (c < nb) column(c);
+            }
+            if (a < nb) {
+                double ra = R[0];
+#pragma unroll
+                for (int c = 1; c < FNB; ++c) ra = a == c ? R[c] : ra;
+                dinv[k0 + a] = ra;
+            }
+            if (bad && a == 0) atomicOr(err, ERR_SOLVE);
+#pragma unroll
+            for (int c = 0; c < FNB; ++c) {
+                const int dd = a - c;
+                if (live && dd >= 0 && dd <= bw) win[si + dd] = P[c];
+                lc[c * WIN + a] = P[c];
+                lc[FNB * WIN + c * WIN + a] = Tt[c];
+                // compact forward coefficients of column k0+c from the lanes
+                // holding it: entry e = a-c is l(k0+c+e, k0+c), 1 <= e <= m
+                if (compact && c < nb && dd >= 0 && dd < CW) {
+                    const int k = k0 + c, m = min(k + bw, n - 1) - k;
+                    coef[(long long)k * CW + dd] = (dd >= 1 && dd <= m) ? P[c] : 0.0;
+                }
+            }
+        } else if (b > 0) {
+            // beside the panel: T2 of the block before (a full block) and its
+            // coefficients
+            const int kp = k0 - FNB;
+            const double* lp = lbuf + ((b - 1) & 1) * 2 * FNB * WIN;
+#pragma unroll
+            for (int r = 0; r < FPP2; ++r) {
+                const int t = tid - 64 + r * 192;
+                if (t >= np2) break;
+                trail(kp, FNB, lp, my2[r]);
+            }
+            coefs(kp, FNB, tid - 64, 192);
+        }
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < FPP1; ++r) {
+            const int t = tid + r * T;
+            if (t >= np1) break;
+            trail(k0, nb, lc, my1[r]);
+        }
+        if (lt >= 0) {
+            // group b+1 is first read by the panel of block b+1
+            store_group(b + 1, p);
+            load_group(b + 3, p);
+        }
+        lds_barrier();
+    };
+    const int nblk = (n + FNB - 1) / FNB;
+    for (int b = 0; b < nblk; b += 2) {
+        block(b, pa);
+        if (b + 1 < nblk) block(b + 1, pb);
+    }
+    // the last block's coefficients (its T2 reaches no row below n)
+    if (nblk > 0) coefs((nblk - 1) * FNB, n - (nblk - 1) * FNB, tid, T);
+}
+
 // L D L^T of the whole zig-zag band G (cases solved as one chain).
 __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ C, const CaseClock* __restrict__ K,
                                                     int off, int mode)
@@ -1477,7 +1740,15 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
     const double* G = S.G;
     const int W = D.bw + 1;
     auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
-    band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err, D.compact != 0);
+    if (D.bw <= WIN - 2 * FNB) {
+        // global (not flat) accesses: flat ones would count in the LDS wait counter too
+        const GAS double* Gg = (const GAS double*)G;
+        auto gg = [Gg, W](int r, int d) { return Gg[(long long)r * W + d]; };
+        band_factor_blocked(gg, D.nseg, D.bw, D.ws, D.npad, (GAS double*)S.coef, (GAS double*)S.dinv, D.err,
+                            D.compact != 0);
+    } else {
+        band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err, D.compact != 0);
+    }
 }
 
 // Separator form of the solve.  With bw the band's lower bandwidth, rows
@@ -1610,14 +1881,6 @@ __global__ __launch_bounds__(256) void k_geo_schur(const DevCase* __restrict__ C
         const int h = e / (b * b), r = e % (b * b), s = r / b, t = r % b;
         Z[(long long)b * b3 + e] = dts[h][s] * Y[h][s][t];
     }
-}
-
-__device__ __forceinline__ double readlane_f64(double v, int lane)
-{
-    long long b = __double_as_longlong(v);
-    int lo = __builtin_amdgcn_readlane((int)b, lane);
-    int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // u* at the step's IB nodes: the moments of the nodes' post-stream
