@@ -141,3 +141,54 @@ def test_determinism(lbmib_mod):
         g.close()
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def geometric_bandwidth(case):
+    """The IB system's band bound as the library takes it (geometric_bandwidth
+    in lbmib_api.hip): the largest zig-zag distance between two segment
+    midpoints of one wing within 5*sqrt(2) lattice units."""
+    p = case.params
+    nw, ppw = p.n_wings, p.pts_per_wing
+    spw = ppw - 1
+    ns = nw * spw
+    xb, yb = np.asarray(case.xb0), np.asarray(case.yb0)
+    xm = np.array([0.5 * (xb[w * ppw + i + 1] + xb[w * ppw + i]) / p.scale for w in range(nw) for i in range(spw)])
+    ym = np.array([0.5 * (yb[w * ppw + i + 1] + yb[w * ppw + i]) / p.scale for w in range(nw) for i in range(spw)])
+    perm = []
+    for w in range(nw):
+        lo, hi, t = 0, spw - 1, 0
+        while lo <= hi:
+            if t % 2 == 0:
+                perm.append(w * spw + lo)
+                lo += 1
+            else:
+                perm.append(w * spw + hi)
+                hi -= 1
+            t += 1
+    iperm = np.empty(ns, int)
+    iperm[np.array(perm)] = np.arange(ns)
+    r2 = 50.0 * (1.0 + 1e-9) + 1e-9
+    bw = 0
+    for a in range(ns):
+        for b in range(a):
+            if a // spw == b // spw and (xm[a] - xm[b]) ** 2 + (ym[a] - ym[b]) ** 2 <= r2:
+                bw = max(bw, abs(int(iperm[a]) - int(iperm[b])))
+    return min(bw, ns - 1)
+
+
+@pytest.mark.parametrize("n_markers,lo,hi", [(140, 32, 48), (160, 32, 48)])
+def test_dense_markers_wide_bands(oracle_mod, lbmib_mod, n_markers, lo, hi):
+    """Bands of 32..48 (the blocked factorisation with the two-per-lane
+    coefficient layout), from denser markers on a 28-node chord, bit for bit
+    across three geometry batches.  (Bands above 48, which take the
+    one-column factorisation, need markers closer than about 0.3 lattice
+    units on this chord: 190 markers already make the system not positive
+    definite in the oracle's own arithmetic.)"""
+    case = cases.make_case(417, 203, 28.0, n_markers, 400.0, 40, heave=0.45, pitch_deg=20.0, phase_deg=80.0,
+                           strouhal=0.3, name="dense_%d" % n_markers)
+    bw = geometric_bandwidth(case)
+    print("  %s: band %d" % (case.name, bw), flush=True)
+    assert lo <= bw <= hi
+    o, g = run_pair(case, 40, oracle_mod, lbmib_mod, chunks=[17, 23])
+    assert_parity(o, g)
+    g.close()
