@@ -1110,7 +1110,7 @@ __global__ __launch_bounds__(1024) void k_geo_fill(const DevCase* __restrict__ C
     int step;
     if (!geo_step(D, K[blockIdx.y], off, mode, blockIdx.x, step)) return;
     const Slot S = slot_of(D, step % D.nslot);
-    __shared__ int part[1024];
+    __shared__ int part[16];  // wavefront totals (1024 threads)
     __shared__ int s_total;
     const int ns = D.nseg, T = blockDim.x, tid = threadIdx.x;
     const int chunk = (ns + T - 1) / T;
@@ -1127,16 +1127,20 @@ __global__ __launch_bounds__(1024) void k_geo_fill(const DevCase* __restrict__ C
 #pragma unroll
         for (int q = 0; q < BOXMAX; ++q) sum += cl[q] == s;
     }
-    part[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    // inclusive scan of the counts: within each wavefront by shuffles, then
+    // over the wavefront totals (one barrier instead of two per doubling)
+    const int lane = tid & 63, wv = tid >> 6;
+    int v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
     }
-    int base = part[tid] - sum;
-    if (tid == T - 1) s_total = part[T - 1];
+    if (lane == 63) part[wv] = v;
+    __syncthreads();
+    for (int q = 0; q < wv; ++q) v += part[q];
+    int base = v - sum;
+    if (tid == T - 1) s_total = v;
     for (int s = s0; s < s1; ++s) {
         const int* b = S.box + 4 * s;
         int w = s / D.spw;
