@@ -1510,7 +1510,7 @@ __device__ __forceinline__ void factor_coefs(const double* __restrict__ win, int
     }
 }
 
-// Blocked form of band_factor (no ltail; bw <= WIN - 2*FNB): the same
+// Blocked form of band_factor (bw <= WIN - 2*FNB): the same
 // right-looking L D L^T with every entry receiving the same updates in
 // increasing k, in blocks of FNB columns k0 .. k0+FNB-1.
 //  * Panel (wavefront 0): lane a holds row k0+a (rows k0 .. k0+FNB-1+bw)
@@ -1546,7 +1546,7 @@ constexpr int FPP2 = ((FBW - FNB) * (FBW - FNB + 1) / 2 + 191) / 192;  // T2 pai
 template <typename GA>
 __device__ __forceinline__ void band_factor_blocked(const GA& ga, int n, int bw, int ws, int np,
                                                     GAS double* __restrict__ coef, GAS double* __restrict__ dinv,
-                                                    int* err, bool compact)
+                                                    GAS double* __restrict__ ltail, int* err, bool compact)
 {
     extern __shared__ double smem[];
     const int tid = threadIdx.x, T = blockDim.x;
@@ -1632,6 +1632,15 @@ __device__ __forceinline__ void band_factor_blocked(const GA& ga, int n, int bw,
             for (int c = 0; c < nb; ++c)
                 for (int u = u0; u < ncoef; u += du) factor_coefs(win, k0 + c, n, bw, ws, np, coef, compact, u);
         }
+        // the final band (d_k, l(k,k-1), .., l(k,k-bw)) of the last bw rows
+        if (ltail && k0 + nb > n - bw)
+            for (int u = u0; u < FNB * W; u += du) {
+                const int c = u / W, d = u % W, k = k0 + c;
+                if (c < nb && k >= n - bw) {
+                    const int sk = (k % WIN) * WIN;
+                    ltail[(long long)(k - (n - bw)) * W + d] = d == 0 ? win[sk] : (d <= k ? win[sk + d] : 0.0);
+                }
+            }
     };
     auto block = [&](int b, double (&p)[FGQ]) {
         const int k0 = b * FNB, nb = min(FNB, n - k0);
@@ -1748,8 +1757,8 @@ __global__ __launch_bounds__(256) void k_geo_factor(const DevCase* __restrict__ 
         // global (not flat) accesses: flat ones would count in the LDS wait counter too
         const GAS double* Gg = (const GAS double*)G;
         auto gg = [Gg, W](int r, int d) { return Gg[(long long)r * W + d]; };
-        band_factor_blocked(gg, D.nseg, D.bw, D.ws, D.npad, (GAS double*)S.coef, (GAS double*)S.dinv, D.err,
-                            D.compact != 0);
+        band_factor_blocked(gg, D.nseg, D.bw, D.ws, D.npad, (GAS double*)S.coef, (GAS double*)S.dinv, nullptr,
+                            D.err, D.compact != 0);
     } else {
         band_factor(ga, D.nseg, D.bw, D.ws, D.npad, S.coef, S.dinv, nullptr, D.err, D.compact != 0);
     }
@@ -1774,13 +1783,20 @@ __global__ __launch_bounds__(256) void k_geo_factor2(const DevCase* __restrict__
     double* coef = S.coef2 + (long long)half * (2 * D.np2 + 256) * CW;
     double* dinv = S.dinv2 + (long long)half * D.np2;
     double* lt = S.ltail2 + (long long)half * D.bw * W;
+    // the blocked form (global, not flat, accesses) where the band allows it
+    const GAS double* Gg = (const GAS double*)G;
+    const bool blk = D.bw <= FBW;
     if (half == 0) {
-        auto ga = [G, W](int r, int d) { return G[(long long)r * W + d]; };
-        band_factor(ga, D.mT, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
+        auto ga = [Gg, W](int r, int d) { return Gg[(long long)r * W + d]; };
+        if (blk) band_factor_blocked(ga, D.mT, D.bw, D.ws, D.np2, (GAS double*)coef, (GAS double*)dinv,
+                                     (GAS double*)lt, D.err, true);
+        else band_factor(ga, D.mT, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
     } else {
         // reversed row r is row ns-1-r; its entry a'(r, r-d) = a(ns-1-r+d, ns-1-r)
-        auto ga = [G, W, ns](int r, int d) { return d <= r ? G[(long long)(ns - 1 - r + d) * W + d] : 0.0; };
-        band_factor(ga, D.mB, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
+        auto ga = [Gg, W, ns](int r, int d) { return d <= r ? Gg[(long long)(ns - 1 - r + d) * W + d] : 0.0; };
+        if (blk) band_factor_blocked(ga, D.mB, D.bw, D.ws, D.np2, (GAS double*)coef, (GAS double*)dinv,
+                                     (GAS double*)lt, D.err, true);
+        else band_factor(ga, D.mB, D.bw, D.ws, D.np2, coef, dinv, lt, D.err, true);
     }
 }
 
